@@ -1,0 +1,957 @@
+// oic.hip -- liboic_hip.so: the C ABI of include/oic.h over the gfx950 kernels.
+//
+// Host orchestration mirrors overlapInCore's driver (src/overlapInCore/overlapInCore.C:161-277):
+// a hash-block loop around Build_Hash_Index, then the ref range searched in both orientations.
+// Ref reads are processed in batches sized to bound device memory; every stage of a batch runs
+// on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/oic.h"
+#include "oic_tables.h"
+#include "oic_sort.h"
+#include "oic_search.h"
+#include "oic_sched.h"
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t b) {
+    if (b <= bytes && p) return hipSuccess;
+    release();
+    size_t nb = b < 256 ? 256 : b;
+    hipError_t e = hipMalloc(&p, nb);
+    if (e != hipSuccess) { p = nullptr; bytes = 0; (void)hipGetLastError(); return e; }
+    bytes = nb;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T *as() const { return (T *)p; }
+  ~DevBuf() { release(); }
+};
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+int ceil_log2(uint64_t x) {
+  int b = 0;
+  while ((1ull << b) < x) b++;
+  return b;
+}
+
+constexpr int kCapSmall = 512;
+constexpr int kCapLarge = 8192;
+
+}  // namespace
+
+struct oic_ctx {
+  oic_params P{};
+  oic::Tables T;
+  std::string err;
+  hipStream_t st = nullptr;
+  int ncu = 0;
+
+  DevBuf d_eb, d_ml, d_minkmers;
+  // reads
+  uint32_t nreads = 0;
+  int32_t maxlen = 0;
+  std::vector<int32_t> h_len;
+  DevBuf d_words, d_nmask, d_b0f, d_b0r, d_len, d_hasN;
+  // index of the current hash block
+  bool have_index = false;
+  uint32_t hbgn = 0, hend = 0, nblock = 0;
+  uint64_t nu = 0, nvalid = 0, npos = 0;
+  int dshift = 0;
+  DevBuf x_ukey, x_ustart, x_dir, x_occ;
+  // sort / scan temporaries
+  DevBuf s_k1, s_v1, s_k2, s_v2, s_hist, s_offs, s_scan, s_flag, s_idx, s_small;
+  // search
+  DevBuf g_cnt, g_off, g_pbase, g_searched;
+  DevBuf p_pairs, p_tmp, p_nodes, p_recs, p_pout, p_order, p_queue, p_stats, p_nout, p_noff, p_outrec, p_retry;
+  DevBuf sc_rowL, sc_rowR, sc_rowOff, sc_rowVal, sc_dbuf, sc_ld, sc_rd;
+  int32_t sc_rows = 0;
+  int64_t sc_val = 0;
+  uint32_t sc_waves = 0;
+  oic_counters ctr{};
+  // last batch's pair table (host copies) for the scheduler / trace export
+  std::vector<PairFeat> last_feat;
+  std::vector<PairOut> last_pout;
+  bool keep_trace = false;
+  oic_sched_model model{};
+  bool have_model = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+static int fail(oic_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(c, x)                                                                      \
+  do {                                                                                    \
+    hipError_t _e = (x);                                                                  \
+    if (_e != hipSuccess) return fail((c), OIC_EHIP, "%s: %s (%s:%d)", #x, hipGetErrorString(_e), __FILE__, __LINE__); \
+  } while (0)
+#define ALLOC(c, buf, bytes)                                                              \
+  do {                                                                                    \
+    if ((buf).ensure(bytes) != hipSuccess)                                                \
+      return fail((c), OIC_ENOMEM, "device allocation of %zu bytes failed (%s:%d)", (size_t)(bytes), __FILE__, __LINE__); \
+  } while (0)
+
+static DPTables dev_tables(oic_ctx *c) {
+  DPTables tb;
+  tb.error_bound = c->d_eb.as<int32_t>();
+  tb.match_limit = c->d_ml.as<int32_t>();
+  tb.bmv = c->T.bmv;
+  tb.slope_min = c->T.slope_min;
+  tb.min_branch_end_dist = 20;
+  tb.partial = c->P.partial;
+  return tb;
+}
+
+static ReadsDev dev_reads(oic_ctx *c) {
+  ReadsDev R;
+  R.words = c->d_words.as<uint32_t>();
+  R.nmask = c->d_nmask.as<uint32_t>();
+  R.b0f = c->d_b0f.as<int64_t>();
+  R.b0r = c->d_b0r.as<int64_t>();
+  R.len = c->d_len.as<int32_t>();
+  R.hasN = c->d_hasN.as<uint8_t>();
+  return R;
+}
+
+extern "C" void oic_params_default(oic_params *p) {
+  memset(p, 0, sizeof(*p));
+  p->kmer = 22;
+  p->max_erate = 0.06;
+  p->align_noise = 1.0;
+  p->min_olap = 0;
+  p->unique = 1;
+  p->hopeless = 1;
+  p->frag_olap_limit = UINT64_MAX;
+  p->hash_bits = 22;
+  p->hash_load = 0.6;
+  p->hash_data_len = 100000000;
+  p->n_threads = 1;
+  p->device = 0;
+  p->schedule = OIC_SCHED_PLAIN;
+}
+
+extern "C" const char *oic_last_error(const oic_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int oic_create(const oic_params *p, oic_ctx **out) {
+  if (!p || !out) return OIC_EINVAL;
+  *out = nullptr;
+  oic_ctx *c = new oic_ctx();
+  c->P = *p;
+  if (p->kmer < 1 || p->kmer > 31) { int r = fail(c, OIC_EINVAL, "kmer %u out of range 1..31", p->kmer); delete c; return r; }
+  if (!(p->max_erate > 0.0) || p->max_erate > 0.5) { delete c; return OIC_EINVAL; }
+  if (p->frag_olap_limit != UINT64_MAX) { delete c; return OIC_EUNSUPPORTED; }
+  if (p->n_threads == 0) c->P.n_threads = 1;
+  if (c->P.max_erate > 0.06) {}
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    delete c;
+    return OIC_EHIP;
+  }
+  if (hipSetDevice(p->device) != hipSuccess) { delete c; return OIC_EHIP; }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, p->device) != hipSuccess) { delete c; return OIC_EHIP; }
+  c->ncu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; return OIC_EHIP; }
+  hipEventCreate(&c->ev0);
+  hipEventCreate(&c->ev1);
+  c->T.build(p->max_erate, p->align_noise);
+  // overlapInCore.C:400-401
+  if (c->T.maxErate > 0.06) c->P.hopeless = 0;
+  int rc = OIC_OK;
+  if (c->d_eb.ensure(c->T.error_bound.size() * 4) != hipSuccess || c->d_ml.ensure(c->T.match_limit.size() * 4) != hipSuccess)
+    rc = OIC_ENOMEM;
+  if (rc == OIC_OK) {
+    hipMemcpy(c->d_eb.p, c->T.error_bound.data(), c->T.error_bound.size() * 4, hipMemcpyHostToDevice);
+    hipMemcpy(c->d_ml.p, c->T.match_limit.data(), c->T.match_limit.size() * 4, hipMemcpyHostToDevice);
+  }
+  if (rc == OIC_OK && c->P.minkmers) {
+    // overlapInCore.C:410-411 and Process_String_Overlaps.C:22-33, in the reference's types
+    const uint64_t K = c->P.kmer;
+    const uint64_t span = (uint64_t)(int64_t)c->P.min_olap - K + 1;
+    const uint64_t filt = (uint64_t)(int)(std::floor(std::exp(-1.0 * (double)K * c->T.maxErate) * (double)span));
+    std::vector<uint32_t> tab(oic::kMaxReadLen + 1);
+    for (uint32_t o = 0; o <= oic::kMaxReadLen; o++) {
+      const double ovl = (double)o;
+      uint64_t ex = 0;
+      if (!(ovl < (double)K)) ex = (uint64_t)(int)(std::floor(std::exp(-1.0 * (double)K * c->T.maxErate) * (ovl - (double)K + 1)));
+      const uint64_t v = filt ? std::max(filt, ex) : 0;
+      tab[o] = (uint32_t)std::min<uint64_t>(v, 0xffffffffu);
+    }
+    if (filt == 0) c->P.minkmers = 0;
+    else if (c->d_minkmers.ensure(tab.size() * 4) != hipSuccess) rc = OIC_ENOMEM;
+    else hipMemcpy(c->d_minkmers.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice);
+  }
+  if (rc != OIC_OK) { delete c; return rc; }
+  *out = c;
+  return OIC_OK;
+}
+
+extern "C" void oic_destroy(oic_ctx *c) {
+  if (!c) return;
+  if (c->st) hipStreamSynchronize(c->st);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+}
+
+extern "C" int oic_tables(const oic_ctx *c, const int32_t **eb, const int32_t **ml, uint32_t *max_errors) {
+  if (!c) return OIC_EINVAL;
+  if (eb) *eb = c->T.error_bound.data();
+  if (ml) *ml = c->T.match_limit.data();
+  if (max_errors) *max_errors = c->T.max_errors;
+  return OIC_OK;
+}
+
+extern "C" int oic_get_counters(const oic_ctx *c, oic_counters *out) {
+  if (!c || !out) return OIC_EINVAL;
+  *out = c->ctr;
+  return OIC_OK;
+}
+
+// ------------------------------------------------------------------------------------ reads
+
+extern "C" int oic_load_reads(oic_ctx *c, const oic_reads *rd) {
+  if (!c || !rd || !rd->bases || !rd->start || !rd->len) return OIC_EINVAL;
+  const uint32_t n = rd->nreads;
+  c->nreads = n;
+  c->have_index = false;
+  c->h_len.assign((size_t)n + 1, 0);
+  std::vector<int64_t> b0f((size_t)n + 1, 0), b0r((size_t)n + 1, 0);
+  std::vector<uint64_t> start((size_t)n + 1, 0);
+  uint64_t lo = UINT64_MAX, hi = 0;
+  int64_t acc = 0;
+  c->maxlen = 0;
+  for (uint32_t r = 1; r <= n; r++) {
+    const uint32_t L = rd->len[r];
+    if (L > oic::kMaxReadLen) return fail(c, OIC_EINVAL, "read %u has length %u > 2^21-1", r, L);
+    c->h_len[r] = (int32_t)L;
+    c->maxlen = std::max<int32_t>(c->maxlen, (int32_t)L);
+    const int64_t span = ((int64_t)(L + 31) / 32 + 2) * 32;
+    b0f[r] = acc; acc += span;
+    b0r[r] = acc; acc += span;
+    lo = std::min<uint64_t>(lo, rd->start[r]);
+    hi = std::max<uint64_t>(hi, rd->start[r] + L);
+  }
+  if (n == 0) { lo = hi = 0; }
+  for (uint32_t r = 1; r <= n; r++) start[r] = rd->start[r] - lo;
+  const uint64_t nwords = (uint64_t)acc / 16 + 4, nmw = (uint64_t)acc / 32 + 4;
+  ALLOC(c, c->d_words, nwords * 4);
+  ALLOC(c, c->d_nmask, nmw * 4);
+  ALLOC(c, c->d_b0f, ((size_t)n + 1) * 8);
+  ALLOC(c, c->d_b0r, ((size_t)n + 1) * 8);
+  ALLOC(c, c->d_len, ((size_t)n + 1) * 4);
+  ALLOC(c, c->d_hasN, (size_t)n + 1);
+  DevBuf raw, dstart;
+  ALLOC(c, raw, (hi - lo) + 1);
+  ALLOC(c, dstart, ((size_t)n + 1) * 8);
+  HIPCHK(c, hipMemsetAsync(c->d_words.p, 0, nwords * 4, c->st));
+  HIPCHK(c, hipMemsetAsync(c->d_nmask.p, 0, nmw * 4, c->st));
+  HIPCHK(c, hipMemsetAsync(c->d_hasN.p, 0, (size_t)n + 1, c->st));
+  HIPCHK(c, hipMemcpyAsync(raw.p, rd->bases + lo, hi - lo, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(dstart.p, start.data(), ((size_t)n + 1) * 8, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->d_b0f.p, b0f.data(), ((size_t)n + 1) * 8, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->d_b0r.p, b0r.data(), ((size_t)n + 1) * 8, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->d_len.p, c->h_len.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice, c->st));
+  if (n) {
+    hipLaunchKernelGGL(k_pack, dim3(n), dim3(256), 0, c->st, raw.as<uint8_t>(), dstart.as<uint64_t>(), c->d_len.as<int32_t>(),
+                       c->d_b0f.as<int64_t>(), c->d_b0r.as<int64_t>(), c->d_words.as<uint32_t>(), c->d_nmask.as<uint32_t>(),
+                       c->d_hasN.as<uint8_t>(), n);
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return OIC_OK;
+}
+
+// ------------------------------------------------------------------------------------ index
+
+static int ensure_sort_temp(oic_ctx *c, uint64_t n) {
+  const uint64_t nb = radix_blocks(n) + 1;
+  ALLOC(c, c->s_k2, n * 8);
+  ALLOC(c, c->s_v2, n * 8);
+  ALLOC(c, c->s_hist, 256 * nb * 4);
+  ALLOC(c, c->s_offs, 256 * nb * 8);
+  ALLOC(c, c->s_scan, scan_scratch_words(std::max<uint64_t>(n, 256 * nb)) * 8);
+  return OIC_OK;
+}
+
+// builds the k-mer table over hash reads [bgn, end]; fills first_t (distinct k-mers first seen per read)
+static int build_table(oic_ctx *c, uint32_t bgn, uint32_t end, std::vector<uint32_t> *first_t) {
+  const int K = (int)c->P.kmer;
+  const uint32_t nb = end - bgn + 1;
+  std::vector<uint8_t> loaded(nb);
+  std::vector<uint64_t> posbase(nb + 1);
+  uint64_t N = 0;
+  for (uint32_t t = 0; t < nb; t++) {
+    const int32_t L = c->h_len[bgn + t];
+    loaded[t] = (L >= c->P.min_olap && L >= K) ? 1 : 0;
+    posbase[t] = N;
+    if (loaded[t]) N += (uint64_t)(L - K + 1);
+  }
+  posbase[nb] = N;
+  c->npos = N;
+  DevBuf d_loaded, d_posbase, d_first;
+  ALLOC(c, d_loaded, nb);
+  ALLOC(c, d_posbase, (nb + 1) * 8);
+  ALLOC(c, d_first, nb * 4);
+  HIPCHK(c, hipMemcpyAsync(d_loaded.p, loaded.data(), nb, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(d_posbase.p, posbase.data(), (nb + 1) * 8, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemsetAsync(d_first.p, 0, nb * 4, c->st));
+  ALLOC(c, c->s_k1, std::max<uint64_t>(N, 1) * 8);
+  ALLOC(c, c->x_occ, std::max<uint64_t>(N, 1) * 8);
+  if (ensure_sort_temp(c, std::max<uint64_t>(N, 1)) != OIC_OK) return OIC_ENOMEM;
+  const uint64_t invalid = 1ull << (2 * K);
+  ReadsDev R = dev_reads(c);
+  uint64_t *keys = c->s_k1.as<uint64_t>(), *vals = c->x_occ.as<uint64_t>();
+  if (N) {
+    hipLaunchKernelGGL(k_kmer_emit, dim3(nb), dim3(256), 0, c->st, R, bgn, nb, d_posbase.as<uint64_t>(), d_loaded.as<uint8_t>(), K,
+                       invalid, keys, vals);
+    RadixTemp tmp{c->s_k2.as<uint64_t>(), c->s_v2.as<uint64_t>(), c->s_hist.as<uint32_t>(), c->s_offs.as<uint64_t>(), c->s_scan.as<uint64_t>()};
+    const int nbits = ((2 * K + 1) + 7) / 8 * 8;
+    HIPCHK(c, radix_sort(&keys, &vals, N, nbits, tmp, c->st));
+    // the sorted arrays may have moved into the temp buffers: swap ownership so x_occ holds vals
+    if (keys != c->s_k1.as<uint64_t>()) std::swap(c->s_k1, c->s_k2);
+    if (vals != c->x_occ.as<uint64_t>()) std::swap(c->x_occ, c->s_v2);
+    keys = c->s_k1.as<uint64_t>();
+    vals = c->x_occ.as<uint64_t>();
+  }
+  ALLOC(c, c->s_flag, std::max<uint64_t>(N, 1) * 4);
+  ALLOC(c, c->s_idx, std::max<uint64_t>(N, 1) * 8);
+  uint64_t nu = 0;
+  if (N) {
+    const unsigned g = (unsigned)((N + 255) / 256);
+    hipLaunchKernelGGL(k_unique_flags, dim3(g), dim3(256), 0, c->st, keys, N, invalid, c->s_flag.as<uint32_t>());
+    HIPCHK(c, scan_excl<uint32_t>(c->s_flag.as<uint32_t>(), c->s_idx.as<uint64_t>(), N, c->s_scan.as<uint64_t>(), c->st, &nu));
+  }
+  ALLOC(c, c->x_ukey, std::max<uint64_t>(nu, 1) * 8);
+  ALLOC(c, c->x_ustart, (nu + 1) * 8);
+  // number of valid positions: invalid keys sort last
+  uint64_t nvalid = N;
+  if (N) {
+    std::vector<uint64_t> tail(1);
+    // binary search on host over device keys would need many copies; count invalid windows instead
+    uint64_t ninv = 0;
+    for (uint32_t t = 0; t < nb; t++) (void)t;
+    // invalid windows only exist in reads with N; find the first invalid key by bisection on device data
+    uint64_t lo = 0, hi = N;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) / 2;
+      uint64_t kv = 0;
+      HIPCHK(c, hipMemcpyAsync(&kv, keys + mid, 8, hipMemcpyDeviceToHost, c->st));
+      HIPCHK(c, hipStreamSynchronize(c->st));
+      if (kv < invalid) lo = mid + 1; else hi = mid;
+    }
+    nvalid = lo;
+    (void)ninv;
+  }
+  if (nu) {
+    const unsigned g = (unsigned)((N + 255) / 256);
+    hipLaunchKernelGGL(k_unique_write, dim3(g), dim3(256), 0, c->st, keys, vals, N, c->s_flag.as<uint32_t>(), c->s_idx.as<uint64_t>(),
+                       c->x_ukey.as<uint64_t>(), c->x_ustart.as<uint64_t>(), d_first.as<uint32_t>());
+  }
+  HIPCHK(c, hipMemcpyAsync(c->x_ustart.as<uint64_t>() + nu, &nvalid, 8, hipMemcpyHostToDevice, c->st));
+  int dbits = std::max(8, std::min(ceil_log2(std::max<uint64_t>(nu, 1)), std::min(2 * K, 28)));
+  if (dbits > 2 * K) dbits = 2 * K;
+  c->dshift = 2 * K - dbits;
+  const uint64_t nbk = 1ull << dbits;
+  ALLOC(c, c->x_dir, (nbk + 2) * 8);
+  {
+    const unsigned g = (unsigned)((nu + 1 + 255) / 256);
+    hipLaunchKernelGGL(k_build_dir, dim3(g), dim3(256), 0, c->st, c->x_ukey.as<uint64_t>(), nu, c->dshift, nbk, c->x_dir.as<uint64_t>());
+  }
+  HIPCHK(c, hipGetLastError());
+  if (first_t) {
+    first_t->resize(nb);
+    HIPCHK(c, hipMemcpyAsync(first_t->data(), d_first.p, nb * 4, hipMemcpyDeviceToHost, c->st));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->nu = nu;
+  c->nvalid = nvalid;
+  c->hbgn = bgn;
+  c->hend = end;
+  c->nblock = nb;
+  c->have_index = true;
+  return OIC_OK;
+}
+
+extern "C" int oic_index_build(oic_ctx *c, uint32_t bgn, uint32_t end, uint32_t *last_loaded) {
+  if (!c) return OIC_EINVAL;
+  if (c->nreads == 0) return fail(c, OIC_ESTATE, "no reads loaded");
+  if (bgn < 1) bgn = 1;
+  if (end > c->nreads) end = c->nreads;
+  if (bgn > end) return fail(c, OIC_EINVAL, "empty hash range %u-%u", bgn, end);
+  const double t0 = now_ms();
+  // Build_Hash_Index.C:504-506: a read is loaded while total_len < Max_Hash_Data_Len and
+  // Hash_Entries (distinct k-mers so far) < Max_Hash_Load * 2^hashbits * 21.
+  const uint64_t maxData = c->P.hash_data_len + oic::kMaxReadLen;
+  const uint64_t limit = (uint64_t)(c->P.hash_load * (double)(1ull << c->P.hash_bits) * 21.0);
+  uint64_t total = 0;
+  uint32_t cand = bgn - 1;
+  for (uint32_t r = bgn; r <= end; r++) {
+    if (total >= maxData) break;
+    cand = r;
+    if (c->h_len[r] >= c->P.min_olap) total += (uint64_t)c->h_len[r] + 1;
+  }
+  std::vector<uint32_t> first;
+  int rc = build_table(c, bgn, cand, &first);
+  if (rc) return rc;
+  uint64_t distinct = 0;
+  uint32_t blk = cand;
+  for (uint32_t r = bgn; r <= cand; r++) {
+    if (distinct >= limit) { blk = r - 1; break; }
+    distinct += first[r - bgn];
+  }
+  if (blk < cand) {
+    rc = build_table(c, bgn, blk, nullptr);
+    if (rc) return rc;
+  }
+  if (last_loaded) *last_loaded = blk;
+  c->ctr.ms_index += now_ms() - t0;
+  return OIC_OK;
+}
+
+// ------------------------------------------------------------------------------------ search
+
+static int ensure_wave_scratch(oic_ctx *c, uint32_t waves, int cap, int64_t val_cap) {
+  const int32_t rows = c->T.error_bound[std::min<int32_t>(c->maxlen, (int32_t)oic::kMaxReadLen)] + 4;
+  ALLOC(c, c->sc_rowL, (size_t)waves * rows * 4);
+  ALLOC(c, c->sc_rowR, (size_t)waves * rows * 4);
+  ALLOC(c, c->sc_rowOff, (size_t)waves * rows * 8);
+  ALLOC(c, c->sc_rowVal, (size_t)waves * val_cap * 4);
+  ALLOC(c, c->sc_dbuf, (size_t)waves * (rows + 4) * 4);
+  ALLOC(c, c->sc_ld, (size_t)waves * (rows + 4) * 4);
+  ALLOC(c, c->sc_rd, (size_t)waves * (rows + 4) * 4);
+  c->sc_rows = rows;
+  c->sc_val = val_cap;
+  c->sc_waves = waves;
+  (void)cap;
+  return OIC_OK;
+}
+
+static DPScratch scratch0(oic_ctx *c) {
+  DPScratch s;
+  s.rowL = c->sc_rowL.as<int32_t>();
+  s.rowR = c->sc_rowR.as<int32_t>();
+  s.rowOff = c->sc_rowOff.as<int64_t>();
+  s.rowVal = c->sc_rowVal.as<int32_t>();
+  s.val_cap = c->sc_val;
+  s.max_rows = c->sc_rows - 2;
+  s.dbuf = c->sc_dbuf.as<int32_t>();
+  s.ldelta = c->sc_ld.as<int32_t>();
+  s.rdelta = c->sc_rd.as<int32_t>();
+  return s;
+}
+
+static SearchConst search_const(oic_ctx *c, bool elide) {
+  SearchConst C;
+  C.tb = dev_tables(c);
+  C.min_olap = c->P.min_olap;
+  C.unique = c->P.unique;
+  C.partial = c->P.partial;
+  C.hopeless = c->P.hopeless;
+  C.K = (int32_t)c->P.kmer;
+  C.min_kmers = c->P.minkmers ? c->d_minkmers.as<uint32_t>() : nullptr;
+  C.elide = elide ? 1 : 0;
+  return C;
+}
+
+// runs k_process_pairs over order[0..n) with the given tile; returns the number of overflowed pairs
+template <int CAP>
+static int run_pairs(oic_ctx *c, PairKernelArgs A, uint64_t n, uint32_t waves, int64_t val_cap, double *ms) {
+  if (n == 0) return OIC_OK;
+  waves = (uint32_t)std::min<uint64_t>(waves, n);
+  int rc = ensure_wave_scratch(c, waves, CAP, val_cap);
+  if (rc) return rc;
+  A.sc0 = scratch0(c);
+  A.row_stride = c->sc_rows;
+  A.val_stride = c->sc_val;
+  A.delta_stride = c->sc_rows + 4;
+  A.norder = n;
+  HIPCHK(c, hipMemsetAsync(A.queue, 0, 4, c->st));
+  HIPCHK(c, hipEventRecord(c->ev0, c->st));
+  hipLaunchKernelGGL(k_process_pairs<CAP>, dim3(waves), dim3(OIC_WAVE), 0, c->st, A);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev1, c->st));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float f = 0;
+  hipEventElapsedTime(&f, c->ev0, c->ev1);
+  *ms += f;
+  return OIC_OK;
+}
+
+// which ref reads a -t T run searches: OverlapDriver hands out chunks of perThread reads from
+// bgnRefID and a chunk starting exactly at endRefID is never run (overlapInCore.C:225,
+// overlapInCore-Process_Overlaps.C:33)
+static uint32_t ref_end_searched(uint32_t bgn, uint32_t end, uint32_t nthreads) {
+  if (end < bgn) return end;
+  const uint32_t per = 1 + (end - bgn) / nthreads / 8;
+  return ((end - bgn) % per == 0) ? end - 1 : end;
+}
+
+static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<uint8_t> &searched, oic_sink *sink) {
+  const int K = (int)c->P.kmer;
+  const double t0 = now_ms();
+  std::vector<uint64_t> pbase(2 * (size_t)ns + 1);
+  uint64_t P = 0;
+  for (uint32_t i = 0; i < ns; i++) {
+    const uint64_t np = searched[i] ? (uint64_t)(c->h_len[s0 + i] - K + 1) : 0;
+    pbase[2 * i] = P; P += np;
+    pbase[2 * i + 1] = P; P += np;
+  }
+  pbase[2 * (size_t)ns] = P;
+  if (P == 0) return OIC_OK;
+  ALLOC(c, c->g_pbase, pbase.size() * 8);
+  ALLOC(c, c->g_searched, ns);
+  ALLOC(c, c->g_cnt, P * 4);
+  ALLOC(c, c->g_off, P * 8);
+  HIPCHK(c, hipMemcpyAsync(c->g_pbase.p, pbase.data(), pbase.size() * 8, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemcpyAsync(c->g_searched.p, searched.data(), ns, hipMemcpyHostToDevice, c->st));
+  GatherArgs GA;
+  GA.R = dev_reads(c);
+  GA.X.ukey = c->x_ukey.as<uint64_t>();
+  GA.X.ustart = c->x_ustart.as<uint64_t>();
+  GA.X.occ = c->x_occ.as<uint64_t>();
+  GA.X.dir = c->x_dir.as<uint64_t>();
+  GA.X.nu = c->nu;
+  GA.X.shift = c->dshift;
+  GA.s0 = s0;
+  GA.ns = ns;
+  GA.hbgn = c->hbgn;
+  GA.k = K;
+  GA.min_olap = c->P.min_olap;
+  GA.searched = c->g_searched.as<uint8_t>();
+  GA.tbits = std::max(1, ceil_log2(c->nblock));
+  const int sbits = std::max(1, ceil_log2(2ull * ns));
+  if (GA.tbits + sbits > 64) return fail(c, OIC_EINVAL, "pair key needs %d bits", GA.tbits + sbits);
+  if (c->nu == 0) return OIC_OK;
+  hipLaunchKernelGGL(k_hit_count, dim3(2 * ns), dim3(256), 0, c->st, GA, c->g_cnt.as<uint32_t>(), c->g_pbase.as<uint64_t>());
+  HIPCHK(c, hipGetLastError());
+  ALLOC(c, c->s_scan, scan_scratch_words(P) * 8);
+  uint64_t H = 0;
+  HIPCHK(c, scan_excl<uint32_t>(c->g_cnt.as<uint32_t>(), c->g_off.as<uint64_t>(), P, c->s_scan.as<uint64_t>(), c->st, &H));
+  c->ctr.kmer_probes += P;
+  c->ctr.hits += H;
+  if (H == 0) { c->ctr.ms_gather += now_ms() - t0; return OIC_OK; }
+  ALLOC(c, c->s_k1, H * 8);
+  ALLOC(c, c->s_v1, H * 8);
+  if (ensure_sort_temp(c, H) != OIC_OK) return OIC_ENOMEM;
+  uint64_t *hk = c->s_k1.as<uint64_t>(), *hv = c->s_v1.as<uint64_t>();
+  hipLaunchKernelGGL(k_hit_emit, dim3(2 * ns), dim3(256), 0, c->st, GA, c->g_off.as<uint64_t>(), c->g_pbase.as<uint64_t>(), hk, hv);
+  HIPCHK(c, hipGetLastError());
+  {
+    RadixTemp tmp{c->s_k2.as<uint64_t>(), c->s_v2.as<uint64_t>(), c->s_hist.as<uint32_t>(), c->s_offs.as<uint64_t>(), c->s_scan.as<uint64_t>()};
+    const int nbits = (GA.tbits + sbits + 7) / 8 * 8;
+    HIPCHK(c, radix_sort(&hk, &hv, H, nbits, tmp, c->st));
+  }
+  ALLOC(c, c->s_flag, H * 4);
+  ALLOC(c, c->s_idx, H * 8);
+  uint64_t np = 0;
+  {
+    const unsigned g = (unsigned)((H + 255) / 256);
+    hipLaunchKernelGGL(k_pair_flags, dim3(g), dim3(256), 0, c->st, hk, H, c->s_flag.as<uint32_t>());
+    HIPCHK(c, scan_excl<uint32_t>(c->s_flag.as<uint32_t>(), c->s_idx.as<uint64_t>(), H, c->s_scan.as<uint64_t>(), c->st, &np));
+    ALLOC(c, c->p_pairs, np * sizeof(PairRec));
+    hipLaunchKernelGGL(k_pair_write, dim3(g), dim3(256), 0, c->st, hk, H, c->s_flag.as<uint32_t>(), c->s_idx.as<uint64_t>(),
+                       c->p_pairs.as<PairRec>(), s0, c->hbgn, GA.tbits);
+    hipLaunchKernelGGL(k_pair_ends, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, H);
+  }
+  c->ctr.pairs += np;
+  c->ctr.ms_gather += now_ms() - t0;
+  const double t1 = now_ms();
+  ALLOC(c, c->p_tmp, H * 16);
+  ALLOC(c, c->p_nodes, H * 16);
+  hipLaunchKernelGGL(k_build_lists, dim3((unsigned)((np + 127) / 128)), dim3(128), 0, c->st, c->p_pairs.as<PairRec>(), np, hv,
+                     c->p_tmp.as<int4>(), c->p_nodes.as<int4>(), K);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->ctr.ms_lists += now_ms() - t1;
+
+  // processing order
+  const double t2 = now_ms();
+  ALLOC(c, c->p_order, np * 4);
+  ALLOC(c, c->p_pout, np * sizeof(PairOut));
+  ALLOC(c, c->p_recs, np * OIC_MAX_DISTINCT * sizeof(OutRec));
+  ALLOC(c, c->p_queue, 256);
+  ALLOC(c, c->p_stats, STAT_NSTATS * 8);
+  HIPCHK(c, hipMemsetAsync(c->p_stats.p, 0, STAT_NSTATS * 8, c->st));
+  HIPCHK(c, hipMemsetAsync(c->p_pout.p, 0, np * sizeof(PairOut), c->st));
+  std::vector<uint32_t> order;
+  {
+    int rc = make_order(c->P.schedule, c->p_pairs.as<PairRec>(), np, c->h_len.data(), c->have_model ? &c->model : nullptr, c->st, order);
+    if (rc) return fail(c, OIC_EHIP, "scheduling failed");
+  }
+  HIPCHK(c, hipMemcpyAsync(c->p_order.p, order.data(), np * 4, hipMemcpyHostToDevice, c->st));
+
+  PairKernelArgs A;
+  A.R = dev_reads(c);
+  A.C = search_const(c, true);
+  A.pairs = c->p_pairs.as<PairRec>();
+  A.order = c->p_order.as<uint32_t>();
+  A.nodes = c->p_nodes.as<int4>();
+  A.recs = c->p_recs.as<OutRec>();
+  A.pout = c->p_pout.as<PairOut>();
+  A.queue = c->p_queue.as<uint32_t>();
+  A.stats = c->p_stats.as<unsigned long long>();
+  double ms = 0;
+  const uint32_t waves = (uint32_t)c->ncu * 8;
+  int rc = run_pairs<kCapSmall>(c, A, np, waves, (int64_t)1 << 20, &ms);
+  if (rc) return rc;
+  // pairs whose band outgrew the small tile: rerun with the wide tile
+  std::vector<PairOut> pout(np);
+  HIPCHK(c, hipMemcpyAsync(pout.data(), c->p_pout.p, np * sizeof(PairOut), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  std::vector<uint32_t> retry;
+  for (uint64_t p = 0; p < np; p++)
+    if (pout[p].status == 1) retry.push_back((uint32_t)p);
+  if (!retry.empty()) {
+    c->ctr.overflow_retries += retry.size();
+    ALLOC(c, c->p_retry, retry.size() * 4);
+    HIPCHK(c, hipMemcpyAsync(c->p_retry.p, retry.data(), retry.size() * 4, hipMemcpyHostToDevice, c->st));
+    A.order = c->p_retry.as<uint32_t>();
+    const int32_t rows = c->T.error_bound[std::min<int32_t>(c->maxlen, (int32_t)oic::kMaxReadLen)] + 4;
+    const int64_t vcap = std::min<int64_t>((int64_t)rows * kCapLarge, (int64_t)1 << 25);
+    rc = run_pairs<kCapLarge>(c, A, retry.size(), std::min<uint32_t>(waves, 256), vcap, &ms);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(pout.data(), c->p_pout.p, np * sizeof(PairOut), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (uint64_t p = 0; p < np; p++)
+      if (pout[p].status == 1) return fail(c, OIC_ECAPACITY, "pair %lu exceeds the widest DP tile", (unsigned long)p);
+  }
+  c->ctr.ms_dp_kernel += ms;
+  c->ctr.ms_extend += now_ms() - t2;
+
+  // records
+  ALLOC(c, c->p_nout, np * 4);
+  ALLOC(c, c->p_noff, np * 8);
+  uint64_t nrec = 0;
+  hipLaunchKernelGGL(k_nout, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->st, c->p_pout.as<PairOut>(), np, c->p_nout.as<uint32_t>());
+  ALLOC(c, c->s_scan, scan_scratch_words(np) * 8);
+  HIPCHK(c, scan_excl<uint32_t>(c->p_nout.as<uint32_t>(), c->p_noff.as<uint64_t>(), np, c->s_scan.as<uint64_t>(), c->st, &nrec));
+  if (nrec) {
+    ALLOC(c, c->p_outrec, nrec * sizeof(OutRec));
+    hipLaunchKernelGGL(k_compact_recs, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->st, c->p_recs.as<OutRec>(), c->p_pout.as<PairOut>(),
+                       c->p_noff.as<uint64_t>(), np, c->p_outrec.as<OutRec>());
+  }
+  uint64_t stats[STAT_NSTATS];
+  HIPCHK(c, hipMemcpyAsync(stats, c->p_stats.p, sizeof stats, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  for (int i = 0; i < 6; i++) sink->stats[i] += stats[i];
+  c->ctr.dp_cells += stats[STAT_CELLS];
+  c->ctr.dp_lane_slots += stats[STAT_SLOTS];
+  c->ctr.extensions += stats[STAT_EXT];
+  c->ctr.dp_calls += stats[STAT_DPCALLS];
+  if (sink->count + nrec > sink->capacity || !sink->records) {
+    sink->records_needed = sink->count + nrec;
+    // keep counting so the caller learns the full size
+    sink->count += nrec;
+    return OIC_OK;
+  }
+  if (nrec)
+    HIPCHK(c, hipMemcpy(sink->records + sink->count, c->p_outrec.p, nrec * sizeof(OutRec), hipMemcpyDeviceToHost));
+  sink->count += nrec;
+  sink->records_needed = sink->count;
+  if (c->keep_trace) {
+    std::vector<PairRec> pr(np);
+    HIPCHK(c, hipMemcpy(pr.data(), c->p_pairs.p, np * sizeof(PairRec), hipMemcpyDeviceToHost));
+    for (uint64_t p = 0; p < np; p++) {
+      PairFeat f;
+      pair_features(pr[p], c->h_len.data(), &f);
+      f.cells = (double)pout[p].cells;
+      f.exit_d = pout[p].exit_d;
+      f.n_ext = pout[p].n_ext;
+      c->last_feat.push_back(f);
+    }
+  }
+  return OIC_OK;
+}
+
+static int search_range(oic_ctx *c, uint32_t bgn, uint32_t end, oic_sink *sink) {
+  if (!c->have_index) return fail(c, OIC_ESTATE, "no index built");
+  const int K = (int)c->P.kmer;
+  const uint64_t POS_CAP = (uint64_t)1 << 27;
+  uint32_t r = bgn;
+  while (r <= end) {
+    uint64_t pos = 0;
+    uint32_t ns = 0;
+    std::vector<uint8_t> searched;
+    while (r + ns <= end) {
+      const int32_t L = c->h_len[r + ns];
+      const bool s = L >= c->P.min_olap && L >= K;
+      const uint64_t np = s ? 2ull * (uint64_t)(L - K + 1) : 0;
+      if (ns > 0 && pos + np > POS_CAP) break;
+      searched.push_back(s ? 1 : 0);
+      pos += np;
+      ns++;
+      if (ns >= (1u << 20)) break;
+    }
+    int rc = search_batch(c, r, ns, searched, sink);
+    if (rc) return rc;
+    r += ns;
+  }
+  return OIC_OK;
+}
+
+extern "C" int oic_search(oic_ctx *c, uint32_t bgn_ref, uint32_t end_ref, oic_sink *sink) {
+  if (!c || !sink) return OIC_EINVAL;
+  if (bgn_ref < 1) bgn_ref = 1;
+  if (end_ref > c->nreads) end_ref = c->nreads;
+  const uint32_t e = ref_end_searched(bgn_ref, end_ref, c->P.n_threads);
+  const double t0 = now_ms();
+  int rc = (e >= bgn_ref) ? search_range(c, bgn_ref, e, sink) : OIC_OK;
+  c->ctr.ms_total += now_ms() - t0;
+  return rc;
+}
+
+extern "C" int oic_run(oic_ctx *c, uint32_t bgn_hash, uint32_t end_hash, uint32_t bgn_ref, uint32_t end_ref, oic_sink *sink) {
+  if (!c || !sink) return OIC_EINVAL;
+  const double t0 = now_ms();
+  c->ctr = oic_counters{};
+  c->last_feat.clear();
+  sink->count = 0;
+  sink->records_needed = 0;
+  memset(sink->stats, 0, sizeof sink->stats);
+  // overlapInCore.C:179-189
+  if (bgn_hash < 1) bgn_hash = 1;
+  if (c->nreads < end_hash) end_hash = c->nreads;
+  if (bgn_ref < 1) bgn_ref = 1;
+  if (end_ref > c->nreads) end_ref = c->nreads;
+  const uint32_t e = ref_end_searched(bgn_ref, end_ref, c->P.n_threads);
+  uint32_t bh = bgn_hash, eh = end_hash;
+  while (bh < end_hash) {   // overlapInCore.C:204
+    if (eh > end_hash) eh = end_hash;
+    uint32_t last = 0;
+    int rc = oic_index_build(c, bh, eh, &last);
+    if (rc) return rc;
+    if (e >= bgn_ref) {
+      rc = search_range(c, bgn_ref, e, sink);
+      if (rc) return rc;
+    }
+    bh = last + 1;
+    eh = end_hash;
+  }
+  c->ctr.ms_total = now_ms() - t0;
+  if (sink->count > sink->capacity) return fail(c, OIC_ECAPACITY, "sink holds %lu records, %lu needed", (unsigned long)sink->capacity,
+                                                 (unsigned long)sink->count);
+  return OIC_OK;
+}
+
+// ------------------------------------------------------------------------------------ DP units
+
+struct BatchArgs {
+  ReadsDev R;
+  DPTables tb;
+  SearchConst C;
+  const oic_dp_task *dtasks;
+  const oic_ext_task *etasks;
+  uint32_t n;
+  oic_dp_result *dres;
+  oic_ext_result *eres;
+  int32_t *delta;
+  uint32_t delta_stride;
+  uint32_t *queue;
+  DPScratch sc0;
+  int64_t row_stride, val_stride, delta_stride_sc;
+};
+
+template <int CAP>
+__device__ __forceinline__ DPScratch batch_scratch(const BatchArgs &A, uint32_t w) {
+  DPScratch s = A.sc0;
+  s.rowL = A.sc0.rowL + (int64_t)w * A.row_stride;
+  s.rowR = A.sc0.rowR + (int64_t)w * A.row_stride;
+  s.rowOff = A.sc0.rowOff + (int64_t)w * A.row_stride;
+  s.rowVal = A.sc0.rowVal + (int64_t)w * A.val_stride;
+  s.dbuf = A.sc0.dbuf + (int64_t)w * A.delta_stride_sc;
+  s.ldelta = A.sc0.ldelta + (int64_t)w * A.delta_stride_sc;
+  s.rdelta = A.sc0.rdelta + (int64_t)w * A.delta_stride_sc;
+  return s;
+}
+
+template <int CAP>
+__global__ void __launch_bounds__(OIC_WAVE) k_dp_batch(BatchArgs A) {
+  __shared__ int32_t bufA[CAP], bufB[CAP];
+  __shared__ uint32_t item;
+  const int lane = lane_id();
+  const DPScratch sc = batch_scratch<CAP>(A, blockIdx.x);
+  while (true) {
+    if (lane == 0) item = atomicAdd(A.queue, 1u);
+    __syncthreads();
+    const uint32_t i = item;
+    __syncthreads();
+    if (i >= A.n) break;
+    const oic_dp_task t = A.dtasks[i];
+    SeqV a, b;
+    if (t.dir == 0) {
+      a = read_view(A.R, t.a_id, t.a_strand); a.b0 += t.a_pos;
+      b = read_view(A.R, t.b_id, t.b_strand); b.b0 += t.b_pos;
+    } else {
+      a = read_view(A.R, t.a_id, 1 - t.a_strand); a.b0 += A.R.len[t.a_id] - 1 - t.a_pos;
+      b = read_view(A.R, t.b_id, 1 - t.b_strand); b.b0 += A.R.len[t.b_id] - 1 - t.b_pos;
+    }
+    uint64_t cells = 0, slots = 0;
+    DPOut o = wave_prefix_dp<CAP>(a, t.m, b, t.n, t.error_limit, t.dir == 0, true, A.tb, bufA, bufB, sc, cells, slots);
+    const int32_t *src = (t.dir == 0) ? sc.rdelta : sc.ldelta;
+    if (A.delta && !o.overflow) {
+      const int32_t nd = o.ndelta < (int32_t)A.delta_stride ? o.ndelta : (int32_t)A.delta_stride;
+      for (int32_t j = lane; j < nd; j += OIC_WAVE) A.delta[(uint64_t)i * A.delta_stride + j] = src[j];
+    }
+    if (lane == 0) {
+      oic_dp_result r;
+      r.errors = o.e; r.a_end = o.a_end; r.t_end = o.t_end; r.match_to_end = o.mte; r.leftover = o.leftover;
+      r.delta_len = o.ndelta; r.status = o.overflow ? OIC_ECAPACITY : 0; r.pad = 0; r.cells = cells;
+      A.dres[i] = r;
+    }
+    __syncthreads();
+  }
+}
+
+template <int CAP>
+__global__ void __launch_bounds__(OIC_WAVE) k_ext_batch(BatchArgs A) {
+  __shared__ int32_t bufA[CAP], bufB[CAP];
+  __shared__ uint32_t item;
+  const int lane = lane_id();
+  const DPScratch sc = batch_scratch<CAP>(A, blockIdx.x);
+  while (true) {
+    if (lane == 0) item = atomicAdd(A.queue, 1u);
+    __syncthreads();
+    const uint32_t i = item;
+    __syncthreads();
+    if (i >= A.n) break;
+    const oic_ext_task t = A.etasks[i];
+    const SeqV S = read_view(A.R, t.s_id, t.s_dir), Src = read_view(A.R, t.s_id, 1 - t.s_dir);
+    const SeqV T = read_view(A.R, t.t_id, 0), Trc = read_view(A.R, t.t_id, 1);
+    uint64_t cells = 0, slots = 0;
+    uint32_t calls = 0;
+    ExtOut x = wave_extend<CAP>(t.start, t.offset, t.len, S, Src, A.R.len[t.s_id], T, Trc, A.R.len[t.t_id], A.C, bufA, bufB, sc, cells, slots, calls);
+    if (A.delta && !x.overflow) {
+      const int32_t nd = x.ndelta < (int32_t)A.delta_stride ? x.ndelta : (int32_t)A.delta_stride;
+      for (int32_t j = lane; j < nd; j += OIC_WAVE) A.delta[(uint64_t)i * A.delta_stride + j] = sc.ldelta[j];
+    }
+    if (lane == 0) {
+      oic_ext_result r;
+      r.kind = x.kind; r.s_lo = x.s_lo; r.s_hi = x.s_hi; r.t_lo = x.t_lo; r.t_hi = x.t_hi; r.errors = x.errors;
+      r.delta_len = x.ndelta; r.status = x.overflow ? OIC_ECAPACITY : 0;
+      A.eres[i] = r;
+    }
+    __syncthreads();
+  }
+}
+
+template <int CAP>
+static int launch_batch(oic_ctx *c, BatchArgs A, bool ext, uint32_t waves, int64_t vcap) {
+  int rc = ensure_wave_scratch(c, waves, CAP, vcap);
+  if (rc) return rc;
+  A.sc0 = scratch0(c);
+  A.row_stride = c->sc_rows;
+  A.val_stride = c->sc_val;
+  A.delta_stride_sc = c->sc_rows + 4;
+  HIPCHK(c, hipMemsetAsync(A.queue, 0, 4, c->st));
+  if (ext) hipLaunchKernelGGL(k_ext_batch<CAP>, dim3(waves), dim3(OIC_WAVE), 0, c->st, A);
+  else     hipLaunchKernelGGL(k_dp_batch<CAP>, dim3(waves), dim3(OIC_WAVE), 0, c->st, A);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return OIC_OK;
+}
+
+extern "C" int oic_prefix_dp_batch(oic_ctx *c, const oic_dp_task *tasks, uint32_t n, oic_dp_result *res, int32_t *delta,
+                                   uint32_t stride) {
+  if (!c || (!tasks && n) || (!res && n)) return OIC_EINVAL;
+  if (n == 0) return OIC_OK;
+  for (uint32_t i = 0; i < n; i++) {
+    const oic_dp_task &t = tasks[i];
+    if (t.a_id < 1 || t.a_id > c->nreads || t.b_id < 1 || t.b_id > c->nreads) return fail(c, OIC_EINVAL, "task %u: read id", i);
+    if (t.m < 0 || t.n < 0 || t.m > t.n || t.error_limit < 0 || t.error_limit >= (int32_t)c->T.max_errors)
+      return fail(c, OIC_EINVAL, "task %u: need 0 <= m <= n, 0 <= limit < max errors", i);
+    const int32_t la = c->h_len[t.a_id], lb = c->h_len[t.b_id];
+    if (t.dir == 0) {
+      if (t.a_pos < 0 || t.a_pos + t.m > la || t.b_pos < 0 || t.b_pos + t.n > lb) return fail(c, OIC_EINVAL, "task %u: window", i);
+    } else {
+      if (t.a_pos >= la || t.a_pos - t.m + 1 < 0 || t.b_pos >= lb || t.b_pos - t.n + 1 < 0) return fail(c, OIC_EINVAL, "task %u: window", i);
+    }
+  }
+  DevBuf dt, dr, dd, q;
+  ALLOC(c, dt, (size_t)n * sizeof(oic_dp_task));
+  ALLOC(c, dr, (size_t)n * sizeof(oic_dp_result));
+  ALLOC(c, q, 256);
+  if (delta) ALLOC(c, dd, (size_t)n * stride * 4);
+  HIPCHK(c, hipMemcpyAsync(dt.p, tasks, (size_t)n * sizeof(oic_dp_task), hipMemcpyHostToDevice, c->st));
+  BatchArgs A{};
+  A.R = dev_reads(c);
+  A.tb = dev_tables(c);
+  A.dtasks = dt.as<oic_dp_task>();
+  A.n = n;
+  A.dres = dr.as<oic_dp_result>();
+  A.delta = delta ? dd.as<int32_t>() : nullptr;
+  A.delta_stride = stride;
+  A.queue = q.as<uint32_t>();
+  const uint32_t waves = std::min<uint32_t>(n, (uint32_t)c->ncu * 4);
+  const int32_t rows = c->T.error_bound[std::min<int32_t>(c->maxlen, (int32_t)oic::kMaxReadLen)] + 4;
+  int rc = launch_batch<kCapLarge>(c, A, false, waves, std::min<int64_t>((int64_t)rows * kCapLarge, (int64_t)1 << 24));
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpy(res, dr.p, (size_t)n * sizeof(oic_dp_result), hipMemcpyDeviceToHost));
+  if (delta) HIPCHK(c, hipMemcpy(delta, dd.p, (size_t)n * stride * 4, hipMemcpyDeviceToHost));
+  return OIC_OK;
+}
+
+extern "C" int oic_extend_batch(oic_ctx *c, const oic_ext_task *tasks, uint32_t n, oic_ext_result *res, int32_t *delta, uint32_t stride) {
+  if (!c || (!tasks && n) || (!res && n)) return OIC_EINVAL;
+  if (n == 0) return OIC_OK;
+  for (uint32_t i = 0; i < n; i++) {
+    const oic_ext_task &t = tasks[i];
+    if (t.s_id < 1 || t.s_id > c->nreads || t.t_id < 1 || t.t_id > c->nreads) return fail(c, OIC_EINVAL, "task %u: read id", i);
+    const int32_t ls = c->h_len[t.s_id], lt = c->h_len[t.t_id];
+    if (t.len < 1 || t.start < 0 || t.offset < 0 || t.start + t.len > ls || t.offset + t.len > lt || (t.s_dir != 0 && t.s_dir != 1))
+      return fail(c, OIC_EINVAL, "task %u: seed outside the reads", i);
+  }
+  DevBuf dt, dr, dd, q;
+  ALLOC(c, dt, (size_t)n * sizeof(oic_ext_task));
+  ALLOC(c, dr, (size_t)n * sizeof(oic_ext_result));
+  ALLOC(c, q, 256);
+  if (delta) ALLOC(c, dd, (size_t)n * stride * 4);
+  HIPCHK(c, hipMemcpyAsync(dt.p, tasks, (size_t)n * sizeof(oic_ext_task), hipMemcpyHostToDevice, c->st));
+  BatchArgs A{};
+  A.R = dev_reads(c);
+  A.tb = dev_tables(c);
+  A.C = search_const(c, false);
+  A.etasks = dt.as<oic_ext_task>();
+  A.n = n;
+  A.eres = dr.as<oic_ext_result>();
+  A.delta = delta ? dd.as<int32_t>() : nullptr;
+  A.delta_stride = stride;
+  A.queue = q.as<uint32_t>();
+  const uint32_t waves = std::min<uint32_t>(n, (uint32_t)c->ncu * 4);
+  const int32_t rows = c->T.error_bound[std::min<int32_t>(c->maxlen, (int32_t)oic::kMaxReadLen)] + 4;
+  int rc = launch_batch<kCapLarge>(c, A, true, waves, std::min<int64_t>((int64_t)rows * kCapLarge, (int64_t)1 << 24));
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpy(res, dr.p, (size_t)n * sizeof(oic_ext_result), hipMemcpyDeviceToHost));
+  if (delta) HIPCHK(c, hipMemcpy(delta, dd.p, (size_t)n * stride * 4, hipMemcpyDeviceToHost));
+  return OIC_OK;
+}

@@ -1,0 +1,925 @@
+// oic_search.h -- seed table, seed-hit gathering, match lists and per-pair Process_Matches on gfx950.
+//
+// Layout in HBM
+//   reads     : 2-bit packed words; read r's forward strand starts at base b0f[r], its reverse
+//               complement at b0r[r] (both multiples of 32, two guard words after each strand);
+//               an N mask (one bit per base, same base index) flags non-ACGT bases.
+//   k-mer table (one hash block): every valid k-mer position of the block's reads as
+//               (key, t_local << 32 | offset), stably radix-sorted by key, so each k-mer's
+//               occurrences are in ascending (read, offset) order; unique keys plus a directory
+//               over the key's top bits locate a k-mer's run with one read and a short search.
+//   hits      : for a batch of ref reads, one (pair key, hit) per seed hit with S_ID < T_ID, emitted
+//               in (S offset asc, T desc, T offset desc) order and stably sorted by pair key =
+//               (S, orientation, T): each candidate pair's hits come out in the order Add_Ref sees
+//               them in Find_Overlaps (overlapInCore-Find_Overlaps.C:251-332).
+//   nodes     : per pair, Match_Node_t list in list order (Offset, Len, Start, removed).
+#pragma once
+
+#include "oic_dp.h"
+
+#define OIC_MAX_DISTINCT 3       // MAX_DISTINCT_OLAPS (overlapInCore.H:107)
+#define OIC_HOPELESS     90      // HOPELESS_MATCH (overlapInCore.H:78)
+#define OIC_MIN_INTER    10      // MIN_INTERSECTION (overlapInCore.H:120)
+#define OIC_SHIFT_SLACK  1       // SHIFT_SLACK (overlapInCore.H:146)
+
+struct ReadsDev {
+  const uint32_t *words;
+  const uint32_t *nmask;
+  const int64_t  *b0f, *b0r;
+  const int32_t  *len;
+  const uint8_t  *hasN;
+};
+
+__device__ __forceinline__ SeqV read_view(const ReadsDev &R, uint32_t r, int strand) {
+  SeqV v;
+  v.w = R.words;
+  v.nm = R.hasN[r] ? R.nmask : nullptr;
+  v.b0 = strand ? R.b0r[r] : R.b0f[r];
+  return v;
+}
+
+__device__ __forceinline__ uint32_t base_code(uint8_t c, bool *isN) {
+  switch (c) {
+    case 'A': case 'a': *isN = false; return 0;
+    case 'C': case 'c': *isN = false; return 1;
+    case 'G': case 'g': *isN = false; return 2;
+    case 'T': case 't': *isN = false; return 3;
+    default: *isN = true; return 0;
+  }
+}
+
+// one block per read: pack forward and reverse-complement strands, N masks
+__global__ void __launch_bounds__(256) k_pack(const uint8_t *__restrict__ raw, const uint64_t *__restrict__ start,
+                                             const int32_t *__restrict__ len, const int64_t *__restrict__ b0f,
+                                             const int64_t *__restrict__ b0r, uint32_t *__restrict__ words,
+                                             uint32_t *__restrict__ nmask, uint8_t *__restrict__ hasN, uint32_t nreads) {
+  const uint32_t r = blockIdx.x + 1;
+  if (r > nreads) return;
+  const int32_t L = len[r];
+  const uint8_t *s = raw + start[r];
+  __shared__ int anyN;
+  if (threadIdx.x == 0) anyN = 0;
+  __syncthreads();
+  const int32_t nw = (L + 31) / 32 + 1;   // 32-base groups incl. one guard group
+  for (int32_t g = threadIdx.x; g < nw; g += blockDim.x) {
+    uint32_t wf0 = 0, wf1 = 0, wr0 = 0, wr1 = 0, mf = 0, mr = 0;
+    for (int j = 0; j < 32; j++) {
+      const int32_t i = g * 32 + j;
+      if (i < L) {
+        bool n1;
+        const uint32_t c = base_code(s[i], &n1);
+        if (j < 16) wf0 |= c << (2 * j); else wf1 |= c << (2 * (j - 16));
+        if (n1) mf |= 1u << j;
+        bool n2;
+        const uint32_t cr = base_code(s[L - 1 - i], &n2);
+        const uint32_t cc = n2 ? 0u : (3u - cr);
+        if (j < 16) wr0 |= cc << (2 * j); else wr1 |= cc << (2 * (j - 16));
+        if (n2) mr |= 1u << j;
+      }
+    }
+    const int64_t pf = b0f[r] + (int64_t)g * 32, pr = b0r[r] + (int64_t)g * 32;
+    words[pf >> 4] = wf0; words[(pf >> 4) + 1] = wf1;
+    words[pr >> 4] = wr0; words[(pr >> 4) + 1] = wr1;
+    nmask[pf >> 5] = mf; nmask[pr >> 5] = mr;
+    if (mf) atomicOr(&anyN, 1);
+  }
+  // guard word after the last group
+  if (threadIdx.x == 0) {
+    const int64_t pf = b0f[r] + (int64_t)nw * 32, pr = b0r[r] + (int64_t)nw * 32;
+    words[pf >> 4] = 0; words[(pf >> 4) + 1] = 0; words[pr >> 4] = 0; words[(pr >> 4) + 1] = 0;
+    nmask[pf >> 5] = 0; nmask[pr >> 5] = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) hasN[r] = (uint8_t)anyN;
+}
+
+// k-mer key of view bases [p, p+k): base j at bits 2j (k <= 31)
+__device__ __forceinline__ uint64_t kmer_at(const uint32_t *__restrict__ w, int64_t p, int k) {
+  const int64_t wi = p >> 4;
+  const uint32_t s = (uint32_t)(p & 15) * 2u;
+  const uint64_t lo = ((uint64_t)w[wi + 1] << 32) | w[wi];
+  const uint64_t hi = w[wi + 2];
+  uint64_t v = s ? ((lo >> s) | (hi << (64 - s))) : lo;
+  return (k >= 32) ? v : (v & ((1ull << (2 * k)) - 1));
+}
+
+__device__ __forceinline__ bool kmer_has_n(const uint32_t *__restrict__ nm, int64_t p, int k) {
+  const int64_t wi = p >> 5;
+  const uint32_t s = (uint32_t)(p & 31);
+  const uint64_t v = (((uint64_t)nm[wi + 1] << 32) | nm[wi]) >> s;   // >= 33 valid bits, k <= 31
+  return (v & ((1ull << k) - 1)) != 0;
+}
+
+// emit (key, t_local << 32 | offset) for every window of every loaded hash read; invalid
+// windows (a non-ACGT base) get a key above every valid one.
+__global__ void __launch_bounds__(256) k_kmer_emit(ReadsDev R, uint32_t bgn, uint32_t nblock, const uint64_t *__restrict__ posbase,
+                                                  const uint8_t *__restrict__ loaded, int k, uint64_t invalid_key,
+                                                  uint64_t *__restrict__ keys, uint64_t *__restrict__ vals) {
+  const uint32_t t = blockIdx.x;
+  if (t >= nblock || !loaded[t]) return;
+  const uint32_t r = bgn + t;
+  const int32_t L = R.len[r];
+  const int32_t npos = L - k + 1;
+  const int64_t b0 = R.b0f[r];
+  const bool hn = R.hasN[r];
+  const uint64_t pb = posbase[t];
+  for (int32_t i = threadIdx.x; i < npos; i += blockDim.x) {
+    uint64_t key = kmer_at(R.words, b0 + i, k);
+    if (hn && kmer_has_n(R.nmask, b0 + i, k)) key = invalid_key;
+    keys[pb + i] = key;
+    vals[pb + i] = ((uint64_t)t << 32) | (uint32_t)i;
+  }
+}
+
+__global__ void k_unique_flags(const uint64_t *__restrict__ keys, uint64_t n, uint64_t invalid_key, uint32_t *__restrict__ flag) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = keys[i];
+  flag[i] = (k != invalid_key && (i == 0 || keys[i - 1] != k)) ? 1u : 0u;
+}
+
+__global__ void k_unique_write(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals, uint64_t n,
+                               const uint32_t *__restrict__ flag, const uint64_t *__restrict__ uidx, uint64_t *__restrict__ ukey,
+                               uint64_t *__restrict__ ustart, uint32_t *__restrict__ first_t) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  const uint64_t u = uidx[i];
+  ukey[u] = keys[i];
+  ustart[u] = i;
+  if (first_t) atomicAdd(&first_t[(uint32_t)(vals[i] >> 32)], 1u);
+}
+
+// dir[b] = first unique index whose key >> shift >= b, for b in [0, 2^bits]
+__global__ void k_build_dir(const uint64_t *__restrict__ ukey, uint64_t nu, int shift, uint64_t nb, uint64_t *__restrict__ dir) {
+  const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u > nu) return;
+  const int64_t bu = (u < nu) ? (int64_t)(ukey[u] >> shift) : (int64_t)nb;
+  const int64_t bp = (u == 0) ? -1 : (int64_t)(ukey[u - 1] >> shift);
+  for (int64_t b = bp + 1; b <= bu; b++) dir[b] = u;
+}
+
+struct KmerTable {
+  const uint64_t *ukey;     // [nu] sorted unique keys
+  const uint64_t *ustart;   // [nu + 1] run starts in occ (ustart[nu] = number of valid positions)
+  const uint64_t *occ;      // [nvalid] t_local << 32 | offset, ascending within a key
+  const uint64_t *dir;      // [2^dbits + 1]
+  uint64_t nu;
+  int shift;
+};
+
+// locate key; returns false if absent
+__device__ __forceinline__ bool table_find(const KmerTable &X, uint64_t key, uint64_t *lo, uint64_t *hi) {
+  const uint64_t b = key >> X.shift;
+  uint64_t a = X.dir[b], z = X.dir[b + 1];
+  while (a < z) {
+    const uint64_t mid = (a + z) >> 1;
+    if (X.ukey[mid] < key) a = mid + 1; else z = mid;
+  }
+  if (a >= X.nu || X.ukey[a] != key) return false;
+  *lo = X.ustart[a];
+  *hi = X.ustart[a + 1];
+  return true;
+}
+
+// first index in occ[lo, hi) whose read (t_local) is > tl
+__device__ __forceinline__ uint64_t occ_upper(const uint64_t *__restrict__ occ, uint64_t lo, uint64_t hi, uint32_t tl) {
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if ((uint32_t)(occ[mid] >> 32) <= tl) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+struct GatherArgs {
+  ReadsDev R;
+  KmerTable X;
+  uint32_t s0, ns;          // ref batch: IDs s0 .. s0+ns-1
+  uint32_t hbgn;            // Hash_String_Num_Offset
+  int32_t  k;
+  int32_t  min_olap;
+  const uint8_t *searched;  // [ns] 1 if this ref read is searched
+  int      tbits;           // bits for t_local in the pair key
+};
+
+// one block per (S, orientation): count hits with S_ID < T_ID for every window
+__global__ void __launch_bounds__(256) k_hit_count(GatherArgs A, uint32_t *__restrict__ cnt, const uint64_t *__restrict__ pbase) {
+  const uint32_t sd = blockIdx.x;
+  const uint32_t si = sd >> 1, dir = sd & 1;
+  if (si >= A.ns || !A.searched[si]) return;
+  const uint32_t S = A.s0 + si;
+  const int32_t L = A.R.len[S];
+  const int32_t npos = L - A.k + 1;
+  const int64_t b0 = dir ? A.R.b0r[S] : A.R.b0f[S];
+  const bool hn = A.R.hasN[S];
+  const uint64_t pb = pbase[sd];
+  const uint32_t tl_max = (S >= A.hbgn) ? (S - A.hbgn) : 0xffffffffu;   // T_local must exceed this
+  for (int32_t o = threadIdx.x; o < npos; o += blockDim.x) {
+    uint32_t c = 0;
+    if (!(hn && kmer_has_n(A.R.nmask, b0 + o, A.k))) {
+      const uint64_t key = kmer_at(A.R.words, b0 + o, A.k);
+      uint64_t lo, hi;
+      if (table_find(A.X, key, &lo, &hi)) {
+        const uint64_t f = (tl_max == 0xffffffffu) ? lo : occ_upper(A.X.occ, lo, hi, tl_max);
+        c = (uint32_t)(hi - f);
+      }
+    }
+    cnt[pb + o] = c;
+  }
+}
+
+// same walk; write the hits in (o asc, T desc, T offset desc) order at the scanned offsets
+__global__ void __launch_bounds__(256) k_hit_emit(GatherArgs A, const uint64_t *__restrict__ off, const uint64_t *__restrict__ pbase,
+                                                  uint64_t *__restrict__ hkey, uint64_t *__restrict__ hval) {
+  const uint32_t sd = blockIdx.x;
+  const uint32_t si = sd >> 1, dir = sd & 1;
+  if (si >= A.ns || !A.searched[si]) return;
+  const uint32_t S = A.s0 + si;
+  const int32_t L = A.R.len[S];
+  const int32_t npos = L - A.k + 1;
+  const int64_t b0 = dir ? A.R.b0r[S] : A.R.b0f[S];
+  const bool hn = A.R.hasN[S];
+  const uint64_t pb = pbase[sd];
+  const uint32_t tl_max = (S >= A.hbgn) ? (S - A.hbgn) : 0xffffffffu;
+  const uint64_t pref = (uint64_t)sd << A.tbits;
+  for (int32_t o = threadIdx.x; o < npos; o += blockDim.x) {
+    if (hn && kmer_has_n(A.R.nmask, b0 + o, A.k)) continue;
+    const uint64_t key = kmer_at(A.R.words, b0 + o, A.k);
+    uint64_t lo, hi;
+    if (!table_find(A.X, key, &lo, &hi)) continue;
+    const uint64_t f = (tl_max == 0xffffffffu) ? lo : occ_upper(A.X.occ, lo, hi, tl_max);
+    const uint64_t mult = hi - lo;
+    const uint64_t mcap = mult > 0x3fffffull ? 0x3fffffull : mult;
+    uint64_t w = off[pb + o];
+    for (uint64_t q = hi; q > f; q--) {
+      const uint64_t oc = A.X.occ[q - 1];
+      const uint32_t tl = (uint32_t)(oc >> 32), toff = (uint32_t)oc;
+      hkey[w] = pref | tl;
+      hval[w] = (uint64_t)o | ((uint64_t)toff << 21) | (mcap << 42);
+      w++;
+    }
+  }
+}
+
+__global__ void k_pair_flags(const uint64_t *__restrict__ hkey, uint64_t n, uint32_t *__restrict__ flag) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flag[i] = (i == 0 || hkey[i] != hkey[i - 1]) ? 1u : 0u;
+}
+
+struct PairRec {
+  uint64_t hb, he;          // hit / node range
+  uint32_t s_id, t_id;
+  int32_t  dir;
+  int32_t  nnodes;
+  int32_t  consistent;
+  int32_t  diag_bgn, diag_end;   // min / max S offset of the pair's hits
+  int32_t  max_mult;
+  int32_t  longest;              // longest node Len
+  int32_t  dmin, dmax;           // min / max node diagonal
+  int32_t  pad;
+};
+
+__global__ void k_pair_write(const uint64_t *__restrict__ hkey, uint64_t n, const uint32_t *__restrict__ flag,
+                             const uint64_t *__restrict__ pidx, PairRec *__restrict__ pairs, uint32_t s0, uint32_t hbgn, int tbits) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  const uint64_t p = pidx[i];
+  const uint64_t k = hkey[i];
+  PairRec pr = {};
+  pr.hb = i;
+  pr.t_id = (uint32_t)(k & ((1ull << tbits) - 1)) + hbgn;
+  const uint64_t sd = k >> tbits;
+  pr.s_id = s0 + (uint32_t)(sd >> 1);
+  pr.dir = (int32_t)(sd & 1);
+  pairs[p] = pr;
+}
+
+__global__ void k_pair_ends(PairRec *__restrict__ pairs, uint64_t np, uint64_t nhits) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  pairs[p].he = (p + 1 < np) ? pairs[p + 1].hb : nhits;
+}
+
+// Add_Ref + Add_Match state machine (overlapInCore-Find_Overlaps.C:24-96, :104-163), one thread
+// per candidate pair.  The linked list lives in tmp (Offset, Len, Start, Next; -1 = end); the
+// finished list is written head-to-tail into nodes.
+__global__ void k_build_lists(PairRec *__restrict__ pairs, uint64_t np, const uint64_t *__restrict__ hval, int4 *__restrict__ tmp,
+                              int4 *__restrict__ nodes, int32_t K) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  PairRec pr = pairs[p];
+  const uint64_t hb = pr.hb, he = pr.he;
+  int4 *T = tmp + hb;
+  int32_t head = -1, navail = 0;
+  int32_t consistent = 1, maxm = 0;
+  for (uint64_t h = hb; h < he; h++) {
+    const uint64_t v = hval[h];
+    const int32_t offset = (int32_t)(v & 0x1fffff);
+    const int32_t toff = (int32_t)((v >> 21) & 0x1fffff);
+    const int32_t mult = (int32_t)(v >> 42);
+    if (mult > maxm) maxm = mult;
+    const int32_t new_diag = toff - offset;
+    int32_t diag = 0, expected_start = 0, num_checked = 0;
+    bool move_to_front = false, done = false;
+    int32_t prev = -1, q = head;
+    while (q != -1) {
+      const int4 nd = T[q];   // x Offset, y Len, z Start, w Next
+      expected_start = nd.z + nd.y - K + 1;
+      diag = nd.x - nd.z;
+      if (expected_start < offset) break;
+      if (expected_start == offset) {
+        if (new_diag == diag) {
+          T[q].y = nd.y + 1;
+          if (move_to_front) {
+            T[prev].w = nd.w;       // prev != -1 here: a node was passed before q
+            T[q].w = head;
+            head = q;
+          }
+          done = true;
+          break;
+        } else
+          move_to_front = true;
+      }
+      num_checked++;
+      prev = q;
+      q = nd.w;
+    }
+    if (done) continue;
+    if (head != -1 && (num_checked > 0 || abs(diag - new_diag) > 3 || offset < expected_start + K - 2))
+      consistent = 0;
+    T[navail] = make_int4(toff, K, offset, head);
+    head = navail++;
+  }
+  int32_t n = 0, longest = 0, dmin = 0x7fffffff, dmax = -0x7fffffff;
+  for (int32_t q = head; q != -1; q = T[q].w) {
+    const int4 nd = T[q];
+    nodes[hb + n] = make_int4(nd.x, nd.y, nd.z, 0);
+    n++;
+    if (nd.y > longest) longest = nd.y;
+    const int32_t dg = nd.x - nd.z;
+    if (dg < dmin) dmin = dg;
+    if (dg > dmax) dmax = dg;
+  }
+  pr.nnodes = n;
+  pr.consistent = consistent;
+  pr.diag_bgn = (int32_t)(hval[hb] & 0x1fffff);
+  pr.diag_end = (int32_t)(hval[he - 1] & 0x1fffff);
+  pr.max_mult = maxm;
+  pr.longest = longest;
+  pr.dmin = dmin;
+  pr.dmax = dmax;
+  pairs[p] = pr;
+}
+
+// ------------------------------------------------------------------ per-pair Process_Matches
+
+struct OlapDev {
+  int32_t s_lo, s_hi, t_lo, t_hi;
+  double  quality;
+  int32_t delta_ct;
+  int32_t slb, srb, tlb, trb;
+  int32_t min_diag, max_diag;
+};
+
+struct ExtOut {
+  int32_t kind, s_lo, s_hi, t_lo, t_hi, errors, ndelta, overflow;
+  int32_t fwd_e;       // errors of the forward DP (exit row)
+};
+
+struct SearchConst {
+  DPTables tb;
+  int32_t  min_olap;
+  int32_t  unique;
+  int32_t  partial;
+  int32_t  hopeless;
+  int32_t  K;
+  const uint32_t *min_kmers; // --minkmers threshold by diagonal span, or nullptr
+  int32_t  elide;          // skip reverse DP / tracebacks whose results Process_Matches cannot use
+};
+
+// Extend_Alignment (prefixEditDistance-extend.C:35-183) for one wave.  Deltas land in sc.ldelta
+// (merged Left_Delta) when needed.
+template <int CAP>
+__device__ ExtOut wave_extend(int32_t Start, int32_t Offset, int32_t Len, const SeqV S, const SeqV Src, int32_t S_Len,
+                              const SeqV T, const SeqV Trc, int32_t T_Len, const SearchConst &C, int32_t *bufA,
+                              int32_t *bufB, const DPScratch &sc, uint64_t &cells, uint64_t &slots, uint32_t &dpcalls) {
+  const int lane = lane_id();
+  ExtOut r = {};
+  const int32_t S_Left_Begin = Start - 1, S_Right_Begin = Start + Len, S_Right_Len = S_Len - S_Right_Begin;
+  const int32_t T_Left_Begin = Offset - 1, T_Right_Begin = Offset + Len, T_Right_Len = T_Len - T_Right_Begin;
+  const int32_t Total_Olap = (Start < Offset ? Start : Offset) + Len + (S_Right_Len < T_Right_Len ? S_Right_Len : T_Right_Len);
+  const int32_t Error_Limit = C.tb.error_bound[Total_Olap];
+  const bool need_delta = !C.elide || C.partial;   // with elision, deltas only after both sides end
+
+  int32_t Right_Errors = 0, Left_Errors = 0, Leftover = 0, nr = 0, nl = 0;
+  bool rmte = true, lmte = true;
+  int32_t S_Hi, T_Hi, S_Lo, T_Lo;
+  bool right_negate = false;
+
+  if (S_Right_Len == 0 || T_Right_Len == 0) {
+    S_Hi = 0; T_Hi = 0; rmte = true;
+  } else {
+    SeqV a = S, b = T;
+    int32_t am, bn;
+    if (S_Right_Len <= T_Right_Len) {
+      a.b0 += S_Right_Begin; am = S_Right_Len; b.b0 += T_Right_Begin; bn = T_Right_Len; right_negate = true;
+    } else {
+      a = T; b = S; a.b0 += T_Right_Begin; am = T_Right_Len; b.b0 += S_Right_Begin; bn = S_Right_Len;
+    }
+    DPOut o = wave_prefix_dp<CAP>(a, am, b, bn, Error_Limit, true, need_delta, C.tb, bufA, bufB, sc, cells, slots);
+    dpcalls++;
+    if (o.overflow) { r.overflow = 1; return r; }
+    Right_Errors = o.e; rmte = o.mte; nr = o.ndelta;
+    if (right_negate) { S_Hi = o.a_end; T_Hi = o.t_end; } else { T_Hi = o.a_end; S_Hi = o.t_end; }
+    if (!need_delta && !rmte) {
+      // Process_Matches only distinguishes DOVETAIL from the rest when not doing partials
+      r.kind = 0; r.fwd_e = Right_Errors;
+      r.s_hi = S_Hi + S_Right_Begin - 1; r.t_hi = T_Hi + T_Right_Begin - 1;
+      r.errors = Right_Errors;
+      return r;
+    }
+  }
+  r.fwd_e = Right_Errors;
+  S_Hi += S_Right_Begin - 1;
+  T_Hi += T_Right_Begin - 1;
+
+  // with elision the right side matched to the end here; its delta is needed if the left side
+  // also does: recompute it lazily below (rows of the right DP were not kept)
+  if (S_Left_Begin < 0 || T_Left_Begin < 0) {
+    S_Lo = 0; T_Lo = 0; lmte = true;
+  } else {
+    SeqV a, b;
+    int32_t am, bn;
+    bool swapped;
+    if (S_Right_Begin <= T_Right_Begin) {
+      a = Src; a.b0 += S_Len - 1 - S_Left_Begin; am = S_Left_Begin + 1;
+      b = Trc; b.b0 += T_Len - 1 - T_Left_Begin; bn = T_Left_Begin + 1;
+      swapped = false;
+    } else {
+      a = Trc; a.b0 += T_Len - 1 - T_Left_Begin; am = T_Left_Begin + 1;
+      b = Src; b.b0 += S_Len - 1 - S_Left_Begin; bn = S_Left_Begin + 1;
+      swapped = true;
+    }
+    DPOut o = wave_prefix_dp<CAP>(a, am, b, bn, Error_Limit - Right_Errors, false, need_delta, C.tb, bufA, bufB, sc, cells, slots);
+    dpcalls++;
+    if (o.overflow) { r.overflow = 1; return r; }
+    Left_Errors = o.e; lmte = o.mte; nl = o.ndelta; Leftover = o.leftover;
+    if (!swapped) { S_Lo = o.a_end; T_Lo = o.t_end; } else { T_Lo = o.a_end; S_Lo = o.t_end; }
+    if (need_delta && swapped && lane == 0)
+      for (int32_t i = 0; i < nl; i++) sc.ldelta[i] = -sc.ldelta[i];
+  }
+  S_Lo += S_Left_Begin + 1;
+  T_Lo += T_Left_Begin + 1;
+
+  r.errors = Left_Errors + Right_Errors;
+  r.kind = (!rmte) ? (!lmte ? 0 : 2) : (!lmte ? 1 : 3);
+  r.s_lo = S_Lo; r.s_hi = S_Hi; r.t_lo = T_Lo; r.t_hi = T_Hi;
+
+  if (!need_delta && r.kind == 3) {
+    // both sides matched to the end: rerun the two DPs with row storage for the deltas
+    int32_t re_e = 0, rr = 0;
+    {
+      SeqV a = S, b = T;
+      int32_t am, bn;
+      if (S_Right_Len == 0 || T_Right_Len == 0) { am = 0; bn = 0; }
+      else if (S_Right_Len <= T_Right_Len) { a.b0 += S_Right_Begin; am = S_Right_Len; b.b0 += T_Right_Begin; bn = T_Right_Len; }
+      else { a = T; b = S; a.b0 += T_Right_Begin; am = T_Right_Len; b.b0 += S_Right_Begin; bn = S_Right_Len; }
+      if (am > 0) {
+        DPOut o = wave_prefix_dp<CAP>(a, am, b, bn, Error_Limit, true, true, C.tb, bufA, bufB, sc, cells, slots);
+        dpcalls++;
+        if (o.overflow) { r.overflow = 1; return r; }
+        rr = o.ndelta; re_e = o.e;
+      }
+      // keep Right_Delta aside in dbuf while the left DP reuses the row storage
+      __syncthreads();
+      if (lane == 0) for (int32_t i = 0; i < rr; i++) sc.dbuf[i] = sc.rdelta[i];
+      __syncthreads();
+    }
+    (void)re_e;
+    if (!(S_Left_Begin < 0 || T_Left_Begin < 0)) {
+      SeqV a, b;
+      int32_t am, bn;
+      bool swapped;
+      if (S_Right_Begin <= T_Right_Begin) {
+        a = Src; a.b0 += S_Len - 1 - S_Left_Begin; am = S_Left_Begin + 1;
+        b = Trc; b.b0 += T_Len - 1 - T_Left_Begin; bn = T_Left_Begin + 1;
+        swapped = false;
+      } else {
+        a = Trc; a.b0 += T_Len - 1 - T_Left_Begin; am = T_Left_Begin + 1;
+        b = Src; b.b0 += S_Len - 1 - S_Left_Begin; bn = S_Left_Begin + 1;
+        swapped = true;
+      }
+      DPOut o = wave_prefix_dp<CAP>(a, am, b, bn, Error_Limit - Right_Errors, false, true, C.tb, bufA, bufB, sc, cells, slots);
+      dpcalls++;
+      if (o.overflow) { r.overflow = 1; return r; }
+      nl = o.ndelta; Leftover = o.leftover;
+      // Set_Left_Delta may move T_End by one (reverse.C:76-92): take the coordinates from this run
+      if (!swapped) { r.s_lo = o.a_end + S_Left_Begin + 1; r.t_lo = o.t_end + T_Left_Begin + 1; }
+      else          { r.t_lo = o.a_end + T_Left_Begin + 1; r.s_lo = o.t_end + S_Left_Begin + 1; }
+      if (swapped && lane == 0)
+        for (int32_t i = 0; i < nl; i++) sc.ldelta[i] = -sc.ldelta[i];
+    } else {
+      nl = 0; Leftover = 0;
+    }
+    __syncthreads();
+    if (lane == 0) for (int32_t i = 0; i < rr; i++) sc.rdelta[i] = sc.dbuf[i];
+    __syncthreads();
+    nr = rr;
+  }
+
+  // merge deltas (extend.C:164-175); Right_Delta was negated when S was the shorter right tail
+  if (need_delta || r.kind == 3) {
+    __syncthreads();
+    if (lane == 0) {
+      int32_t n = nl;
+      if (nr > 0) {
+        const int32_t r0 = right_negate ? -sc.rdelta[0] : sc.rdelta[0];
+        sc.ldelta[n++] = (r0 > 0) ? -(r0 + Leftover + Len) : -(r0 - Leftover - Len);
+      }
+      for (int32_t i = 1; i < nr; i++) {
+        const int32_t ri = right_negate ? -sc.rdelta[i] : sc.rdelta[i];
+        sc.ldelta[n++] = -ri;
+      }
+      nl = n;
+    }
+    r.ndelta = __shfl(nl, 0);
+    __syncthreads();
+  }
+  return r;
+}
+
+// Lies_On_Alignment (overlapInCore-Process_String_Overlaps.C:260-281)
+__device__ __forceinline__ bool lies_on_alignment(int32_t start, int32_t offset, int32_t s_lo, int32_t t_lo,
+                                                  const int32_t *__restrict__ delta, int32_t nd) {
+  int32_t diag = t_lo - s_lo;
+  const int32_t new_diag = offset - start;
+  for (int32_t i = 0; i < nd; i++) {
+    const int32_t dl = delta[i];
+    s_lo += dl < 0 ? -dl : dl;
+    if (start < s_lo) return abs(new_diag - diag) <= OIC_SHIFT_SLACK;
+    if (dl < 0) diag++;
+    else { s_lo++; diag--; }
+  }
+  return abs(new_diag - diag) <= OIC_SHIFT_SLACK;
+}
+
+// Add_Overlap (Process_String_Overlaps.C:175-244), executed by lane 0 on the LDS copy
+__device__ void add_overlap(const SearchConst &C, int32_t s_lo, int32_t s_hi, int32_t t_lo, int32_t t_hi, double qual,
+                            int32_t delta_ct, OlapDev *olap, int32_t *ct) {
+  if (!C.partial) {
+    const int32_t new_diag = t_lo - s_lo;
+    for (int32_t i = 0; i < *ct; i++) {
+      const int32_t old_diag = olap[i].t_lo - olap[i].s_lo;
+      if ((new_diag > 0 && old_diag > 0 && olap[i].trb - new_diag - olap[i].slb >= OIC_MIN_INTER) ||
+          (new_diag <= 0 && old_diag <= 0 && olap[i].srb + new_diag - olap[i].tlb >= OIC_MIN_INTER)) {
+        if (new_diag < olap[i].min_diag) olap[i].min_diag = new_diag;
+        if (new_diag > olap[i].max_diag) olap[i].max_diag = new_diag;
+        if (s_lo < olap[i].slb) olap[i].slb = s_lo;
+        if (s_hi > olap[i].srb) olap[i].srb = s_hi;
+        if (t_lo < olap[i].tlb) olap[i].tlb = t_lo;
+        if (t_hi > olap[i].trb) olap[i].trb = t_hi;
+        if (qual < olap[i].quality) {
+          olap[i].s_lo = s_lo; olap[i].s_hi = s_hi; olap[i].t_lo = t_lo; olap[i].t_hi = t_hi;
+          olap[i].quality = qual;
+          olap[i].delta_ct = delta_ct;
+        }
+        return;
+      }
+    }
+  }
+  if (*ct >= OIC_MAX_DISTINCT) return;
+  OlapDev &o = olap[*ct];
+  o.s_lo = o.slb = s_lo; o.s_hi = o.srb = s_hi; o.t_lo = o.tlb = t_lo; o.t_hi = o.trb = t_hi;
+  o.quality = qual;
+  o.delta_ct = delta_ct;
+  o.min_diag = o.max_diag = t_lo - s_lo;
+  (*ct)++;
+}
+
+__device__ __forceinline__ uint32_t encode_evalue(double q) {
+  return (q < (65535 / 100000.0)) ? (uint32_t)(100000.0 * q + 0.5) : 65535u;
+}
+
+struct OutRec { uint32_t a_iid, b_iid; uint64_t w0, w1; };
+
+__device__ __forceinline__ uint64_t pack_w0(uint32_t ahg5, uint32_t ahg3, uint32_t ev, uint32_t fl, uint32_t obt, uint32_t dup, uint32_t utg) {
+  const uint64_t F = 0x1fffffull;
+  return ((uint64_t)ahg5 & F) | (((uint64_t)ahg3 & F) << 21) | (((uint64_t)ev & 0xffff) << 42) | ((uint64_t)(fl & 1) << 58) |
+         ((uint64_t)(obt & 1) << 59) | ((uint64_t)(dup & 1) << 60) | ((uint64_t)(utg & 1) << 61);
+}
+__device__ __forceinline__ uint64_t pack_w1(uint32_t bhg5, uint32_t bhg3, uint32_t span) {
+  const uint64_t F = 0x1fffffull;
+  return ((uint64_t)bhg5 & F) | (((uint64_t)bhg3 & F) << 21) | (((uint64_t)span & F) << 42);
+}
+
+// Output_Overlap (overlapInCore-Output.C:26-191); returns 1 if counted as contained
+__device__ int output_overlap(uint32_t S_ID, int32_t S_Len, int32_t S_Dir, uint32_t T_ID, int32_t T_Len, const OlapDev &o, OutRec *rec) {
+  const uint32_t span = (uint32_t)((o.s_hi - o.s_lo) + (o.t_hi - o.t_lo) + o.delta_ct) / 2u;
+  const int32_t S_Right_Hang = S_Len - o.s_hi - 1;
+  const int32_t T_Right_Hang = T_Len - o.t_hi - 1;
+  const bool Sleft = (o.s_lo > o.t_lo) || ((o.s_lo == o.t_lo) && (S_Right_Hang > T_Right_Hang));
+  rec->a_iid = Sleft ? S_ID : T_ID;
+  rec->b_iid = Sleft ? T_ID : S_ID;
+  char orient;
+  int32_t ahg, bhg;
+  if (Sleft) { orient = (S_Dir == 0) ? 'N' : 'O'; ahg = o.s_lo; bhg = T_Right_Hang - S_Right_Hang; }
+  else       { orient = (S_Dir == 0) ? 'N' : 'I'; ahg = o.t_lo; bhg = S_Right_Hang - T_Right_Hang; }
+  if (orient == 'O' && S_Right_Hang >= T_Right_Hang) {
+    orient = 'I';
+    ahg = -(T_Right_Hang - S_Right_Hang);
+    bhg = -(o.s_lo);
+  }
+  int32_t ah, bh;
+  uint32_t fl;
+  if (orient == 'N')      { ah = ahg;  bh = bhg;  fl = 0; }
+  else if (orient == 'I') { ah = ahg;  bh = bhg;  fl = 1; }
+  else                    { ah = -bhg; bh = -ahg; fl = 1; }
+  const uint32_t ahg5 = ah < 0 ? 0u : (uint32_t)ah, bhg5 = ah < 0 ? (uint32_t)(-ah) : 0u;
+  const uint32_t bhg3 = bh < 0 ? 0u : (uint32_t)bh, ahg3 = bh < 0 ? (uint32_t)(-bh) : 0u;
+  rec->w0 = pack_w0(ahg5, ahg3, encode_evalue(o.quality), fl, 0, 0, 1);
+  rec->w1 = pack_w1(bhg5, bhg3, span);
+  return bhg <= 0;
+}
+
+// Output_Partial_Overlap (overlapInCore-Output.C:195-264)
+__device__ void output_partial(uint32_t s_id, uint32_t t_id, int32_t dir, const OlapDev &o, int32_t s_len, int32_t t_len, OutRec *rec) {
+  const uint32_t span = (uint32_t)((o.s_hi - o.s_lo) + (o.t_hi - o.t_lo) + o.delta_ct) / 2u;
+  rec->a_iid = s_id;
+  rec->b_iid = t_id;
+  uint32_t ahg5, ahg3, bhg5, bhg3, fl;
+  if (dir == 0) { ahg5 = o.s_lo; ahg3 = s_len - (o.s_hi + 1); bhg5 = o.t_lo; bhg3 = t_len - (o.t_hi + 1); fl = 0; }
+  else          { ahg5 = s_len - (o.s_hi + 1); ahg3 = o.s_lo; bhg5 = t_len - (o.t_hi + 1); bhg3 = o.t_lo; fl = 1; }
+  rec->w0 = pack_w0(ahg5, ahg3, encode_evalue(o.quality), fl, 1, 1, 0);
+  rec->w1 = pack_w1(bhg5, bhg3, span);
+}
+
+// per-pair outcome, also the trace the cost model learns from
+struct PairOut {
+  int32_t  nout;          // records written (<= 3)
+  int32_t  n_ext;         // Extend_Alignment calls
+  uint64_t cells;         // DP cells executed for this pair
+  int32_t  status;        // 0 ok, 1 overflow (retry with a wider tile), 2 hopeless, 3 skipped by --minkmers
+  int32_t  exit_d;        // first extension: diagonal at the right end
+  uint32_t contained, dovetail;
+};
+
+#define STAT_WITHOUT 0
+#define STAT_WITH    1
+#define STAT_MULTI   2
+#define STAT_TOTAL   3
+#define STAT_CONT    4
+#define STAT_DOVE    5
+#define STAT_CELLS   8
+#define STAT_SLOTS   9
+#define STAT_EXT     10
+#define STAT_DPCALLS 11
+#define STAT_NSTATS  16
+
+struct PairKernelArgs {
+  ReadsDev R;
+  SearchConst C;
+  PairRec *pairs;
+  const uint32_t *order;     // processing order (pair indices)
+  uint64_t norder;
+  int4 *nodes;               // per pair node list (x Offset, y Len, z Start, w removed)
+  OutRec *recs;              // [np * 3]
+  PairOut *pout;             // [np]
+  uint32_t *queue;           // work counter
+  unsigned long long *stats; // [STAT_NSTATS]
+  DPScratch sc0;             // scratch of wave 0; wave w adds w * stride
+  int64_t row_stride, val_stride, delta_stride;
+};
+
+template <int CAP>
+__device__ __forceinline__ DPScratch wave_scratch(const PairKernelArgs &A, uint32_t w) {
+  DPScratch s = A.sc0;
+  s.rowL = A.sc0.rowL + (int64_t)w * A.row_stride;
+  s.rowR = A.sc0.rowR + (int64_t)w * A.row_stride;
+  s.rowOff = A.sc0.rowOff + (int64_t)w * A.row_stride;
+  s.rowVal = A.sc0.rowVal + (int64_t)w * A.val_stride;
+  s.dbuf = A.sc0.dbuf + (int64_t)w * A.delta_stride;
+  s.ldelta = A.sc0.ldelta + (int64_t)w * A.delta_stride;
+  s.rdelta = A.sc0.rdelta + (int64_t)w * A.delta_stride;
+  return s;
+}
+
+// Process_Matches (overlapInCore-Process_String_Overlaps.C:353-547) for one candidate pair per
+// wavefront.  Persistent: each wave pulls the next pair index from a global counter.
+template <int CAP>
+__global__ void __launch_bounds__(OIC_WAVE) k_process_pairs(PairKernelArgs A) {
+  __shared__ int32_t bufA[CAP], bufB[CAP];
+  __shared__ OlapDev olap[OIC_MAX_DISTINCT];
+  __shared__ int32_t olap_ct;
+  __shared__ uint32_t next_item;
+  const int lane = lane_id();
+  const uint32_t w = blockIdx.x;
+  const DPScratch sc = wave_scratch<CAP>(A, w);
+  const SearchConst &C = A.C;
+
+  uint64_t cells = 0, slots = 0;
+  uint32_t dpcalls = 0, exts = 0;
+  while (true) {
+    if (lane == 0) next_item = atomicAdd(A.queue, 1u);
+    __syncthreads();
+    const uint32_t qi = next_item;
+    __syncthreads();
+    if (qi >= A.norder) break;
+    const uint32_t p = A.order[qi];
+    const PairRec pr = A.pairs[p];
+    int4 *nodes = A.nodes + pr.hb;
+    const int32_t nn = pr.nnodes;
+    const uint32_t S_ID = pr.s_id, T_ID = pr.t_id;
+    const int32_t S_Len = A.R.len[S_ID], T_Len = A.R.len[T_ID];
+    const SeqV S = read_view(A.R, S_ID, pr.dir), Src = read_view(A.R, S_ID, 1 - pr.dir);
+    const SeqV T = read_view(A.R, T_ID, 0), Trc = read_view(A.R, T_ID, 1);
+    PairOut po = {};
+    const uint64_t cells0 = cells;
+
+    // --minkmers filter (Process_String_Overlaps.C:618-621); the threshold per diagonal span is a
+    // host-computed table (computeMinimumKmers, :22-33)
+    if (C.min_kmers) {
+      const uint64_t mk = C.min_kmers[pr.diag_end - pr.diag_bgn];
+      if (mk > (uint64_t)(int64_t)(pr.he - pr.hb)) {
+        if (lane == 0) { po.status = 3; A.pout[p] = po; }
+        continue;
+      }
+    }
+
+    // hopeless check for a singleton match (Process_String_Overlaps.C:384-415)
+    if (C.hopeless && nn == 1 && !C.partial) {
+      const int4 nd = nodes[0];
+      const int32_t s_head = nd.z, t_head = nd.x;
+      bool hopeless = false;
+      if (s_head <= t_head) { if (s_head > OIC_HOPELESS) hopeless = true; }
+      else                  { if (t_head > OIC_HOPELESS) hopeless = true; }
+      const int32_t s_tail = S_Len - s_head - nd.y + 1, t_tail = T_Len - t_head - nd.y + 1;
+      if (s_tail <= t_tail) { if (s_tail > OIC_HOPELESS) hopeless = true; }
+      else                  { if (t_tail > OIC_HOPELESS) hopeless = true; }
+      if (hopeless) {
+        if (lane == 0) {
+          po.status = 2;
+          A.pout[p] = po;
+          atomicAdd(&A.stats[STAT_WITHOUT], 1ull);
+        }
+        continue;
+      }
+    }
+
+    if (lane == 0) olap_ct = 0;
+    __syncthreads();
+    bool overflow = false;
+    bool first = true;
+    while (true) {
+      // first node with the maximal Len (Process_String_Overlaps.C:424-431)
+      int32_t bl = -1, bi = 0x7fffffff;
+      for (int32_t i = lane; i < nn; i += OIC_WAVE) {
+        const int4 nd = nodes[i];
+        if (!nd.w && nd.y > bl) { bl = nd.y; bi = i; }
+      }
+      wave_argmax_first(bl, bi);
+      if (bl < 0) break;
+      const int4 lm = nodes[bi];
+      ExtOut x = wave_extend<CAP>(lm.z, lm.x, lm.y, S, Src, S_Len, T, Trc, T_Len, C, bufA, bufB, sc, cells, slots, dpcalls);
+      exts++;
+      po.n_ext++;
+      if (x.overflow) { overflow = true; break; }
+      if (first) { po.exit_d = x.t_hi - x.s_hi; first = false; }
+      if (x.kind == 3 || C.partial) {
+        if (1 + x.s_hi - x.s_lo >= C.min_olap && 1 + x.t_hi - x.t_lo >= C.min_olap) {
+          const int32_t a = x.s_hi - x.s_lo, b = x.t_hi - x.t_lo;
+          const int32_t olen = 1 + (a < b ? a : b);
+          const double q = (double)x.errors / (double)olen;
+          if (x.errors <= C.tb.error_bound[olen]) {
+            if (lane == 0) add_overlap(C, x.s_lo, x.s_hi, x.t_lo, x.t_hi, q, x.ndelta, olap, &olap_ct);
+          }
+        }
+      }
+      __syncthreads();
+      if (pr.consistent) break;
+      // remove the longest match and, after a dovetail, every match on the alignment (:476-490)
+      if (lane == 0) nodes[bi].w = 1;
+      if (x.kind == 3 || C.partial) {
+        for (int32_t i = lane; i < nn; i += OIC_WAVE) {
+          const int4 nd = nodes[i];
+          if (nd.w || i == bi) continue;
+          if (x.s_lo - OIC_SHIFT_SLACK <= nd.z && nd.z + nd.y <= (x.s_hi + 1) + OIC_SHIFT_SLACK - 1 &&
+              lies_on_alignment(nd.z, nd.x, x.s_lo, x.t_lo, sc.ldelta, x.ndelta))
+            nodes[i].w = 1;
+        }
+      }
+      __threadfence_block();
+      __syncthreads();
+    }
+
+    if (overflow) {
+      // leave the node list untouched for the retry: clear removal marks
+      for (int32_t i = lane; i < nn; i += OIC_WAVE) nodes[i].w = 0;
+      if (lane == 0) { po.status = 1; A.pout[p] = po; }
+      __syncthreads();
+      continue;
+    }
+
+    if (lane == 0) {
+      const int32_t ct = olap_ct;
+      int32_t deleted[OIC_MAX_DISTINCT] = {0, 0, 0};
+      if (ct > 0) {
+        if (C.partial) {
+          if (C.unique) {   // Choose_Best_Partial (:289-311)
+            int32_t best = 0;
+            double mb0 = (1.0 - olap[0].quality) * (2 + olap[0].s_hi - olap[0].s_lo + olap[0].t_hi - olap[0].t_lo);
+            for (int32_t i = 1; i < ct; i++) {
+              const double mb = (1.0 - olap[i].quality) * (2 + olap[i].s_hi - olap[i].s_lo + olap[i].t_hi - olap[i].t_lo);
+              if (mb0 < mb || (mb0 == mb && olap[i].quality < olap[best].quality)) best = i;
+            }
+            for (int32_t i = 0; i < ct; i++) deleted[i] = (i != best);
+          }
+        } else if (C.unique) {   // Combine_Into_One_Olap (:42-96)
+          int32_t best = 0;
+          int32_t mnd = olap[0].min_diag, mxd = olap[0].max_diag, slb = olap[0].slb, srb = olap[0].srb, tlb = olap[0].tlb, trb = olap[0].trb;
+          for (int32_t i = 1; i < ct; i++) {
+            const int32_t si = olap[i].s_hi - olap[i].s_lo, ti = olap[i].t_hi - olap[i].t_lo;
+            const int32_t sb = olap[best].s_hi - olap[best].s_lo, tb2 = olap[best].t_hi - olap[best].t_lo;
+            const int32_t leni = 1 + (si < ti ? si : ti), lenb = 1 + (sb < tb2 ? sb : tb2);
+            if (olap[i].quality < olap[best].quality || (olap[i].quality == olap[best].quality && leni > lenb)) best = i;
+            if (olap[i].min_diag < mnd) mnd = olap[i].min_diag;
+            if (olap[i].max_diag > mxd) mxd = olap[i].max_diag;
+            if (olap[i].slb < slb) slb = olap[i].slb;
+            if (olap[i].srb > srb) srb = olap[i].srb;
+            if (olap[i].tlb < tlb) tlb = olap[i].tlb;
+            if (olap[i].trb > trb) trb = olap[i].trb;
+          }
+          olap[best].min_diag = mnd; olap[best].max_diag = mxd; olap[best].slb = slb; olap[best].srb = srb;
+          olap[best].tlb = tlb; olap[best].trb = trb;
+          for (int32_t i = 0; i < ct; i++) deleted[i] = (i != best);
+        } else {   // Merge_Intersecting_Olaps (:106-162)
+          for (int32_t i = 0; i < ct - 1; i++)
+            for (int32_t j = i + 1; j < ct; j++) {
+              if (deleted[i] || deleted[j]) continue;
+              const int32_t lo = olap[i].min_diag, hi = olap[i].max_diag;
+              if ((lo <= 0 && olap[j].min_diag > 0) || (lo > 0 && olap[j].min_diag <= 0)) continue;
+              if ((lo >= 0 && olap[j].trb - lo - olap[j].slb >= OIC_MIN_INTER) || (lo <= 0 && olap[j].srb + lo - olap[j].tlb >= OIC_MIN_INTER) ||
+                  (hi >= 0 && olap[j].trb - hi - olap[j].slb >= OIC_MIN_INTER) || (hi <= 0 && olap[j].srb + hi - olap[j].tlb >= OIC_MIN_INTER)) {
+                OlapDev *keep, *disc;
+                if (olap[i].quality < olap[j].quality) { keep = &olap[i]; disc = &olap[j]; deleted[j] = 1; }
+                else                                   { keep = &olap[j]; disc = &olap[i]; deleted[i] = 1; }
+                if (disc->min_diag < keep->min_diag) keep->min_diag = disc->min_diag;
+                if (disc->max_diag > keep->max_diag) keep->max_diag = disc->max_diag;
+                if (disc->slb < keep->slb) keep->slb = disc->slb;
+                if (disc->srb > keep->srb) keep->srb = disc->srb;
+                if (disc->tlb < keep->tlb) keep->tlb = disc->tlb;
+                if (disc->trb > keep->trb) keep->trb = disc->trb;
+              }
+            }
+        }
+      }
+      int32_t nout = 0;
+      uint32_t cont = 0, dove = 0;
+      for (int32_t i = 0; i < ct; i++) {
+        if (deleted[i]) continue;
+        OutRec rr;
+        if (C.partial) output_partial(S_ID, T_ID, pr.dir, olap[i], S_Len, T_Len, &rr);
+        else {
+          if (output_overlap(S_ID, S_Len, pr.dir, T_ID, T_Len, olap[i], &rr)) cont++; else dove++;
+        }
+        A.recs[(uint64_t)p * OIC_MAX_DISTINCT + nout] = rr;
+        nout++;
+      }
+      po.nout = nout;
+      po.contained = cont;
+      po.dovetail = dove;
+      po.cells = cells - cells0;
+      A.pout[p] = po;
+      if (nout == 0) atomicAdd(&A.stats[STAT_WITHOUT], 1ull);
+      else {
+        atomicAdd(&A.stats[STAT_WITH], 1ull);
+        if (nout > 1) atomicAdd(&A.stats[STAT_MULTI], 1ull);
+      }
+      atomicAdd(&A.stats[STAT_TOTAL], (unsigned long long)nout);
+      if (!C.partial) {
+        atomicAdd(&A.stats[STAT_CONT], (unsigned long long)cont);
+        atomicAdd(&A.stats[STAT_DOVE], (unsigned long long)dove);
+      }
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
+    atomicAdd(&A.stats[STAT_CELLS], (unsigned long long)cells);
+    atomicAdd(&A.stats[STAT_SLOTS], (unsigned long long)slots);
+    atomicAdd(&A.stats[STAT_EXT], (unsigned long long)exts);
+    atomicAdd(&A.stats[STAT_DPCALLS], (unsigned long long)dpcalls);
+  }
+}
+
+// record compaction: pairs' records into a dense array at scanned offsets
+__global__ void k_compact_recs(const OutRec *__restrict__ recs, const PairOut *__restrict__ pout, const uint64_t *__restrict__ off,
+                               uint64_t np, OutRec *__restrict__ out) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int32_t n = pout[p].nout;
+  for (int32_t i = 0; i < n; i++) out[off[p] + i] = recs[p * OIC_MAX_DISTINCT + i];
+}
+
+__global__ void k_nout(const PairOut *__restrict__ pout, uint64_t np, uint32_t *__restrict__ n) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < np) n[p] = (uint32_t)pout[p].nout;
+}

@@ -1,0 +1,210 @@
+"""Python host mirror of the overlapInCore C ABI (include/oic.h, liboic_hip.so).
+
+The engine runs on the GPU only: if the HIP library is missing or no device is visible,
+constructing an Engine raises.  Names follow canu's overlapInCore surface:
+
+    Engine(kmer=22, maxerate=0.045, minlength=500, ...)   # oicParameters / argv (overlapInCore.C:293-411)
+    eng.load_reads(reads)                                  # sqStore / sqCache reads
+    eng.build_hash_index(bgn, end) -> last loaded ID       # Build_Hash_Index
+    eng.search(bgn_ref, end_ref) -> Result                 # Process_Overlaps over a ref range
+    eng.run(...) -> Result                                 # OverlapDriver: hash-block loop
+    eng.extend_alignment(tasks)                            # prefixEditDistance::Extend_Alignment
+    eng.forward(tasks) / eng.reverse(tasks)                # prefixEditDistance::forward / ::reverse
+"""
+import ctypes
+import dataclasses
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboic_hip.so")
+
+OIC_OK, OIC_EINVAL, OIC_EHIP, OIC_ENOMEM, OIC_ESTATE, OIC_ECAPACITY, OIC_EUNSUPPORTED = 0, -1, -2, -3, -4, -5, -6
+SCHED_PLAIN, SCHED_BATCHED, SCHED_LEARNED = 0, 1, 2
+
+EXPORTS = [
+    "oic_params_default", "oic_create", "oic_destroy", "oic_last_error", "oic_load_reads", "oic_index_build",
+    "oic_search", "oic_run", "oic_get_counters", "oic_extend_batch", "oic_prefix_dp_batch", "oic_tables",
+]
+
+RECORD_DTYPE = np.dtype([("a_iid", "<u4"), ("b_iid", "<u4"), ("w0", "<u8"), ("w1", "<u8")])
+
+
+class OicError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"oic error {code}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("kmer", ctypes.c_uint32), ("max_erate", ctypes.c_double), ("align_noise", ctypes.c_double),
+        ("min_olap", ctypes.c_int32), ("partial", ctypes.c_int32), ("unique", ctypes.c_int32),
+        ("hopeless", ctypes.c_int32), ("minkmers", ctypes.c_int32), ("frag_olap_limit", ctypes.c_uint64),
+        ("hash_bits", ctypes.c_uint32), ("hash_load", ctypes.c_double), ("hash_data_len", ctypes.c_uint64),
+        ("n_threads", ctypes.c_uint32), ("device", ctypes.c_int32), ("schedule", ctypes.c_int32),
+    ]
+
+
+class ReadsT(ctypes.Structure):
+    _fields_ = [("bases", ctypes.c_void_p), ("start", ctypes.c_void_p), ("len", ctypes.c_void_p), ("nreads", ctypes.c_uint32)]
+
+
+class Sink(ctypes.Structure):
+    _fields_ = [("records", ctypes.c_void_p), ("capacity", ctypes.c_uint64), ("count", ctypes.c_uint64),
+                ("records_needed", ctypes.c_uint64), ("stats", ctypes.c_uint64 * 8)]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("kmer_probes", "hits", "pairs", "nodes", "extensions", "dp_calls",
+                                               "dp_cells", "dp_lane_slots", "overflow_retries")] + \
+               [(n, ctypes.c_double) for n in ("ms_index", "ms_gather", "ms_lists", "ms_extend", "ms_total", "ms_dp_kernel")]
+
+
+EXT_TASK = np.dtype([("s_id", "<u4"), ("t_id", "<u4"), ("s_dir", "<i4"), ("start", "<i4"), ("offset", "<i4"), ("len", "<i4")])
+EXT_RESULT = np.dtype([("kind", "<i4"), ("s_lo", "<i4"), ("s_hi", "<i4"), ("t_lo", "<i4"), ("t_hi", "<i4"),
+                       ("errors", "<i4"), ("delta_len", "<i4"), ("status", "<i4")])
+DP_TASK = np.dtype([("a_id", "<u4"), ("b_id", "<u4"), ("a_strand", "<i4"), ("b_strand", "<i4"), ("a_pos", "<i4"),
+                    ("b_pos", "<i4"), ("m", "<i4"), ("n", "<i4"), ("error_limit", "<i4"), ("dir", "<i4")])
+DP_RESULT = np.dtype([("errors", "<i4"), ("a_end", "<i4"), ("t_end", "<i4"), ("match_to_end", "<i4"),
+                      ("leftover", "<i4"), ("delta_len", "<i4"), ("status", "<i4"), ("pad", "<i4"), ("cells", "<u8")])
+
+_lib = None
+
+
+def lib():
+    """Load liboic_hip.so (fails loudly when it is missing)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.oic_params_default.argtypes = [ctypes.POINTER(Params)]
+        L.oic_create.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(vp)]
+        L.oic_destroy.argtypes = [vp]
+        L.oic_last_error.restype = ctypes.c_char_p
+        L.oic_last_error.argtypes = [vp]
+        L.oic_load_reads.argtypes = [vp, ctypes.POINTER(ReadsT)]
+        L.oic_index_build.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        L.oic_search.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Sink)]
+        L.oic_run.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Sink)]
+        L.oic_get_counters.argtypes = [vp, ctypes.POINTER(Counters)]
+        L.oic_extend_batch.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32]
+        L.oic_prefix_dp_batch.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32]
+        L.oic_tables.argtypes = [vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
+                                 ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)), ctypes.POINTER(ctypes.c_uint32)]
+        _lib = L
+    return _lib
+
+
+@dataclasses.dataclass
+class Result:
+    records: np.ndarray           # RECORD_DTYPE, in output order
+    stats: list                   # the 8 .stats counters
+    counters: dict
+
+    def stats_text(self):
+        """The .stats file overlapInCore writes (overlapInCore.C:550-558)."""
+        names = [" Kmer hits without olaps", "    Kmer hits with olaps", "  Multiple overlaps/pair",
+                 " Total overlaps produced", "      Contained overlaps", "       Dovetail overlaps",
+                 "Rejected by short window", " Rejected by long window"]
+        return "".join(f"{n} = {v}\n" for n, v in zip(names, self.stats))
+
+
+class Engine:
+    def __init__(self, kmer=22, maxerate=0.06, minlength=0, alignnoise=1.0, partial=False, unique=True,
+                 hopeless=True, minkmers=False, hashbits=22, hashload=0.6, hashdatalen=100000000, threads=1,
+                 device=0, schedule=SCHED_PLAIN):
+        L = lib()
+        p = Params()
+        L.oic_params_default(ctypes.byref(p))
+        p.kmer, p.max_erate, p.align_noise, p.min_olap = kmer, maxerate, alignnoise, minlength
+        p.partial, p.unique, p.hopeless, p.minkmers = int(partial), int(unique), int(hopeless), int(minkmers)
+        p.hash_bits, p.hash_load, p.hash_data_len, p.n_threads = hashbits, hashload, hashdatalen, threads
+        p.device, p.schedule = device, schedule
+        self._ctx = ctypes.c_void_p()
+        rc = L.oic_create(ctypes.byref(p), ctypes.byref(self._ctx))
+        if rc != 0:
+            raise OicError(rc, "oic_create failed (no HIP device, or invalid parameters)")
+        self.params = p
+        self._reads = None
+
+    def close(self):
+        if self._ctx:
+            lib().oic_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise OicError(rc, lib().oic_last_error(self._ctx).decode())
+
+    def load_reads(self, reads):
+        self._reads = (np.ascontiguousarray(reads.bases), np.ascontiguousarray(reads.start, np.uint64),
+                       np.ascontiguousarray(reads.length, np.uint32))
+        r = ReadsT(self._reads[0].ctypes.data, self._reads[1].ctypes.data, self._reads[2].ctypes.data, int(reads.n))
+        self._check(lib().oic_load_reads(self._ctx, ctypes.byref(r)))
+        self.nreads = int(reads.n)
+
+    def build_hash_index(self, bgn, end):
+        last = ctypes.c_uint32()
+        self._check(lib().oic_index_build(self._ctx, bgn, end, ctypes.byref(last)))
+        return last.value
+
+    def _collect(self, call):
+        cap = 1 << 16
+        while True:
+            buf = np.zeros(cap, RECORD_DTYPE)
+            s = Sink(buf.ctypes.data, cap, 0, 0)
+            rc = call(s)
+            if rc == OIC_ECAPACITY or s.count > cap:
+                cap = int(max(s.records_needed, s.count)) + 1024
+                continue
+            self._check(rc)
+            return Result(buf[:s.count].copy(), list(s.stats), self.counters())
+
+    def search(self, bgn_ref, end_ref):
+        return self._collect(lambda s: lib().oic_search(self._ctx, bgn_ref, end_ref, ctypes.byref(s)))
+
+    def run(self, bgn_hash=1, end_hash=0xFFFFFFFF, bgn_ref=1, end_ref=0xFFFFFFFF):
+        return self._collect(lambda s: lib().oic_run(self._ctx, bgn_hash, end_hash, bgn_ref, end_ref, ctypes.byref(s)))
+
+    def counters(self):
+        c = Counters()
+        self._check(lib().oic_get_counters(self._ctx, ctypes.byref(c)))
+        return {n: getattr(c, n) for n, _ in Counters._fields_}
+
+    def extend_alignment(self, tasks, delta_stride=4096):
+        tasks = np.ascontiguousarray(tasks, EXT_TASK)
+        res = np.zeros(len(tasks), EXT_RESULT)
+        delta = np.zeros((len(tasks), delta_stride), np.int32)
+        self._check(lib().oic_extend_batch(self._ctx, tasks.ctypes.data, len(tasks), res.ctypes.data, delta.ctypes.data, delta_stride))
+        return res, delta
+
+    def prefix_dp(self, tasks, delta_stride=4096):
+        tasks = np.ascontiguousarray(tasks, DP_TASK)
+        res = np.zeros(len(tasks), DP_RESULT)
+        delta = np.zeros((len(tasks), delta_stride), np.int32)
+        self._check(lib().oic_prefix_dp_batch(self._ctx, tasks.ctypes.data, len(tasks), res.ctypes.data, delta.ctypes.data, delta_stride))
+        return res, delta
+
+    def tables(self):
+        eb = ctypes.POINTER(ctypes.c_int32)()
+        ml = ctypes.POINTER(ctypes.c_int32)()
+        me = ctypes.c_uint32()
+        self._check(lib().oic_tables(self._ctx, ctypes.byref(eb), ctypes.byref(ml), ctypes.byref(me)))
+        return (np.ctypeslib.as_array(eb, ((1 << 21),)).copy(), np.ctypeslib.as_array(ml, (me.value,)).copy())
+
+
+def sort_records(recs):
+    """Canonical order of ovOverlap::operator< (ovOverlap.H:263-277)."""
+    if len(recs) == 0:
+        return recs
+    return recs[np.lexsort((recs["w1"], recs["w0"], recs["b_iid"], recs["a_iid"]))]

@@ -1,0 +1,191 @@
+/*
+ * oic.h -- C ABI of the MI355X overlapInCore engine (liboic_hip.so).
+ *
+ * Plain C types only.  One host thread per context; a context owns its HIP stream and all device
+ * memory; calls are not re-entrant.  Every entry point returns 0 on success and a negative
+ * OIC_E* code on failure (never exits); oic_last_error() describes the last failure.
+ *
+ * Read IDs are 1-based, as in canu's sqStore.  Reads are passed as one byte array
+ * (ACGTN, any case) with per-read start offsets and lengths; index 0 of start/len is unused.
+ *
+ * Reference interfaces replaced (paths relative to canu's src/):
+ *   oic_create / oic_destroy   oicParameters + OverlapDriver's setup and teardown
+ *                              (overlapInCore/overlapInCore.H:367-473, overlapInCore.C:161-277, :283-527)
+ *   oic_load_reads             sqStore / sqCache reads of the hash and ref ranges
+ *                              (overlapInCore-Build_Hash_Index.C:504-545, overlapInCore-Process_Overlaps.C:48-67,
+ *                               stores/sqCache.C:262-318)
+ *   oic_index_build            Build_Hash_Index (overlapInCore-Build_Hash_Index.C:414-631, overlapInCore.H:513-514)
+ *   oic_search                 Process_Overlaps -> Find_Overlaps -> Process_String_Olaps -> Process_Matches
+ *                              -> Output_Overlap over a ref range (overlapInCore-Process_Overlaps.C:24-123,
+ *                              overlapInCore-Find_Overlaps.C:234-336, overlapInCore-Process_String_Overlaps.C:353-690,
+ *                              overlapInCore-Output.C:26-264)
+ *   oic_run                    OverlapDriver's hash-block loop (overlapInCore.C:204-263)
+ *   oic_extend_batch           prefixEditDistance::Extend_Alignment (liboverlap/prefixEditDistance.H:93-98,
+ *                              prefixEditDistance-extend.C:36-183)
+ *   oic_prefix_dp_batch        prefixEditDistance::forward / ::reverse (prefixEditDistance.H:75-90,
+ *                              prefixEditDistance-forward.C:94-314, prefixEditDistance-reverse.C:114-330)
+ *   oic_tables                 prefixEditDistance constructor tables (prefixEditDistance.C:23-88,
+ *                              Binomial_Bound.C:104-195)
+ */
+#ifndef OIC_H
+#define OIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OIC_OK            0
+#define OIC_EINVAL       -1   /* bad argument */
+#define OIC_EHIP         -2   /* HIP runtime error */
+#define OIC_ENOMEM       -3   /* device allocation failed */
+#define OIC_ESTATE       -4   /* call out of order (e.g. search before index) */
+#define OIC_ECAPACITY    -5   /* caller buffer too small; *_needed reports the size */
+#define OIC_EUNSUPPORTED -6   /* option the engine does not implement */
+
+typedef struct oic_ctx oic_ctx;
+
+/* overlapInCore command-line surface (overlapInCore.C:293-411).  Rates are given as the decimal
+ * the user typed; the engine rounds them to float as strtof() does in the reference. */
+typedef struct oic_params {
+  uint32_t kmer;             /* -k */
+  double   max_erate;        /* --maxerate */
+  double   align_noise;      /* --alignnoise (1.0 default) */
+  int32_t  min_olap;         /* --minlength */
+  int32_t  partial;          /* -partial */
+  int32_t  unique;           /* -u (1, default) / -m (0) */
+  int32_t  hopeless;         /* 1 unless -z (forced 0 when max_erate > 0.06) */
+  int32_t  minkmers;         /* --minkmers */
+  uint64_t frag_olap_limit;  /* -l (UINT64_MAX = unlimited; the only supported value) */
+  uint32_t hash_bits;        /* --hashbits (22) */
+  double   hash_load;        /* --hashload (0.6) */
+  uint64_t hash_data_len;    /* --hashdatalen (100000000) */
+  uint32_t n_threads;        /* -t: only its effect on which ref reads are searched is reproduced */
+  int32_t  device;           /* HIP device ordinal */
+  int32_t  schedule;         /* OIC_SCHED_* */
+} oic_params;
+
+#define OIC_SCHED_PLAIN    0  /* one candidate pair per wavefront, pairs in discovery order */
+#define OIC_SCHED_BATCHED  1  /* binned, load-balanced persistent launches (static length bins) */
+#define OIC_SCHED_LEARNED  2  /* as BATCHED, pairs binned and ordered by the learned cost model */
+
+void oic_params_default(oic_params *p);
+
+typedef struct oic_reads {
+  const char     *bases;   /* all reads, back to back */
+  const uint64_t *start;   /* [nreads + 1], start[i] = offset of read i */
+  const uint32_t *len;     /* [nreads + 1] */
+  uint32_t        nreads;
+} oic_reads;
+
+/* ovOverlap as ovFile stores it (ovStoreFile.C:253-283, ovOverlap.H:49-78): the two 64-bit
+ * ovOverlapDAT words hold ahg5:21 ahg3:21 evalue:16 flipped forOBT forDUP forUTG spare:2 and
+ * bhg5:21 bhg3:21 span:21 spare:1, least significant bit first. */
+typedef struct oic_overlap {
+  uint32_t a_iid, b_iid;
+  uint64_t dat[2];
+} oic_overlap;
+
+/* Caller-owned output sink.  If the records do not fit, the call fails with OIC_ECAPACITY,
+ * sets records_needed, and may be repeated with a larger buffer. */
+typedef struct oic_sink {
+  oic_overlap *records;
+  uint64_t     capacity;
+  uint64_t     count;           /* out */
+  uint64_t     records_needed;  /* out */
+  /* .stats counters, overlapInCore.C:550-558 order: kmer hits without olaps, with olaps,
+   * multiple overlaps/pair, total overlaps, contained, dovetail, short window, long window */
+  uint64_t     stats[8];        /* out, accumulated */
+} oic_sink;
+
+/* work counters of the last oic_search / oic_run (accumulated over blocks) */
+typedef struct oic_counters {
+  uint64_t kmer_probes;        /* S windows looked up */
+  uint64_t hits;               /* seed hits with S_ID < T_ID */
+  uint64_t pairs;              /* candidate (S, orientation, T) pairs */
+  uint64_t nodes;              /* match nodes */
+  uint64_t extensions;         /* Extend_Alignment calls */
+  uint64_t dp_calls;           /* forward / reverse DP runs executed */
+  uint64_t dp_cells;           /* (e, d) cell updates executed, row-0 slide counted as one */
+  uint64_t dp_lane_slots;      /* lane-cells issued (cells plus idle lanes of partial chunks) */
+  uint64_t overflow_retries;   /* DP tasks re-run with a wider band tile */
+  double   ms_index, ms_gather, ms_lists, ms_extend, ms_total;
+  double   ms_dp_kernel;       /* device time of the extension kernels (HIP events) */
+} oic_counters;
+
+int         oic_create(const oic_params *p, oic_ctx **out);
+void        oic_destroy(oic_ctx *ctx);
+const char *oic_last_error(const oic_ctx *ctx);
+
+/* Upload and 2-bit pack reads 1..nreads (both strands) into device memory. */
+int oic_load_reads(oic_ctx *ctx, const oic_reads *reads);
+
+/* Build_Hash_Index over hash reads [bgn_id, end_id]: stops at the reference's data-length and
+ * hash-load limits; *last_loaded receives the last read ID the block covers. */
+int oic_index_build(oic_ctx *ctx, uint32_t bgn_id, uint32_t end_id, uint32_t *last_loaded);
+
+/* Search ref reads [bgn_ref, end_ref] (both orientations) against the current index and append
+ * the overlaps to sink.  Applies the reference's -t chunking rule for which ref reads are run. */
+int oic_search(oic_ctx *ctx, uint32_t bgn_ref, uint32_t end_ref, oic_sink *sink);
+
+/* The whole overlapInCore job: hash-block loop over [bgn_hash, end_hash] x [bgn_ref, end_ref]. */
+int oic_run(oic_ctx *ctx, uint32_t bgn_hash, uint32_t end_hash, uint32_t bgn_ref, uint32_t end_ref,
+            oic_sink *sink);
+
+int oic_get_counters(const oic_ctx *ctx, oic_counters *out);
+
+/* ---- DP units ---- */
+
+/* One Extend_Alignment call: seed Match_Node (Start on S, Offset on T, Len) between read s_id in
+ * orientation s_dir (0 forward, 1 reverse complement) and read t_id (forward), both loaded. */
+typedef struct oic_ext_task {
+  uint32_t s_id, t_id;
+  int32_t  s_dir;
+  int32_t  start, offset, len;
+} oic_ext_task;
+
+typedef struct oic_ext_result {
+  int32_t kind;                 /* Overlap_t: 0 NONE, 1 LEFT_BRANCH_PT, 2 RIGHT_BRANCH_PT, 3 DOVETAIL */
+  int32_t s_lo, s_hi, t_lo, t_hi, errors;
+  int32_t delta_len;            /* merged Left_Delta length */
+  int32_t status;               /* 0 ok, else OIC_E* */
+} oic_ext_result;
+
+/* delta (optional): delta_stride ints per task receive the merged Left_Delta. */
+int oic_extend_batch(oic_ctx *ctx, const oic_ext_task *tasks, uint32_t n, oic_ext_result *results,
+                     int32_t *delta, uint32_t delta_stride);
+
+/* One prefixEditDistance::forward (dir 0) or ::reverse (dir 1) call on explicit windows.
+ * Forward compares a[a_pos ..] with b[b_pos ..] rightwards; reverse compares a[a_pos], a[a_pos-1], ...
+ * with b[b_pos], b[b_pos-1], ... as the reference's negative indexing does.  Windows are
+ * (read id, strand 0/1, position, length) of loaded reads; m <= n is required. */
+typedef struct oic_dp_task {
+  uint32_t a_id, b_id;
+  int32_t  a_strand, b_strand;
+  int32_t  a_pos, b_pos;
+  int32_t  m, n;
+  int32_t  error_limit;
+  int32_t  dir;
+} oic_dp_task;
+
+typedef struct oic_dp_result {
+  int32_t errors, a_end, t_end, match_to_end, leftover, delta_len;
+  int32_t status;
+  int32_t pad;
+  uint64_t cells;
+} oic_dp_result;
+
+int oic_prefix_dp_batch(oic_ctx *ctx, const oic_dp_task *tasks, uint32_t n, oic_dp_result *results,
+                        int32_t *delta, uint32_t delta_stride);
+
+/* Host copies of the DP tables the context uses. error_bound has 2^21 entries,
+ * match_limit has max_errors entries. */
+int oic_tables(const oic_ctx *ctx, const int32_t **error_bound, const int32_t **match_limit,
+               uint32_t *max_errors);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1638 @@
+/*
+ * oic_oracle.c -- CPU restatement of canu's overlapInCore hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, and the timed CPU baseline "port").
+ * Nothing under canu_amd/ links, loads or calls this file.
+ *
+ * Restated from (paths relative to canu's tree):
+ *   src/overlapInCore/liboverlap/Binomial_Bound.C          tables
+ *   src/overlapInCore/liboverlap/prefixEditDistance*.C/H   DP, deltas, Extend_Alignment
+ *   src/overlapInCore/overlapInCore-Build_Hash_Index.C     seed index semantics
+ *   src/overlapInCore/overlapInCore-Find_Overlaps.C        Add_Ref / Add_Match / Find_Overlaps
+ *   src/overlapInCore/overlapInCore-Process_String_Overlaps.C
+ *   src/overlapInCore/overlapInCore-Output.C               ovOverlap canonicalisation
+ *   src/overlapInCore/overlapInCore.C                      driver (block loop, chunk quirk)
+ *   src/stores/ovOverlap.H                                 record layout, evalue codec
+ *
+ * Compile with -ffp-contract=off: the branch-point score and slope are double expressions
+ * whose rounding decides branch points.
+ */
+#include "oic_oracle.h"
+
+#include <assert.h>
+#include <ctype.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+#include <time.h>
+
+#define Sign(a) (((a) > 0) - ((a) < 0))          /* prefixEditDistance.H:31 */
+
+/* overlapInCore.H constants */
+#define HOPELESS_MATCH        90                  /* overlapInCore.H:78 */
+#define MAX_DISTINCT_OLAPS    3                   /* overlapInCore.H:107 */
+#define MIN_INTERSECTION      10                  /* overlapInCore.H:120 */
+#define SHIFT_SLACK           1                   /* overlapInCore.H:146 */
+#define STRING_OLAP_SHIFT     8                   /* overlapInCore.H:149 */
+#define STRING_OLAP_MODULUS   (1 << STRING_OLAP_SHIFT)
+#define STRING_OLAP_MASK      (STRING_OLAP_MODULUS - 1)
+#define ENTRIES_PER_BUCKET    21                  /* overlapInCore.H:58 */
+
+enum { NONE = 0, LEFT_BRANCH_PT = 1, RIGHT_BRANCH_PT = 2, DOVETAIL = 3 };   /* prefixEditDistance.H:35-40 */
+enum { FORWARD = 0, REVERSE = 1 };                                       /* overlapInCore.H:189-192 */
+
+static double now_s(void) {
+  struct timespec ts; clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+double oo_cli_rate(double v) { return (double)(float)v; }
+
+/* ======================================================================================
+ * Tables
+ * ====================================================================================== */
+
+#define NORMAL_DISTRIB_THOLD  3.62        /* Binomial_Bound.C:24 */
+#define EDIT_DIST_PROB_BOUND  1e-4        /* Binomial_Bound.C:28 */
+
+/* Binomial_Bound.C:35-98: smallest n >= Start with P[>= e errors in n trials] > 1e-4, using the
+ * exact binomial sum for n <= 35 and the normal shortcut / Poisson sum above. */
+int oo_binomial_bound(int e, double p, int Start) {
+  double Normal_Z, Mu_Power, Factorial, Poisson_Coeff;
+  double q, Sum, P_Power, Q_Power, X;
+  int k, n, Bin_Coeff, Ct;
+
+  q = 1.0 - p;
+  if (Start < e)
+    Start = e;
+
+  for (n = Start; n < (int)OO_MAX_READLEN; n++) {
+    if (n <= 35) {
+      Sum = 0.0;
+      Bin_Coeff = 1;
+      Ct = 0;
+      P_Power = 1.0;
+      Q_Power = pow(q, n);
+
+      for (k = 0; k < e && 1.0 - Sum > EDIT_DIST_PROB_BOUND; k++) {
+        X = Bin_Coeff * P_Power * Q_Power;
+        Sum += X;
+        Bin_Coeff *= n - Ct;
+        Bin_Coeff /= ++Ct;
+        P_Power *= p;
+        Q_Power /= q;
+      }
+
+      if (1.0 - Sum > EDIT_DIST_PROB_BOUND)
+        return n;
+    } else {
+      Normal_Z = (e - 0.5 - n * p) / sqrt(n * p * q);
+      if (Normal_Z <= NORMAL_DISTRIB_THOLD)
+        return n;
+
+      Sum = 0.0;
+      Mu_Power = 1.0;
+      Factorial = 1.0;
+      Poisson_Coeff = exp(-n * p);
+      for (k = 0; k < e; k++) {
+        Sum += Mu_Power * Poisson_Coeff / Factorial;
+        Mu_Power *= n * p;
+        Factorial *= k + 1;
+      }
+
+      if (1.0 - Sum > EDIT_DIST_PROB_BOUND)
+        return n;
+    }
+  }
+  return (int)OO_MAX_READLEN;
+}
+
+/* Binomial_Bound.C:103-195 (AS_MAX_READLEN_BITS == 21 slope, :159-161) */
+void oo_init_match_limit(int32_t *ml, double maxErate, int32_t maxErrors) {
+  int32_t e = 0, s = 1;
+  int32_t l = maxErrors < 2000 ? maxErrors : 2000;
+  int32_t ERRORS_FOR_FREE = 1;
+
+  while (e <= ERRORS_FOR_FREE)
+    ml[e++] = 0;
+
+  while (e < l) {
+    s = oo_binomial_bound(e - ERRORS_FOR_FREE, maxErate, s);
+    ml[e] = s - 1;
+    assert(ml[e] >= ml[e - 1]);
+    e++;
+  }
+
+  double sl = 0.982064188397525 / maxErate + 0.067835741959926;
+  double vl = ml[e - 1] + sl;
+
+  while (e < maxErrors) {
+    ml[e] = (int32_t)ceil(vl);
+    vl += sl;
+    e++;
+  }
+}
+
+/* prefixEditDistance.C:23-88 */
+oo_tables *oo_tables_new(double maxErate, double alignNoise, int partial) {
+  oo_tables *tb = (oo_tables *)calloc(1, sizeof(oo_tables));
+  tb->maxErate              = maxErate;
+  tb->alignNoise            = alignNoise;
+  tb->partial               = partial;
+  tb->MAX_ERRORS            = 1 + (int)ceil(maxErate * OO_MAX_READLEN);
+  tb->MIN_BRANCH_END_DIST   = 20;
+  tb->MIN_BRANCH_TAIL_SLOPE = (maxErate > 0.06) ? 1.0 : 0.20;
+  tb->Edit_Match_Limit      = (int32_t *)calloc(tb->MAX_ERRORS + 1, sizeof(int32_t));
+  /* overlapInCore.C:138-140 passes maxErate * alignNoise as the match-limit rate */
+  oo_init_match_limit(tb->Edit_Match_Limit, maxErate * alignNoise, tb->MAX_ERRORS);
+  tb->Error_Bound = (int32_t *)malloc(sizeof(int32_t) * (OO_MAX_READLEN + 1));
+  for (uint32_t i = 0; i <= OO_MAX_READLEN; i++)
+    tb->Error_Bound[i] = (int32_t)ceil(i * maxErate);
+  tb->Branch_Match_Value = maxErate / (1 + maxErate);
+  return tb;
+}
+
+void oo_tables_free(oo_tables *tb) {
+  if (!tb) return;
+  free(tb->Edit_Match_Limit);
+  free(tb->Error_Bound);
+  free(tb);
+}
+
+/* ======================================================================================
+ * DP workspace.  Edit_Array_Lazy[e] spans [-2-e, 2+e] (prefixEditDistance-allocateMoreSpace.C:51-54);
+ * rows are stored back to back, row e starting at e*e + 4*e, so EAL(e,d) = base[e*e+5*e+2+d].
+ * ====================================================================================== */
+
+struct oo_dp {
+  const oo_tables *tb;
+  int32_t  *eal;
+  int64_t   eal_rows;          /* rows 0 .. eal_rows-1 allocated */
+  int32_t  *Left_Delta, *Right_Delta, *Delta_Stack;
+  int32_t   Left_Delta_Len, Right_Delta_Len;
+  uint64_t  cells, bases, calls, rows;
+  int       poison;
+  int       poison_hits;
+};
+
+#define POISON_VAL ((int32_t)0x7f7f7f7f)
+
+static inline int64_t eal_index(int64_t e, int64_t d) { return e * e + 5 * e + 2 + d; }
+
+static void ensure_rows(oo_dp *W, int64_t e) {
+  if (e < W->eal_rows) return;
+  int64_t rows = W->eal_rows ? W->eal_rows : 64;
+  while (rows <= e) rows *= 2;
+  int64_t need = rows * rows + 4 * rows;
+  W->eal = (int32_t *)realloc(W->eal, sizeof(int32_t) * need);
+  if (!W->eal) { fprintf(stderr, "oracle: out of memory for %lld DP rows\n", (long long)rows); abort(); }
+  W->eal_rows = rows;
+}
+
+static inline int32_t EALget(oo_dp *W, int64_t e, int64_t d) {
+  int32_t v = W->eal[eal_index(e, d)];
+  if (W->poison && v == POISON_VAL) W->poison_hits++;
+  return v;
+}
+static inline void EALset(oo_dp *W, int64_t e, int64_t d, int32_t v) { W->eal[eal_index(e, d)] = v; }
+
+static void poison_row(oo_dp *W, int64_t e) {
+  if (!W->poison) return;
+  for (int64_t d = -2 - e; d <= 2 + e; d++) EALset(W, e, d, POISON_VAL);
+}
+
+oo_dp *oo_dp_new(const oo_tables *tb) {
+  oo_dp *W = (oo_dp *)calloc(1, sizeof(oo_dp));
+  W->tb = tb;
+  W->Left_Delta  = (int32_t *)malloc(sizeof(int32_t) * tb->MAX_ERRORS);
+  W->Right_Delta = (int32_t *)malloc(sizeof(int32_t) * tb->MAX_ERRORS);
+  W->Delta_Stack = (int32_t *)malloc(sizeof(int32_t) * tb->MAX_ERRORS);
+  ensure_rows(W, 64);
+  return W;
+}
+
+void oo_dp_free(oo_dp *W) {
+  if (!W) return;
+  free(W->eal); free(W->Left_Delta); free(W->Right_Delta); free(W->Delta_Stack);
+  free(W);
+}
+
+void oo_dp_set_poison(oo_dp *W, int on) { W->poison = on; W->poison_hits = 0; }
+int  oo_dp_poison_hits(const oo_dp *W) { return W->poison_hits; }
+
+int32_t oo_dp_left_delta(const oo_dp *W, int32_t *out, int32_t cap) {
+  int32_t n = W->Left_Delta_Len < cap ? W->Left_Delta_Len : cap;
+  if (out) memcpy(out, W->Left_Delta, sizeof(int32_t) * n);
+  return W->Left_Delta_Len;
+}
+int32_t oo_dp_right_delta(const oo_dp *W, int32_t *out, int32_t cap) {
+  int32_t n = W->Right_Delta_Len < cap ? W->Right_Delta_Len : cap;
+  if (out) memcpy(out, W->Right_Delta, sizeof(int32_t) * n);
+  return W->Right_Delta_Len;
+}
+void oo_dp_counters(const oo_dp *W, uint64_t out[4]) {
+  out[0] = W->cells; out[1] = W->bases; out[2] = W->calls; out[3] = W->rows;
+}
+void oo_dp_reset_counters(oo_dp *W) { W->cells = W->bases = W->calls = W->rows = 0; }
+
+/* pruneAlign, original method (prefixEditDistance.H:249-257) */
+static inline int pruneAlign(oo_dp *W, int32_t e, int32_t i, int32_t extra) {
+  return EALget(W, e, i) + extra < W->tb->Edit_Match_Limit[e];
+}
+
+/* prefixEditDistance-forward.C:31-71 */
+static void Set_Right_Delta(oo_dp *W, int e, int d) {
+  int from, last, max;
+  int i, j, k;
+
+  last = EALget(W, e, d);
+  W->Right_Delta_Len = 0;
+
+  for (k = e; k > 0; k--) {
+    from = d;
+    max = 1 + EALget(W, k - 1, d);
+    if ((j = EALget(W, k - 1, d - 1)) > max) { from = d - 1; max = j; }
+    if ((j = 1 + EALget(W, k - 1, d + 1)) > max) { from = d + 1; max = j; }
+    if (from == d - 1) {
+      W->Delta_Stack[W->Right_Delta_Len++] = max - last - 1;
+      d--;
+      last = EALget(W, k - 1, from);
+    } else if (from == d + 1) {
+      W->Delta_Stack[W->Right_Delta_Len++] = last - (max - 1);
+      d++;
+      last = EALget(W, k - 1, from);
+    }
+  }
+  W->Delta_Stack[W->Right_Delta_Len++] = last + 1;
+
+  k = 0;
+  for (i = W->Right_Delta_Len - 1; i > 0; i--)
+    W->Right_Delta[k++] = abs(W->Delta_Stack[i]) * Sign(W->Delta_Stack[i - 1]);
+  W->Right_Delta_Len--;
+}
+
+/* prefixEditDistance-forward.C:93-314 */
+int32_t oo_forward(oo_dp *W, const char *A, int32_t m, const char *T, int32_t n,
+                   int32_t Error_Limit, int32_t *A_End, int32_t *T_End, int32_t *Match_To_End) {
+  const oo_tables *tb = W->tb;
+  double Score;
+  int Max_Score_Len = 0, Max_Score_Best_d = 0, Max_Score_Best_e = 0;
+  int Best_d, Best_e, Longest, Row;
+  int d, e, j;
+
+  assert(m <= n);
+  Best_d = Best_e = Longest = 0;
+  W->Right_Delta_Len = 0;
+  W->calls++;
+
+  for (Row = 0; Row < m && (A[Row] == T[Row] || A[Row] == 'n' || T[Row] == 'n'); Row++)
+    ;
+  W->cells++; W->bases += Row + 1; W->rows++;
+
+  poison_row(W, 0);
+  EALset(W, 0, 0, Row);
+
+  if (Row == m) {
+    *A_End = *T_End = m;
+    *Match_To_End = 1;
+    return 0;
+  }
+
+  int32_t Left = 0, Right = 0;
+  double Max_Score = 0.0;
+
+  for (e = 1; e <= Error_Limit; e++) {
+    ensure_rows(W, e);
+    poison_row(W, e);
+    W->rows++;
+
+    Left  = (Left - 1 > -e) ? Left - 1 : -e;
+    Right = (Right + 1 < e) ? Right + 1 : e;
+
+    EALset(W, e - 1, Left,      -2);
+    EALset(W, e - 1, Left - 1,  -2);
+    EALset(W, e - 1, Right,     -2);
+    EALset(W, e - 1, Right + 1, -2);
+
+    for (d = Left; d <= Right; d++) {
+      Row = 1 + EALget(W, e - 1, d);
+      if ((j = EALget(W, e - 1, d - 1)) > Row)     Row = j;
+      if ((j = 1 + EALget(W, e - 1, d + 1)) > Row) Row = j;
+
+      int r0 = Row;
+      while (Row < m && Row + d < n && (A[Row] == T[Row + d] || A[Row] == 'n' || T[Row + d] == 'n'))
+        Row++;
+      W->cells++; W->bases += (uint64_t)(Row - r0) + 1;
+
+      EALset(W, e, d, Row);
+
+      if (Row == m || Row + d == n) {
+        Score = Row * tb->Branch_Match_Value - e;
+        int32_t Tail_Len = Row - Max_Score_Len;
+        int abort_ = 0;
+        double slope = (double)(Max_Score - Score) / Tail_Len;
+
+        if (tb->partial && (Score < Max_Score))
+          abort_ = 1;
+
+        /* e and Tail_Len are compared against a uint32 (prefixEditDistance.H:286) */
+        if (((uint32_t)e > tb->MIN_BRANCH_END_DIST / 2) &&
+            ((uint32_t)Tail_Len >= tb->MIN_BRANCH_END_DIST) &&
+            (slope >= tb->MIN_BRANCH_TAIL_SLOPE))
+          abort_ = 1;
+
+        if (abort_) {
+          *A_End = Max_Score_Len;
+          *T_End = Max_Score_Len + Max_Score_Best_d;
+          Set_Right_Delta(W, Max_Score_Best_e, Max_Score_Best_d);
+          *Match_To_End = 0;
+          return Max_Score_Best_e;
+        }
+
+        /* Force last error to be mismatch rather than insertion (forward.C:215-221) */
+        if ((Row == m) && (1 + EALget(W, e - 1, d + 1) == EALget(W, e, d)) && (d < Right)) {
+          d++;
+          EALset(W, e, d, EALget(W, e, d - 1));
+        }
+
+        *A_End = Row;
+        *T_End = Row + d;
+        Set_Right_Delta(W, e, d);
+        *Match_To_End = 1;
+        return e;
+      }
+    }
+
+    while ((Left <= Right) && (Left < 0) && pruneAlign(W, e, Left, 0))
+      Left++;
+    if (Left >= 0)
+      while ((Left <= Right) && pruneAlign(W, e, Left, Left))
+        Left++;
+    if (Left > Right)
+      break;
+    while ((Right > 0) && pruneAlign(W, e, Right, Right))
+      Right--;
+    if (Right <= 0)
+      while (pruneAlign(W, e, Right, 0))
+        Right--;
+    assert(Left <= Right);
+
+    for (d = Left; d <= Right; d++)
+      if (EALget(W, e, d) > Longest) {
+        Best_d = d;
+        Best_e = e;
+        Longest = EALget(W, e, d);
+      }
+
+    Score = Longest * tb->Branch_Match_Value - e;
+    if (Score > Max_Score) {
+      Max_Score = Score;
+      Max_Score_Len = Longest;
+      Max_Score_Best_d = Best_d;
+      Max_Score_Best_e = Best_e;
+    }
+  }
+
+  *A_End = Max_Score_Len;
+  *T_End = Max_Score_Len + Max_Score_Best_d;
+  Set_Right_Delta(W, Max_Score_Best_e, Max_Score_Best_d);
+  *Match_To_End = 0;
+  return Max_Score_Best_e;
+}
+
+/* prefixEditDistance-reverse.C:35-93 */
+static void Set_Left_Delta(oo_dp *W, int32_t e, int32_t d, int32_t *leftover, int32_t *t_end, int32_t t_len) {
+  int from, last, max;
+  int j, k;
+
+  last = EALget(W, e, d);
+  W->Left_Delta_Len = 0;
+
+  for (k = e; k > 0; k--) {
+    from = d;
+    max = 1 + EALget(W, k - 1, d);
+    if ((j = EALget(W, k - 1, d - 1)) > max) { from = d - 1; max = j; }
+    if ((j = 1 + EALget(W, k - 1, d + 1)) > max) { from = d + 1; max = j; }
+    if (from == d - 1) {
+      W->Left_Delta[W->Left_Delta_Len++] = max - last - 1;
+      d--;
+      last = EALget(W, k - 1, from);
+    } else if (from == d + 1) {
+      W->Left_Delta[W->Left_Delta_Len++] = last - (max - 1);
+      d++;
+      last = EALget(W, k - 1, from);
+    }
+  }
+  *leftover = last;
+
+  assert(W->Left_Delta_Len == 0 || W->Left_Delta[0] != -1);
+
+  if (W->Left_Delta_Len > 1 && W->Left_Delta[0] == 1 && *t_end + t_len > 0) {
+    int i;
+    if (W->Left_Delta[1] > 0)
+      W->Left_Delta[0] = W->Left_Delta[1] + 1;
+    else
+      W->Left_Delta[0] = W->Left_Delta[1] - 1;
+    for (i = 2; i < W->Left_Delta_Len; i++)
+      W->Left_Delta[i - 1] = W->Left_Delta[i];
+    W->Left_Delta_Len--;
+    (*t_end)--;
+    if (W->Left_Delta_Len == 0)
+      (*leftover)++;
+  }
+}
+
+/* prefixEditDistance-reverse.C:113-330 */
+int32_t oo_reverse(oo_dp *W, const char *A, int32_t m, const char *T, int32_t n,
+                   int32_t Error_Limit, int32_t *A_End, int32_t *T_End, int32_t *Leftover,
+                   int32_t *Match_To_End) {
+  const oo_tables *tb = W->tb;
+  double Score;
+  int Max_Score_Len = 0, Max_Score_Best_d = 0, Max_Score_Best_e = 0;
+  int Best_d, Best_e, Longest, Row;
+  int d, e, j;
+
+  assert(m <= n);
+  Best_d = Best_e = Longest = 0;
+  W->Left_Delta_Len = 0;
+  W->calls++;
+
+  for (Row = 0; Row < m && (A[-Row] == T[-Row] || A[-Row] == 'n' || T[-Row] == 'n'); Row++)
+    ;
+  W->cells++; W->bases += Row + 1; W->rows++;
+
+  poison_row(W, 0);
+  EALset(W, 0, 0, Row);
+
+  if (Row == m) {
+    *A_End = *T_End = -m;
+    *Leftover = m;
+    *Match_To_End = 1;
+    return 0;
+  }
+
+  int32_t Left = 0, Right = 0;
+  double Max_Score = 0.0;
+
+  for (e = 1; e <= Error_Limit; e++) {
+    ensure_rows(W, e);
+    poison_row(W, e);
+    W->rows++;
+
+    Left  = (Left - 1 > -e) ? Left - 1 : -e;
+    Right = (Right + 1 < e) ? Right + 1 : e;
+
+    EALset(W, e - 1, Left,      -2);
+    EALset(W, e - 1, Left - 1,  -2);
+    EALset(W, e - 1, Right,     -2);
+    EALset(W, e - 1, Right + 1, -2);
+
+    for (d = Left; d <= Right; d++) {
+      Row = 1 + EALget(W, e - 1, d);
+      if ((j = EALget(W, e - 1, d - 1)) > Row)     Row = j;
+      if ((j = 1 + EALget(W, e - 1, d + 1)) > Row) Row = j;
+
+      int r0 = Row;
+      while (Row < m && Row + d < n && (A[-Row] == T[-Row - d] || A[-Row] == 'n' || T[-Row - d] == 'n'))
+        Row++;
+      W->cells++; W->bases += (uint64_t)(Row - r0) + 1;
+
+      EALset(W, e, d, Row);
+
+      if (Row == m || Row + d == n) {
+        Score = Row * tb->Branch_Match_Value - e;
+        int32_t Tail_Len = Row - Max_Score_Len;
+        int abort_ = 0;
+        double slope = (double)(Max_Score - Score) / Tail_Len;
+
+        if (tb->partial && (Score < Max_Score))
+          abort_ = 1;
+
+        if (((uint32_t)e > tb->MIN_BRANCH_END_DIST / 2) &&
+            ((uint32_t)Tail_Len >= tb->MIN_BRANCH_END_DIST) &&
+            (slope >= tb->MIN_BRANCH_TAIL_SLOPE))
+          abort_ = 1;
+
+        if (abort_) {
+          *A_End = -Max_Score_Len;
+          *T_End = -Max_Score_Len - Max_Score_Best_d;
+          Set_Left_Delta(W, Max_Score_Best_e, Max_Score_Best_d, Leftover, T_End, n);
+          *Match_To_End = 0;
+          return Max_Score_Best_e;
+        }
+
+        *A_End = -Row;
+        *T_End = -Row - d;
+        Set_Left_Delta(W, e, d, Leftover, T_End, n);
+        *Match_To_End = 1;
+        return e;
+      }
+    }
+
+    while ((Left <= Right) && (Left < 0) && pruneAlign(W, e, Left, 0))
+      Left++;
+    if (Left >= 0)
+      while ((Left <= Right) && pruneAlign(W, e, Left, Left))
+        Left++;
+    if (Left > Right)
+      break;
+    while ((Right > 0) && pruneAlign(W, e, Right, Right))
+      Right--;
+    if (Right <= 0)
+      while (pruneAlign(W, e, Right, 0))
+        Right--;
+    assert(Left <= Right);
+
+    for (d = Left; d <= Right; d++)
+      if (EALget(W, e, d) > Longest) {
+        Best_d = d;
+        Best_e = e;
+        Longest = EALget(W, e, d);
+      }
+
+    Score = Longest * tb->Branch_Match_Value - e;
+    if (Score > Max_Score) {
+      Max_Score = Score;
+      Max_Score_Len = Longest;
+      Max_Score_Best_d = Best_d;
+      Max_Score_Best_e = Best_e;
+    }
+  }
+
+  *A_End = -Max_Score_Len;
+  *T_End = -Max_Score_Len - Max_Score_Best_d;
+  Set_Left_Delta(W, Max_Score_Best_e, Max_Score_Best_d, Leftover, T_End, n);
+  *Match_To_End = 0;
+  return Max_Score_Best_e;
+}
+
+/* prefixEditDistance-extend.C:35-183 */
+int32_t oo_extend(oo_dp *W, const oo_match *Match,
+                  const char *S, uint32_t S_ID, int32_t S_Len,
+                  const char *T, uint32_t T_ID, int32_t T_Len,
+                  int32_t *S_Lo, int32_t *S_Hi, int32_t *T_Lo, int32_t *T_Hi, int32_t *Errors) {
+  (void)S_ID; (void)T_ID;
+  int32_t Right_Errors = 0, Left_Errors = 0, Leftover = 0;
+  int32_t rMatchToEnd = 1, lMatchToEnd = 1;
+
+  int32_t S_Left_Begin  = Match->Start - 1;
+  int32_t S_Right_Begin = Match->Start + Match->Len;
+  int32_t S_Right_Len   = S_Len - S_Right_Begin;
+
+  int32_t T_Left_Begin  = Match->Offset - 1;
+  int32_t T_Right_Begin = Match->Offset + Match->Len;
+  int32_t T_Right_Len   = T_Len - T_Right_Begin;
+
+  int32_t mn1 = Match->Start < Match->Offset ? Match->Start : Match->Offset;
+  int32_t mn2 = S_Right_Len < T_Right_Len ? S_Right_Len : T_Right_Len;
+  int32_t Total_Olap  = mn1 + Match->Len + mn2;
+  int32_t Error_Limit = W->tb->Error_Bound[Total_Olap];
+
+  W->Left_Delta_Len  = 0;
+  W->Right_Delta_Len = 0;
+
+  if ((S_Right_Len == 0) || (T_Right_Len == 0)) {
+    *S_Hi = 0;
+    *T_Hi = 0;
+    rMatchToEnd = 1;
+  } else if (S_Right_Len <= T_Right_Len) {
+    Right_Errors = oo_forward(W, S + S_Right_Begin, S_Right_Len, T + T_Right_Begin, T_Right_Len,
+                              Error_Limit, S_Hi, T_Hi, &rMatchToEnd);
+    for (int32_t i = 0; i < W->Right_Delta_Len; i++)
+      W->Right_Delta[i] *= -1;
+  } else {
+    Right_Errors = oo_forward(W, T + T_Right_Begin, T_Right_Len, S + S_Right_Begin, S_Right_Len,
+                              Error_Limit, T_Hi, S_Hi, &rMatchToEnd);
+  }
+
+  *S_Hi += S_Right_Begin - 1;
+  *T_Hi += T_Right_Begin - 1;
+
+  if ((S_Left_Begin < 0) || (T_Left_Begin < 0)) {
+    *S_Lo = 0;
+    *T_Lo = 0;
+    lMatchToEnd = 1;
+  } else if (S_Right_Begin <= T_Right_Begin) {
+    Left_Errors = oo_reverse(W, S + S_Left_Begin, S_Left_Begin + 1, T + T_Left_Begin, T_Left_Begin + 1,
+                             Error_Limit - Right_Errors, S_Lo, T_Lo, &Leftover, &lMatchToEnd);
+  } else {
+    Left_Errors = oo_reverse(W, T + T_Left_Begin, T_Left_Begin + 1, S + S_Left_Begin, S_Left_Begin + 1,
+                             Error_Limit - Right_Errors, T_Lo, S_Lo, &Leftover, &lMatchToEnd);
+    for (int32_t i = 0; i < W->Left_Delta_Len; i++)
+      W->Left_Delta[i] *= -1;
+  }
+
+  *S_Lo += S_Left_Begin + 1;
+  *T_Lo += T_Left_Begin + 1;
+
+  *Errors = Left_Errors + Right_Errors;
+
+  assert(Right_Errors <= Error_Limit);
+  assert(Left_Errors  <= Error_Limit);
+  assert(*Errors      <= Error_Limit);
+
+  int32_t return_type = (rMatchToEnd == 0) ? ((lMatchToEnd == 0) ? NONE : RIGHT_BRANCH_PT)
+                                           : ((lMatchToEnd == 0) ? LEFT_BRANCH_PT : DOVETAIL);
+
+  if (W->Right_Delta_Len > 0) {
+    if (W->Right_Delta[0] > 0)
+      W->Left_Delta[W->Left_Delta_Len++] = -(W->Right_Delta[0] + Leftover + Match->Len);
+    else
+      W->Left_Delta[W->Left_Delta_Len++] = -(W->Right_Delta[0] - Leftover - Match->Len);
+  }
+  for (int32_t i = 1; i < W->Right_Delta_Len; i++)
+    W->Left_Delta[W->Left_Delta_Len++] = -W->Right_Delta[i];
+  W->Right_Delta_Len = 0;
+
+  return return_type;
+}
+
+/* ======================================================================================
+ * Seed index.  The reference's open-addressing table of 21-entry buckets
+ * (Build_Hash_Index.C:267-404) makes every valid k-mer of every loaded hash read findable,
+ * with its occurrences chained newest-first: descending (T id, T offset) once coalesced
+ * (Build_Hash_Index.C:613-628).  Hash_Entries counts distinct k-mers and stops a block at
+ * Max_Hash_Load * 2^hashbits * 21 (Build_Hash_Index.C:445,504-506).  The restatement keeps
+ * those observable semantics with a plain hash map of distinct keys.
+ * ====================================================================================== */
+
+typedef struct kmap {
+  uint64_t *keys;    /* key+1, 0 = empty */
+  uint32_t *cnt;
+  uint64_t *first;   /* index into occ[] after layout */
+  uint64_t  cap, n;
+} kmap;
+
+static inline uint64_t kmix(uint64_t k) {
+  k ^= k >> 33; k *= 0xff51afd7ed558ccdULL; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL; k ^= k >> 33;
+  return k;
+}
+
+static void kmap_init(kmap *M, uint64_t cap) {
+  uint64_t c = 1024; while (c < cap) c <<= 1;
+  M->cap = c; M->n = 0;
+  M->keys  = (uint64_t *)calloc(c, sizeof(uint64_t));
+  M->cnt   = (uint32_t *)calloc(c, sizeof(uint32_t));
+  M->first = NULL;
+}
+static void kmap_free(kmap *M) { free(M->keys); free(M->cnt); free(M->first); memset(M, 0, sizeof(*M)); }
+
+static void kmap_grow(kmap *M);
+/* returns slot; *isnew set when inserted */
+static uint64_t kmap_insert(kmap *M, uint64_t key, int *isnew) {
+  if (2 * (M->n + 1) > M->cap) kmap_grow(M);
+  uint64_t mask = M->cap - 1, h = kmix(key) & mask;
+  while (M->keys[h] != 0) {
+    if (M->keys[h] == key + 1) { *isnew = 0; return h; }
+    h = (h + 1) & mask;
+  }
+  M->keys[h] = key + 1; M->n++; *isnew = 1;
+  return h;
+}
+static void kmap_grow(kmap *M) {
+  kmap N; kmap_init(&N, M->cap * 2);
+  for (uint64_t i = 0; i < M->cap; i++)
+    if (M->keys[i]) {
+      int nw; uint64_t h = kmap_insert(&N, M->keys[i] - 1, &nw);
+      N.cnt[h] = M->cnt[i];
+    }
+  free(M->keys); free(M->cnt); free(M->first);
+  *M = N;
+}
+static int64_t kmap_find(const kmap *M, uint64_t key) {
+  uint64_t mask = M->cap - 1, h = kmix(key) & mask;
+  while (M->keys[h] != 0) {
+    if (M->keys[h] == key + 1) return (int64_t)h;
+    h = (h + 1) & mask;
+  }
+  return -1;
+}
+
+static int8_t code_of[256];   /* Bit_Equivalent / Char_Is_Bad, overlapInCore.C:500-512 */
+static void init_codes(void) {
+  for (int i = 0; i < 256; i++) code_of[i] = -1;
+  code_of['a'] = code_of['A'] = 0; code_of['c'] = code_of['C'] = 1;
+  code_of['g'] = code_of['G'] = 2; code_of['t'] = code_of['T'] = 3;
+}
+
+typedef struct oic_index {
+  uint32_t  bgnID, endID;           /* hash reads in this block (IDs) */
+  uint32_t  nstr;                   /* String_Ct */
+  char     *lower;                  /* lowercased hash reads, NUL separated: basesData */
+  int64_t  *strStart;               /* String_Start (-1 = not loaded) */
+  int32_t  *strLen;                 /* String_Info.length */
+  kmap      map;
+  uint64_t *occ;                    /* (T_local << 32) | offset, per key descending */
+  uint64_t  nocc;
+} oic_index;
+
+static inline uint64_t occ_pack(uint32_t tl, uint32_t off) { return ((uint64_t)tl << 32) | off; }
+
+/* Build_Hash_Index.C:414-631 restated: returns the last hash ID considered (curID - 1). */
+static uint32_t build_index(const oo_params *P, const char *bases, const uint64_t *start,
+                            const uint32_t *len, uint32_t bgnID, uint32_t endID, oic_index *X) {
+  const uint32_t k = P->kmer;
+  uint64_t hashTableSize = (uint64_t)1 << P->hashBits;
+  uint64_t hash_entry_limit = (uint64_t)(P->hashLoad * hashTableSize * ENTRIES_PER_BUCKET);
+  uint64_t maxData = P->hashDataLen + OO_MAX_READLEN;    /* overlapInCore.C:454 */
+  const uint64_t kmask = (k == 32) ? ~0ULL : (((uint64_t)1 << (2 * k)) - 1);
+
+  memset(X, 0, sizeof(*X));
+  X->bgnID = bgnID;
+  uint32_t cap = endID - bgnID + 1;
+  X->strStart = (int64_t *)malloc(sizeof(int64_t) * cap);
+  X->strLen   = (int32_t *)calloc(cap, sizeof(int32_t));
+  uint64_t dataCap = 0;
+  for (uint32_t id = bgnID; id <= endID; id++) dataCap += len[id] + 1;
+  X->lower = (char *)malloc(dataCap + 1);
+  kmap_init(&X->map, 1 << 16);
+
+  uint64_t total_len = 0, nkmers = 0;
+  uint32_t curID, sc = 0;
+  for (curID = bgnID; (total_len < maxData) && (X->map.n < hash_entry_limit) && (curID <= endID); curID++, sc++) {
+    X->strStart[sc] = -1;
+    X->strLen[sc] = 0;
+    uint32_t L = len[curID];
+    if ((int64_t)L < P->minOlap)
+      continue;
+    X->strStart[sc] = (int64_t)total_len;
+    X->strLen[sc] = (int32_t)L;
+    const char *s = bases + start[curID];
+    char *dst = X->lower + total_len;
+    for (uint32_t i = 0; i < L; i++) dst[i] = (char)tolower((unsigned char)s[i]);
+    dst[L] = 0;
+    total_len += L + 1;
+    /* Put_String_In_Hash (Build_Hash_Index.C:330-404): every window whose k bases are all ACGT */
+    uint64_t key = 0; uint32_t valid = 0;
+    for (uint32_t i = 0; i < L; i++) {
+      int c = code_of[(unsigned char)dst[i]];
+      if (c < 0) { valid = 0; key = 0; continue; }
+      key = ((key << 2) | (uint64_t)c) & kmask;      /* forward-rolled big-endian key: any bijection works */
+      if (++valid >= k) {
+        int nw; uint64_t h = kmap_insert(&X->map, key, &nw);
+        X->map.cnt[h]++;
+        nkmers++;
+      }
+    }
+  }
+  X->nstr = sc;
+  X->endID = curID - 1;
+
+  /* lay out occurrence lists; fill ascending (T, off), then read each list backwards */
+  X->map.first = (uint64_t *)malloc(sizeof(uint64_t) * X->map.cap);
+  uint64_t acc = 0;
+  for (uint64_t h = 0; h < X->map.cap; h++) { X->map.first[h] = acc; acc += X->map.cnt[h]; }
+  X->nocc = acc;
+  X->occ = (uint64_t *)malloc(sizeof(uint64_t) * (acc ? acc : 1));
+  uint32_t *fill = (uint32_t *)calloc(X->map.cap, sizeof(uint32_t));
+  for (uint32_t t = 0; t < X->nstr; t++) {
+    if (X->strStart[t] < 0) continue;
+    const char *s = X->lower + X->strStart[t];
+    uint32_t L = (uint32_t)X->strLen[t];
+    uint64_t key = 0; uint32_t valid = 0;
+    for (uint32_t i = 0; i < L; i++) {
+      int c = code_of[(unsigned char)s[i]];
+      if (c < 0) { valid = 0; key = 0; continue; }
+      key = ((key << 2) | (uint64_t)c) & kmask;
+      if (++valid >= k) {
+        int64_t h = kmap_find(&X->map, key);
+        uint64_t pos = X->map.first[h] + X->map.cnt[h] - 1 - fill[h]++;   /* descending */
+        X->occ[pos] = occ_pack(t, i + 1 - k);
+      }
+    }
+  }
+  free(fill);
+  (void)nkmers;
+  return X->endID;
+}
+
+static void free_index(oic_index *X) {
+  free(X->strStart); free(X->strLen); free(X->lower); free(X->occ);
+  kmap_free(&X->map);
+  memset(X, 0, sizeof(*X));
+}
+
+/* ======================================================================================
+ * Search: Work_Area_t, Find_Overlaps, Process_String_Olaps, Process_Matches, Output
+ * ====================================================================================== */
+
+typedef struct olap_info {            /* Olap_Info_t, overlapInCore.H:207-216 */
+  int s_lo, s_hi, t_lo, t_hi;
+  double quality;
+  int32_t *delta;
+  int delta_ct;
+  int s_left_boundary, s_right_boundary, t_left_boundary, t_right_boundary;
+  int min_diag, max_diag;
+} olap_info;
+
+typedef struct string_olap {          /* String_Olap_t, overlapInCore.H:194-204 */
+  uint32_t String_Num;
+  int32_t  Match_List;
+  double   diag_sum;
+  int32_t  diag_ct;
+  int      diag_bgn, diag_end;
+  int32_t  Next;
+  int      Full, consistent;
+  int32_t  max_mult;                  /* trace only */
+} string_olap;
+
+typedef struct work_area {
+  string_olap *so;  int32_t so_size, so_next;
+  oo_match    *mn;  int32_t mn_size, mn_next;
+  int32_t      A_Olaps_For_Frag, B_Olaps_For_Frag;
+  int          left_end_screened, right_end_screened;
+  oo_dp       *ed;
+  olap_info    distinct[MAX_DISTINCT_OLAPS];
+  /* output */
+  oo_record   *rec; uint64_t nrec, caprec;
+  uint64_t     Total_Overlaps, Contained_Overlap_Ct, Dovetail_Overlap_Ct;
+  uint64_t     Kmer_Hits_Without_Olap_Ct, Kmer_Hits_With_Olap_Ct, Kmer_Hits_Skipped_Ct, Multi_Overlap_Ct;
+  uint64_t     extensions, hits, pairs;
+  oo_pair_trace *tr; uint64_t ntr, captr; int collect;
+} work_area;
+
+typedef struct search_ctx {
+  const oo_params *P;
+  const oo_tables *tb;
+  const oic_index *X;
+  uint64_t Filter_By_Kmer_Count;
+  int      hopeless;
+} search_ctx;
+
+static void wa_init(work_area *WA, const oo_tables *tb, int collect) {
+  memset(WA, 0, sizeof(*WA));
+  WA->so_size = 5000; WA->so = (string_olap *)calloc(WA->so_size, sizeof(string_olap));
+  WA->mn_size = 10000; WA->mn = (oo_match *)calloc(WA->mn_size, sizeof(oo_match));
+  WA->ed = oo_dp_new(tb);
+  for (int i = 0; i < MAX_DISTINCT_OLAPS; i++)
+    WA->distinct[i].delta = (int32_t *)malloc(sizeof(int32_t) * (tb->MAX_ERRORS + 1));
+  WA->caprec = 1024; WA->rec = (oo_record *)malloc(sizeof(oo_record) * WA->caprec);
+  WA->collect = collect;
+  if (collect) { WA->captr = 1024; WA->tr = (oo_pair_trace *)malloc(sizeof(oo_pair_trace) * WA->captr); }
+}
+
+static void wa_free(work_area *WA) {
+  free(WA->so); free(WA->mn); oo_dp_free(WA->ed);
+  for (int i = 0; i < MAX_DISTINCT_OLAPS; i++) free(WA->distinct[i].delta);
+  free(WA->rec); free(WA->tr);
+}
+
+/* Find_Overlaps.C:24-96 */
+static void Add_Match(const search_ctx *C, uint64_t toff, int32_t *start, int offset, int *consistent, work_area *WA) {
+  int32_t *p, save;
+  int diag = 0, new_diag, expected_start = 0, num_checked = 0;
+  int move_to_front = 0;
+  const int K = (int)C->P->kmer;
+
+  new_diag = (int)toff - offset;
+
+  for (p = start; (*p) != 0; p = &(WA->mn[*p].Next)) {
+    expected_start = WA->mn[*p].Start + WA->mn[*p].Len - K + 1;
+    diag = WA->mn[*p].Offset - WA->mn[*p].Start;
+    if (expected_start < offset)
+      break;
+    if (expected_start == offset) {
+      if (new_diag == diag) {
+        WA->mn[*p].Len += 1;
+        if (move_to_front) {
+          save = *p;
+          *p = WA->mn[*p].Next;
+          WA->mn[save].Next = *start;
+          *start = save;
+        }
+        return;
+      } else
+        move_to_front = 1;
+    }
+    num_checked++;
+  }
+
+  if (WA->mn_next == WA->mn_size) {
+    WA->mn_size *= 2;
+    WA->mn = (oo_match *)realloc(WA->mn, sizeof(oo_match) * WA->mn_size);
+    /* p may dangle now but is not used below */
+  }
+
+  if ((*start) != 0 && (num_checked > 0 || abs(diag - new_diag) > 3 || offset < expected_start + K - 2))
+    *consistent = 0;
+
+  save = *start;
+  *start = WA->mn_next;
+  WA->mn_next++;
+
+  WA->mn[*start].Offset = (int32_t)toff;
+  WA->mn[*start].Len    = K;
+  WA->mn[*start].Start  = offset;
+  WA->mn[*start].Next   = save;
+}
+
+/* Find_Overlaps.C:100-163.  The Match_List lives inside the String_Olap table, which may be
+ * reallocated below; Add_Match therefore works on a local copy of the list head. */
+static void Add_Ref(const search_ctx *C, uint32_t StrNum, uint32_t toff, int Offset, int32_t mult, work_area *WA) {
+  uint32_t Prev, Sub;
+  int consistent;
+
+  Sub = (StrNum ^ (StrNum >> STRING_OLAP_SHIFT)) & STRING_OLAP_MASK;
+
+  while (WA->so[Sub].Full && WA->so[Sub].String_Num != StrNum) {
+    Prev = Sub;
+    Sub = WA->so[Sub].Next;
+    if (Sub == 0) {
+      if (WA->so_next == WA->so_size) {
+        WA->so_size *= 2;
+        WA->so = (string_olap *)realloc(WA->so, sizeof(string_olap) * WA->so_size);
+      }
+      Sub = WA->so_next++;
+      WA->so[Prev].Next = Sub;
+      WA->so[Sub].Full = 0;
+      break;
+    }
+  }
+
+  if (!WA->so[Sub].Full) {
+    WA->so[Sub].String_Num = StrNum;
+    WA->so[Sub].Match_List = 0;
+    WA->so[Sub].diag_sum   = 0.0;
+    WA->so[Sub].diag_ct    = 0;
+    WA->so[Sub].diag_bgn   = OO_MAX_READLEN;
+    WA->so[Sub].diag_end   = 0;
+    WA->so[Sub].Next       = 0;
+    WA->so[Sub].Full       = 1;
+    WA->so[Sub].consistent = 1;
+    WA->so[Sub].max_mult   = 0;
+  }
+
+  consistent = WA->so[Sub].consistent;
+  WA->so[Sub].diag_sum += (double)toff - Offset;
+  WA->so[Sub].diag_ct++;
+  if (WA->so[Sub].diag_bgn > Offset) WA->so[Sub].diag_bgn = Offset;
+  if (WA->so[Sub].diag_end < Offset) WA->so[Sub].diag_end = Offset;
+  if (WA->so[Sub].max_mult < mult) WA->so[Sub].max_mult = mult;
+
+  int32_t head = WA->so[Sub].Match_List;
+  Add_Match(C, toff, &head, Offset, &consistent, WA);
+  WA->so[Sub].Match_List = head;
+  WA->so[Sub].consistent = consistent;
+  WA->hits++;
+}
+
+/* Process_String_Overlaps.C:42-96 */
+static void Combine_Into_One_Olap(olap_info olap[], int ct, int deleted[]) {
+  int min_diag, max_diag, s_left_boundary, s_right_boundary, t_left_boundary, t_right_boundary;
+  int i, best = 0;
+
+  min_diag = olap[0].min_diag; max_diag = olap[0].max_diag;
+  s_left_boundary = olap[0].s_left_boundary; s_right_boundary = olap[0].s_right_boundary;
+  t_left_boundary = olap[0].t_left_boundary; t_right_boundary = olap[0].t_right_boundary;
+
+  for (i = 1; i < ct; i++) {
+    int32_t si = olap[i].s_hi - olap[i].s_lo, ti = olap[i].t_hi - olap[i].t_lo;
+    int32_t sb = olap[best].s_hi - olap[best].s_lo, tbb = olap[best].t_hi - olap[best].t_lo;
+    int32_t leni = 1 + (si < ti ? si : ti);
+    int32_t lenb = 1 + (sb < tbb ? sb : tbb);
+
+    if ((olap[i].quality < olap[best].quality) ||
+        (olap[i].quality == olap[best].quality && leni > lenb))
+      best = i;
+    if (olap[i].min_diag < min_diag) min_diag = olap[i].min_diag;
+    if (olap[i].max_diag > max_diag) max_diag = olap[i].max_diag;
+    if (olap[i].s_left_boundary < s_left_boundary) s_left_boundary = olap[i].s_left_boundary;
+    if (olap[i].s_right_boundary > s_right_boundary) s_right_boundary = olap[i].s_right_boundary;
+    if (olap[i].t_left_boundary < t_left_boundary) t_left_boundary = olap[i].t_left_boundary;
+    if (olap[i].t_right_boundary > t_right_boundary) t_right_boundary = olap[i].t_right_boundary;
+  }
+
+  olap[best].min_diag = min_diag; olap[best].max_diag = max_diag;
+  olap[best].s_left_boundary = s_left_boundary; olap[best].s_right_boundary = s_right_boundary;
+  olap[best].t_left_boundary = t_left_boundary; olap[best].t_right_boundary = t_right_boundary;
+
+  for (i = 0; i < ct; i++)
+    deleted[i] = (i != best);
+}
+
+/* Process_String_Overlaps.C:106-162 */
+static void Merge_Intersecting_Olaps(olap_info p[], int ct, int deleted[]) {
+  int i, j, lo_diag, hi_diag;
+  for (i = 0; i < ct - 1; i++)
+    for (j = i + 1; j < ct; j++) {
+      if (deleted[i] || deleted[j])
+        continue;
+      lo_diag = p[i].min_diag;
+      hi_diag = p[i].max_diag;
+      if ((lo_diag <= 0 && p[j].min_diag > 0) || (lo_diag > 0 && p[j].min_diag <= 0))
+        continue;
+      if ((lo_diag >= 0 && p[j].t_right_boundary - lo_diag - p[j].s_left_boundary >= MIN_INTERSECTION) ||
+          (lo_diag <= 0 && p[j].s_right_boundary + lo_diag - p[j].t_left_boundary >= MIN_INTERSECTION) ||
+          (hi_diag >= 0 && p[j].t_right_boundary - hi_diag - p[j].s_left_boundary >= MIN_INTERSECTION) ||
+          (hi_diag <= 0 && p[j].s_right_boundary + hi_diag - p[j].t_left_boundary >= MIN_INTERSECTION)) {
+        olap_info *discard, *keep;
+        if (p[i].quality < p[j].quality) { keep = p + i; discard = p + j; deleted[j] = 1; }
+        else                             { keep = p + j; discard = p + i; deleted[i] = 1; }
+        if (discard->min_diag < keep->min_diag) keep->min_diag = discard->min_diag;
+        if (discard->max_diag > keep->max_diag) keep->max_diag = discard->max_diag;
+        if (discard->s_left_boundary < keep->s_left_boundary) keep->s_left_boundary = discard->s_left_boundary;
+        if (discard->s_right_boundary > keep->s_right_boundary) keep->s_right_boundary = discard->s_right_boundary;
+        if (discard->t_left_boundary < keep->t_left_boundary) keep->t_left_boundary = discard->t_left_boundary;
+        if (discard->t_right_boundary > keep->t_right_boundary) keep->t_right_boundary = discard->t_right_boundary;
+      }
+    }
+}
+
+/* Process_String_Overlaps.C:175-244 */
+static void Add_Overlap(const search_ctx *C, int s_lo, int s_hi, int t_lo, int t_hi, double qual,
+                        olap_info *olap, int *ct, work_area *WA) {
+  if (C->P->partial == 0) {
+    int32_t new_diag = t_lo - s_lo;
+    for (int32_t i = 0; i < *ct; i++) {
+      int32_t old_diag = olap[i].t_lo - olap[i].s_lo;
+      if ((new_diag > 0 && old_diag > 0 && olap[i].t_right_boundary - new_diag - olap[i].s_left_boundary >= MIN_INTERSECTION) ||
+          (new_diag <= 0 && old_diag <= 0 && olap[i].s_right_boundary + new_diag - olap[i].t_left_boundary >= MIN_INTERSECTION)) {
+        if (new_diag < olap[i].min_diag) olap[i].min_diag = new_diag;
+        if (new_diag > olap[i].max_diag) olap[i].max_diag = new_diag;
+        if (s_lo < olap[i].s_left_boundary)  olap[i].s_left_boundary  = s_lo;
+        if (s_hi > olap[i].s_right_boundary) olap[i].s_right_boundary = s_hi;
+        if (t_lo < olap[i].t_left_boundary)  olap[i].t_left_boundary  = t_lo;
+        if (t_hi > olap[i].t_right_boundary) olap[i].t_right_boundary = t_hi;
+        if (qual < olap[i].quality) {
+          olap[i].s_lo = s_lo; olap[i].s_hi = s_hi; olap[i].t_lo = t_lo; olap[i].t_hi = t_hi;
+          olap[i].quality = qual;
+          memcpy(olap[i].delta, WA->ed->Left_Delta, WA->ed->Left_Delta_Len * sizeof(int32_t));
+          olap[i].delta_ct = WA->ed->Left_Delta_Len;
+        }
+        return;
+      }
+    }
+  }
+
+  if (*ct >= MAX_DISTINCT_OLAPS)
+    return;
+
+  olap[*ct].s_lo = olap[*ct].s_left_boundary  = s_lo;
+  olap[*ct].s_hi = olap[*ct].s_right_boundary = s_hi;
+  olap[*ct].t_lo = olap[*ct].t_left_boundary  = t_lo;
+  olap[*ct].t_hi = olap[*ct].t_right_boundary = t_hi;
+  olap[*ct].quality = qual;
+  memcpy(olap[*ct].delta, WA->ed->Left_Delta, WA->ed->Left_Delta_Len * sizeof(int32_t));
+  olap[*ct].delta_ct = WA->ed->Left_Delta_Len;
+  olap[*ct].min_diag = t_lo - s_lo;
+  olap[*ct].max_diag = t_lo - s_lo;
+  (*ct)++;
+}
+
+/* Process_String_Overlaps.C:260-281 */
+static int Lies_On_Alignment(int start, int offset, int s_lo, int t_lo, work_area *WA) {
+  int i, diag, new_diag;
+  diag = t_lo - s_lo;
+  new_diag = offset - start;
+  for (i = 0; i < WA->ed->Left_Delta_Len; i++) {
+    s_lo += abs(WA->ed->Left_Delta[i]);
+    if (start < s_lo)
+      return (abs(new_diag - diag) <= SHIFT_SLACK);
+    if (WA->ed->Left_Delta[i] < 0)
+      diag++;
+    else {
+      s_lo++;
+      diag--;
+    }
+  }
+  return (abs(new_diag - diag) <= SHIFT_SLACK);
+}
+
+/* Process_String_Overlaps.C:289-311 */
+static void Choose_Best_Partial(olap_info *olap, int ct, int deleted[]) {
+  double matching_bases;
+  int i, best = 0;
+  matching_bases = (1.0 - olap[0].quality) * (2 + olap[0].s_hi - olap[0].s_lo + olap[0].t_hi - olap[0].t_lo);
+  for (i = 1; i < ct; i++) {
+    double mb = (1.0 - olap[i].quality) * (2 + olap[i].s_hi - olap[i].s_lo + olap[i].t_hi - olap[i].t_lo);
+    if (matching_bases < mb || (matching_bases == mb && olap[i].quality < olap[best].quality))
+      best = i;
+  }
+  for (i = 0; i < ct; i++)
+    deleted[i] = (i != best);
+}
+
+/* ovOverlap.H:31-35 */
+uint32_t oo_encode_evalue(double q) {
+  return (q < (65535 / 100000.0)) ? (uint32_t)(100000.0 * q + 0.5) : 65535u;
+}
+
+#define F21 0x1fffffULL
+static inline uint64_t w0_pack(uint32_t ahg5, uint32_t ahg3, uint32_t ev, int flipped, int obt, int dup, int utg) {
+  return ((uint64_t)ahg5 & F21) | (((uint64_t)ahg3 & F21) << 21) | (((uint64_t)ev & 0xffff) << 42) |
+         ((uint64_t)(flipped & 1) << 58) | ((uint64_t)(obt & 1) << 59) | ((uint64_t)(dup & 1) << 60) |
+         ((uint64_t)(utg & 1) << 61);
+}
+static inline uint64_t w1_pack(uint32_t bhg5, uint32_t bhg3, uint32_t span) {
+  return ((uint64_t)bhg5 & F21) | (((uint64_t)bhg3 & F21) << 21) | (((uint64_t)span & F21) << 42);
+}
+
+static oo_record *push_rec(work_area *WA) {
+  if (WA->nrec == WA->caprec) {
+    WA->caprec *= 2;
+    WA->rec = (oo_record *)realloc(WA->rec, sizeof(oo_record) * WA->caprec);
+  }
+  return WA->rec + WA->nrec++;
+}
+
+/* overlapInCore-Output.C:26-191 */
+static void Output_Overlap(uint32_t S_ID, int S_Len, int S_Dir, uint32_t T_ID, int T_Len,
+                           const olap_info *olap, work_area *WA) {
+  oo_record *ovs = push_rec(WA);
+  uint32_t span = (uint32_t)((olap->s_hi - olap->s_lo) + (olap->t_hi - olap->t_lo) + olap->delta_ct);
+  assert((span % 2) == 0);
+  span /= 2;
+  assert(S_ID < T_ID);
+
+  int32_t S_Right_Hang = S_Len - olap->s_hi - 1;
+  int32_t T_Right_Hang = T_Len - olap->t_hi - 1;
+  int Sleft;
+  char orient = 0;
+  int32_t ahg = 0, bhg = 0;
+
+  assert(olap->s_lo < olap->s_hi);
+  assert(olap->t_lo < olap->t_hi);
+  assert((olap->s_lo == 0) || (olap->t_lo == 0));
+  assert((olap->s_hi == S_Len - 1) || (olap->t_hi == T_Len - 1));
+
+  if ((olap->s_lo > olap->t_lo) || ((olap->s_lo == olap->t_lo) && (S_Right_Hang > T_Right_Hang)))
+    Sleft = 1;
+  else
+    Sleft = 0;
+
+  if (Sleft) { ovs->a_iid = S_ID; ovs->b_iid = T_ID; }
+  else       { ovs->a_iid = T_ID; ovs->b_iid = S_ID; }
+
+  if (Sleft) {
+    orient = (S_Dir == FORWARD) ? 'N' : 'O';
+    ahg = olap->s_lo;
+    bhg = T_Right_Hang - S_Right_Hang;
+  } else {
+    orient = (S_Dir == FORWARD) ? 'N' : 'I';
+    ahg = olap->t_lo;
+    bhg = S_Right_Hang - T_Right_Hang;
+  }
+
+  if ((orient == 'O') && (S_Right_Hang >= T_Right_Hang)) {
+    orient = 'I';
+    ahg = -(T_Right_Hang - S_Right_Hang);
+    bhg = -(olap->s_lo);
+  }
+
+  int32_t ah, bh; int flipped;
+  if (orient == 'N')      { ah = ahg;  bh = bhg;  flipped = 0; }
+  else if (orient == 'I') { ah = ahg;  bh = bhg;  flipped = 1; }
+  else                    { ah = -bhg; bh = -ahg; flipped = 1; }   /* 'O' */
+
+  /* ovOverlap::a_hang / b_hang (ovOverlap.H:125-126) */
+  uint32_t ahg5 = (ah < 0) ? 0 : (uint32_t)ah, bhg5 = (ah < 0) ? (uint32_t)(-ah) : 0;
+  uint32_t bhg3 = (bh < 0) ? 0 : (uint32_t)bh, ahg3 = (bh < 0) ? (uint32_t)(-bh) : 0;
+
+  ovs->w0 = w0_pack(ahg5, ahg3, oo_encode_evalue(olap->quality), flipped, 0, 0, 1);
+  ovs->w1 = w1_pack(bhg5, bhg3, span);
+
+  WA->Total_Overlaps++;
+  if (bhg <= 0) WA->Contained_Overlap_Ct++;
+  else          WA->Dovetail_Overlap_Ct++;
+}
+
+/* overlapInCore-Output.C:195-264 (the unlocked global Total_Overlaps++ is kept per thread here) */
+static void Output_Partial_Overlap(uint32_t s_id, uint32_t t_id, int dir, const olap_info *olap,
+                                   int s_len, int t_len, work_area *WA) {
+  WA->Total_Overlaps++;
+  oo_record *ovl = push_rec(WA);
+  assert(s_id < t_id);
+  ovl->a_iid = s_id;
+  ovl->b_iid = t_id;
+  uint32_t span = (uint32_t)((olap->s_hi - olap->s_lo) + (olap->t_hi - olap->t_lo) + olap->delta_ct);
+  assert((span % 2) == 0);
+  span /= 2;
+  uint32_t ahg5, ahg3, bhg5, bhg3; int flipped;
+  if (dir == FORWARD) {
+    ahg5 = olap->s_lo; ahg3 = s_len - (olap->s_hi + 1);
+    bhg5 = olap->t_lo; bhg3 = t_len - (olap->t_hi + 1);
+    flipped = 0;
+  } else {
+    ahg5 = s_len - (olap->s_hi + 1); ahg3 = olap->s_lo;
+    bhg5 = t_len - (olap->t_hi + 1); bhg3 = olap->t_lo;
+    flipped = 1;
+  }
+  ovl->w0 = w0_pack(ahg5, ahg3, oo_encode_evalue(olap->quality), flipped, 1, 1, 0);
+  ovl->w1 = w1_pack(bhg5, bhg3, span);
+}
+
+static oo_pair_trace *push_trace(work_area *WA) {
+  if (WA->ntr == WA->captr) {
+    WA->captr *= 2;
+    WA->tr = (oo_pair_trace *)realloc(WA->tr, sizeof(oo_pair_trace) * WA->captr);
+  }
+  return WA->tr + WA->ntr++;
+}
+
+static int32_t list_len(const work_area *WA, int32_t p) { int32_t n = 0; while (p) { n++; p = WA->mn[p].Next; } return n; }
+
+/* Process_String_Overlaps.C:353-547 */
+static void Process_Matches(const search_ctx *C, int32_t *Start, const char *S, int S_Len, uint32_t S_ID,
+                            int Dir, const char *T, int t_len, uint32_t T_ID, work_area *WA,
+                            int consistent, const string_olap *so) {
+  const oo_params *P = C->P;
+  int32_t P_, *Ref;
+  olap_info *distinct_olap = NULL;
+  oo_match *Longest_Match, *Ptr;
+  int Kind_Of_Olap = NONE;
+  double Quality;
+  int Olap_Len;
+  int overlaps_output = 0;
+  int distinct_olap_ct;
+  int Max_Len, S_Lo = 0, S_Hi = 0, T_Lo = 0, T_Hi = 0;
+  int Errors = 0;
+
+  oo_pair_trace *tr = NULL;
+  uint64_t cells0 = WA->ed->cells;
+  if (WA->collect) {
+    tr = push_trace(WA);
+    memset(tr, 0, sizeof(*tr));
+    tr->s_id = S_ID; tr->t_id = T_ID; tr->dir = Dir; tr->s_len = S_Len; tr->t_len = t_len;
+    tr->n_hits = so->diag_ct; tr->n_nodes = list_len(WA, *Start);
+    tr->consistent = consistent; tr->max_mult = so->max_mult;
+    int dmin = 1 << 30, dmax = -(1 << 30), lg = 0;
+    for (int32_t p = *Start; p; p = WA->mn[p].Next) {
+      int dg = WA->mn[p].Offset - WA->mn[p].Start;
+      if (dg < dmin) dmin = dg;
+      if (dg > dmax) dmax = dg;
+      if (WA->mn[p].Len > lg) lg = WA->mn[p].Len;
+    }
+    tr->diag_min = dmin; tr->diag_max = dmax; tr->longest = lg;
+    tr->exit_d = 0;
+  }
+  WA->pairs++;
+
+  assert((*Start) != 0);
+
+  if (C->hopeless && WA->mn[*Start].Next == 0 && !P->partial) {
+    int s_head, t_head, s_tail, t_tail;
+    int is_hopeless = 0;
+    s_head = WA->mn[*Start].Start;
+    t_head = WA->mn[*Start].Offset;
+    if (s_head <= t_head) {
+      if (s_head > HOPELESS_MATCH && !WA->left_end_screened) is_hopeless = 1;
+    } else {
+      if (t_head > HOPELESS_MATCH /* && ! t_info.lfrag_end_screened: false for loaded reads */) is_hopeless = 1;
+    }
+    s_tail = S_Len - s_head - WA->mn[*Start].Len + 1;
+    t_tail = t_len - t_head - WA->mn[*Start].Len + 1;
+    if (s_tail <= t_tail) {
+      if (s_tail > HOPELESS_MATCH && !WA->right_end_screened) is_hopeless = 1;
+    } else {
+      if (t_tail > HOPELESS_MATCH) is_hopeless = 1;
+    }
+    if (is_hopeless) {
+      *Start = 0;
+      WA->Kmer_Hits_Without_Olap_Ct++;
+      return;
+    }
+  }
+
+  distinct_olap = WA->distinct;
+  distinct_olap_ct = 0;
+  int first_ext = 1;
+
+  while ((*Start) != 0) {
+    int a_hang, b_hang;
+    int hit_limit = 0;
+
+    Max_Len = WA->mn[*Start].Len;
+    Longest_Match = WA->mn + (*Start);
+    for (P_ = WA->mn[*Start].Next; P_ != 0; P_ = WA->mn[P_].Next)
+      if (WA->mn[P_].Len > Max_Len) {
+        Max_Len = WA->mn[P_].Len;
+        Longest_Match = WA->mn + P_;
+      }
+
+    a_hang = Longest_Match->Start - Longest_Match->Offset;
+    b_hang = a_hang + S_Len - t_len;
+    hit_limit = (((uint64_t)WA->A_Olaps_For_Frag >= P->fragOlapLimit && a_hang <= 0) ||
+                 ((uint64_t)WA->B_Olaps_For_Frag >= P->fragOlapLimit && b_hang <= 0));
+
+    if (!hit_limit) {
+      Kind_Of_Olap = oo_extend(WA->ed, Longest_Match, S, S_ID, S_Len, T, T_ID, t_len,
+                               &S_Lo, &S_Hi, &T_Lo, &T_Hi, &Errors);
+      WA->extensions++;
+      if (tr) {
+        tr->n_ext++;
+        if (first_ext) { tr->exit_d = (T_Hi - S_Hi); first_ext = 0; }
+      }
+
+      if (Kind_Of_Olap == DOVETAIL || P->partial) {
+        if (1 + S_Hi - S_Lo >= P->minOlap && 1 + T_Hi - T_Lo >= P->minOlap) {
+          int a = S_Hi - S_Lo, b = T_Hi - T_Lo;
+          Olap_Len = 1 + (a < b ? a : b);
+          Quality = (double)Errors / Olap_Len;
+          if (Errors <= C->tb->Error_Bound[Olap_Len])
+            Add_Overlap(C, S_Lo, S_Hi, T_Lo, T_Hi, Quality, distinct_olap, &distinct_olap_ct, WA);
+        }
+      }
+    }
+
+    if (consistent)
+      *Start = 0;
+
+    for (Ref = Start; (*Ref) != 0;) {
+      Ptr = WA->mn + (*Ref);
+      if (Ptr == Longest_Match ||
+          ((Kind_Of_Olap == DOVETAIL || P->partial) &&
+           S_Lo - SHIFT_SLACK <= Ptr->Start &&
+           Ptr->Start + Ptr->Len <= (S_Hi + 1) + SHIFT_SLACK - 1 &&
+           Lies_On_Alignment(Ptr->Start, Ptr->Offset, S_Lo, T_Lo, WA)))
+        *Ref = Ptr->Next;
+      else
+        Ref = &(Ptr->Next);
+    }
+  }
+
+  if (distinct_olap_ct > 0) {
+    int deleted[MAX_DISTINCT_OLAPS] = {0};
+    if (P->partial) {
+      if (P->unique) Choose_Best_Partial(distinct_olap, distinct_olap_ct, deleted);
+    } else {
+      if (P->unique) Combine_Into_One_Olap(distinct_olap, distinct_olap_ct, deleted);
+      else           Merge_Intersecting_Olaps(distinct_olap, distinct_olap_ct, deleted);
+    }
+    const olap_info *p = distinct_olap;
+    for (int i = 0; i < distinct_olap_ct; i++) {
+      if (!deleted[i]) {
+        if (P->partial) Output_Partial_Overlap(S_ID, T_ID, Dir, p, S_Len, t_len, WA);
+        else            Output_Overlap(S_ID, S_Len, Dir, T_ID, t_len, p, WA);
+        overlaps_output++;
+        if (p->s_lo == 0) WA->A_Olaps_For_Frag++;
+        if (p->s_hi >= S_Len - 1) WA->B_Olaps_For_Frag++;
+      }
+      p++;
+    }
+  }
+
+  if (overlaps_output == 0)
+    WA->Kmer_Hits_Without_Olap_Ct++;
+  else {
+    WA->Kmer_Hits_With_Olap_Ct++;
+    if (overlaps_output > 1)
+      WA->Multi_Overlap_Ct++;
+  }
+  if (tr) { tr->n_olaps = overlaps_output; tr->cells = WA->ed->cells - cells0; }
+}
+
+/* Process_String_Overlaps.C:22-33 */
+static uint64_t computeExpected(uint64_t kmerSize, double ovlLen, double erate) {
+  if (ovlLen < kmerSize) return 0;
+  return (uint64_t)(int)(floor(exp(-1.0 * (double)kmerSize * erate) * (ovlLen - kmerSize + 1)));
+}
+static uint64_t computeMinimumKmers(const search_ctx *C, uint64_t kmerSize, double ovlLen, double erate) {
+  if (C->Filter_By_Kmer_Count == 0) return 0;
+  ovlLen = (ovlLen < 0 ? ovlLen * -1.0 : ovlLen);
+  uint64_t e = computeExpected(kmerSize, ovlLen, erate);
+  return C->Filter_By_Kmer_Count > e ? C->Filter_By_Kmer_Count : e;
+}
+
+static int by_diag_sum(const void *a, const void *b) {
+  const string_olap *x = (const string_olap *)a, *y = (const string_olap *)b;
+  if (x->diag_sum < y->diag_sum) return -1;
+  if (x->diag_sum > y->diag_sum) return 1;
+  return 0;
+}
+
+/* Process_String_Overlaps.C:580-690 */
+static int Process_String_Olaps(const search_ctx *C, const char *S, int Len, uint32_t ID, int Dir, work_area *WA) {
+  const oic_index *X = C->X;
+  const oo_params *P = C->P;
+  int32_t i, ct, root_num;
+  uint32_t off = X->bgnID;     /* Hash_String_Num_Offset */
+
+  for (i = ct = 0; i < WA->so_next; i++)
+    if (WA->so[i].Full) {
+      root_num = (int32_t)WA->so[i].String_Num;
+      if ((uint32_t)root_num + off > ID) {
+        if (WA->so[i].Match_List == 0) { fprintf(stderr, "oracle: String %u has no matches\n", ID); exit(2); }
+        if (i != ct) WA->so[ct] = WA->so[i];
+        assert(WA->so[ct].diag_ct > 0);
+        WA->so[ct].diag_sum /= WA->so[ct].diag_ct;
+        ct++;
+      }
+    }
+
+  if (ct == 0)
+    return ct;
+
+  if ((uint64_t)ct <= P->fragOlapLimit) {
+    for (i = 0; i < ct; i++) {
+      root_num = (int32_t)WA->so[i].String_Num;
+      if (computeMinimumKmers(C, P->kmer, WA->so[i].diag_end - WA->so[i].diag_bgn, C->tb->maxErate) > (uint64_t)(int64_t)WA->so[i].diag_ct) {
+        WA->Kmer_Hits_Skipped_Ct++;
+        continue;
+      }
+      Process_Matches(C, &WA->so[i].Match_List, S, Len, ID, Dir,
+                      X->lower + X->strStart[root_num], X->strLen[root_num], (uint32_t)root_num + off,
+                      WA, WA->so[i].consistent, &WA->so[i]);
+      assert(WA->so[i].Match_List == 0);
+    }
+    return ct;
+  }
+
+  /* -l path: qsort is not stable; the order of equal diag_sum entries is unpinned */
+  qsort(WA->so, ct, sizeof(string_olap), by_diag_sum);
+  int start;
+  for (start = 0; start < ct && WA->so[start].diag_sum < 0; start++)
+    ;
+  int processed_ct = 0;
+  for (i = start; i < ct && (uint64_t)WA->A_Olaps_For_Frag < P->fragOlapLimit; i++) {
+    root_num = (int32_t)WA->so[i].String_Num;
+    if (computeMinimumKmers(C, P->kmer, WA->so[i].diag_end - WA->so[i].diag_bgn, C->tb->maxErate) > (uint64_t)(int64_t)WA->so[i].diag_ct) { WA->Kmer_Hits_Skipped_Ct++; continue; }
+    Process_Matches(C, &WA->so[i].Match_List, S, Len, ID, Dir, X->lower + X->strStart[root_num],
+                    X->strLen[root_num], (uint32_t)root_num + off, WA, WA->so[i].consistent, &WA->so[i]);
+    processed_ct++;
+  }
+  for (i = start - 1; i >= 0 && (uint64_t)WA->B_Olaps_For_Frag < P->fragOlapLimit; i--) {
+    root_num = (int32_t)WA->so[i].String_Num;
+    if (computeMinimumKmers(C, P->kmer, WA->so[i].diag_end - WA->so[i].diag_bgn, C->tb->maxErate) > (uint64_t)(int64_t)WA->so[i].diag_ct) { WA->Kmer_Hits_Skipped_Ct++; continue; }
+    Process_Matches(C, &WA->so[i].Match_List, S, Len, ID, Dir, X->lower + X->strStart[root_num],
+                    X->strLen[root_num], (uint32_t)root_num + off, WA, WA->so[i].consistent, &WA->so[i]);
+    processed_ct++;
+  }
+  return processed_ct;
+}
+
+/* Find_Overlaps.C:234-336.  The check-array / bucket probing is replaced by an exact lookup with
+ * the same outcome: an S window matches iff its k bases are all ACGT and occur in a loaded read. */
+static void Find_Overlaps(const search_ctx *C, const char *Frag, int Frag_Len, uint32_t Frag_Num, int Dir, work_area *WA) {
+  const oic_index *X = C->X;
+  const uint32_t k = C->P->kmer;
+  const uint64_t kmask = (k == 32) ? ~0ULL : (((uint64_t)1 << (2 * k)) - 1);
+
+  memset(WA->so, 0, STRING_OLAP_MODULUS * sizeof(string_olap));
+  WA->so_next = STRING_OLAP_MODULUS;
+  WA->mn_next = 1;
+  assert(Frag_Len >= (int)k);
+
+  WA->left_end_screened = 0;
+  WA->right_end_screened = 0;
+  WA->A_Olaps_For_Frag = 0;
+  WA->B_Olaps_For_Frag = 0;
+
+  uint64_t key = 0; uint32_t valid = 0;
+  for (int i = 0; i < Frag_Len; i++) {
+    int c = code_of[(unsigned char)Frag[i]];
+    if (c < 0) { valid = 0; key = 0; continue; }
+    key = ((key << 2) | (uint64_t)c) & kmask;
+    if (++valid < k) continue;
+    int Offset = i + 1 - (int)k;
+    int64_t h = kmap_find(&X->map, key);
+    if (h < 0) continue;
+    uint64_t f = X->map.first[h], n = X->map.cnt[h];
+    for (uint64_t q = 0; q < n; q++) {
+      uint64_t r = X->occ[f + q];
+      uint32_t tl = (uint32_t)(r >> 32), toff = (uint32_t)r;
+      if (Frag_Num < tl + X->bgnID)
+        Add_Ref(C, tl, toff, Offset, (int32_t)n, WA);
+    }
+  }
+
+  Process_String_Olaps(C, Frag, Frag_Len, Frag_Num, Dir, WA);
+}
+
+void oo_params_default(oo_params *P) {
+  memset(P, 0, sizeof(*P));
+  P->kmer = 22;
+  P->maxErate = 0.06;
+  P->alignNoise = 1.0;
+  P->minOlap = 0;
+  P->partial = 0;
+  P->unique = 1;
+  P->hopeless = 1;
+  P->minkmers = 0;
+  P->fragOlapLimit = UINT64_MAX;
+  P->hashBits = 22;
+  P->hashLoad = 0.6;
+  P->hashDataLen = 100000000;
+  P->bgnHash = 1; P->endHash = UINT32_MAX;
+  P->bgnRef = 1;  P->endRef = UINT32_MAX;
+  P->nThreads = 1;
+  P->ompThreads = 0;
+}
+
+static void reverse_complement(char *s, int len) {   /* textbook: a<->t, c<->g, others kept */
+  static char comp[256];
+  static int init = 0;
+  if (!init) {
+    for (int i = 0; i < 256; i++) comp[i] = (char)i;
+    comp['a'] = 't'; comp['t'] = 'a'; comp['c'] = 'g'; comp['g'] = 'c';
+    comp['A'] = 'T'; comp['T'] = 'A'; comp['C'] = 'G'; comp['G'] = 'C';
+    init = 1;
+  }
+  for (int i = 0, j = len - 1; i <= j; i++, j--) {
+    char a = comp[(unsigned char)s[i]], b = comp[(unsigned char)s[j]];
+    s[i] = b; s[j] = a;
+  }
+}
+
+/* overlapInCore.C:161-277 (driver) + overlapInCore-Process_Overlaps.C:24-123 */
+int oo_run(const oo_params *Pin, const char *bases, const uint64_t *start, const uint32_t *len,
+           uint32_t nreads, int collect_trace, oo_result *out) {
+  init_codes();
+  memset(out, 0, sizeof(*out));
+  oo_params P = *Pin;
+  P.maxErate   = oo_cli_rate(P.maxErate);
+  P.alignNoise = oo_cli_rate(P.alignNoise);
+  if (P.maxErate > 0.06) P.hopeless = 0;                   /* overlapInCore.C:400-401 */
+  if (P.nThreads == 0) P.nThreads = 1;
+
+  search_ctx C;
+  memset(&C, 0, sizeof(C));
+  C.P = &P;
+  C.hopeless = P.hopeless;
+  if (P.minkmers)                                          /* overlapInCore.C:410-411 */
+    C.Filter_By_Kmer_Count = (uint64_t)(int)(floor(exp(-1.0 * (double)P.kmer * P.maxErate) * (P.minOlap - (int64_t)P.kmer + 1)));
+
+  oo_tables *tb = oo_tables_new(P.maxErate, P.alignNoise, P.partial);
+  C.tb = tb;
+
+  if (P.bgnHash < 1) P.bgnHash = 1;
+  if (nreads < P.endHash) P.endHash = nreads;
+  if (P.bgnRef < 1) P.bgnRef = 1;
+  if (P.endRef > nreads) P.endRef = nreads;
+
+  int nomp = P.ompThreads ? (int)P.ompThreads : (int)P.nThreads;
+  work_area *WAs = (work_area *)calloc(nomp, sizeof(work_area));
+  for (int i = 0; i < nomp; i++) wa_init(WAs + i, tb, collect_trace);
+
+  /* Ref reads actually searched.  OverlapDriver hands out chunks of perThread reads starting at
+   * bgnRefID; a thread only works on a chunk whose first read is < endRefID
+   * (Process_Overlaps.C:33), so a chunk that starts exactly at endRefID is never searched. */
+  uint32_t perThread = 1 + (P.endRef - P.bgnRef) / P.nThreads / 8;     /* overlapInCore.C:225 */
+  int64_t refEnd = P.endRef;
+  if (P.endRef >= P.bgnRef && ((P.endRef - P.bgnRef) % perThread) == 0)
+    refEnd = (int64_t)P.endRef - 1;
+
+  uint32_t bgnHashID = P.bgnHash, endHashID = P.endHash;
+  while (bgnHashID < P.endHash) {                                         /* overlapInCore.C:204 */
+    if (endHashID > P.endHash) endHashID = P.endHash;
+    oic_index X;
+    double t0 = now_s();
+    endHashID = build_index(&P, bases, start, len, bgnHashID, endHashID, &X);
+    out->t_index += now_s() - t0;
+    if (out->nblocks < 64) out->hashEnd[out->nblocks] = endHashID;
+    out->nblocks++;
+    C.X = &X;
+
+    t0 = now_s();
+#pragma omp parallel num_threads(nomp)
+    {
+      work_area *WA = WAs + omp_get_thread_num();
+      char *buf = (char *)malloc(OO_MAX_READLEN + 2);
+#pragma omp for schedule(dynamic, 4)
+      for (int64_t fi = P.bgnRef; fi <= refEnd; fi++) {
+        uint32_t L = len[fi];
+        if ((int64_t)L < P.minOlap) continue;
+        const char *s = bases + start[fi];
+        for (uint32_t i = 0; i < L; i++) buf[i] = (char)tolower((unsigned char)s[i]);
+        buf[L] = 0;
+        Find_Overlaps(&C, buf, (int)L, (uint32_t)fi, FORWARD, WA);
+        reverse_complement(buf, (int)L);
+        Find_Overlaps(&C, buf, (int)L, (uint32_t)fi, REVERSE, WA);
+      }
+      free(buf);
+    }
+    out->t_search += now_s() - t0;
+    free_index(&X);
+    bgnHashID = endHashID + 1;
+    endHashID = P.endHash;
+  }
+
+  uint64_t nrec = 0, ntr = 0;
+  for (int i = 0; i < nomp; i++) { nrec += WAs[i].nrec; ntr += WAs[i].ntr; }
+  out->rec = (oo_record *)malloc(sizeof(oo_record) * (nrec ? nrec : 1));
+  if (collect_trace) out->trace = (oo_pair_trace *)malloc(sizeof(oo_pair_trace) * (ntr ? ntr : 1));
+  for (int i = 0; i < nomp; i++) {
+    work_area *WA = WAs + i;
+    memcpy(out->rec + out->nrec, WA->rec, sizeof(oo_record) * WA->nrec);
+    out->nrec += WA->nrec;
+    if (collect_trace) { memcpy(out->trace + out->ntrace, WA->tr, sizeof(oo_pair_trace) * WA->ntr); out->ntrace += WA->ntr; }
+    out->stats[0] += WA->Kmer_Hits_Without_Olap_Ct;
+    out->stats[1] += WA->Kmer_Hits_With_Olap_Ct;
+    out->stats[2] += WA->Multi_Overlap_Ct;
+    out->stats[3] += WA->Total_Overlaps;
+    out->stats[4] += WA->Contained_Overlap_Ct;
+    out->stats[5] += WA->Dovetail_Overlap_Ct;
+    out->cells += WA->ed->cells; out->bases += WA->ed->bases; out->dp_calls += WA->ed->calls;
+    out->extensions += WA->extensions; out->hits += WA->hits; out->pairs += WA->pairs;
+    wa_free(WA);
+  }
+  free(WAs);
+  oo_tables_free(tb);
+  return 0;
+}
+
+void oo_result_free(oo_result *r) {
+  free(r->rec); free(r->trace);
+  memset(r, 0, sizeof(*r));
+}
