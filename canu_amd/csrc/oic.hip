@@ -232,6 +232,16 @@ extern "C" int oic_tables(const oic_ctx *c, const int32_t **eb, const int32_t **
   return OIC_OK;
 }
 
+extern "C" int oic_dp_tables(double max_erate, double align_noise, int32_t *eb, int32_t *ml, uint32_t *max_errors) {
+  if (!(max_erate > 0.0) || max_erate > 0.5 || !max_errors) return OIC_EINVAL;
+  oic::Tables T;
+  T.build(max_erate, align_noise);
+  if (eb) memcpy(eb, T.error_bound.data(), T.error_bound.size() * 4);
+  if (ml) memcpy(ml, T.match_limit.data(), (size_t)T.max_errors * 4);
+  *max_errors = T.max_errors;
+  return OIC_OK;
+}
+
 extern "C" int oic_get_counters(const oic_ctx *c, oic_counters *out) {
   if (!c || !out) return OIC_EINVAL;
   *out = c->ctr;

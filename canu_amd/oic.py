@@ -26,6 +26,7 @@ SCHED_PLAIN, SCHED_BATCHED, SCHED_LEARNED = 0, 1, 2
 EXPORTS = [
     "oic_params_default", "oic_create", "oic_destroy", "oic_last_error", "oic_load_reads", "oic_index_build",
     "oic_search", "oic_run", "oic_get_counters", "oic_extend_batch", "oic_prefix_dp_batch", "oic_tables",
+    "oic_dp_tables",
 ]
 
 RECORD_DTYPE = np.dtype([("a_iid", "<u4"), ("b_iid", "<u4"), ("w0", "<u8"), ("w1", "<u8")])
@@ -95,8 +96,21 @@ def lib():
         L.oic_prefix_dp_batch.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32]
         L.oic_tables.argtypes = [vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
                                  ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)), ctypes.POINTER(ctypes.c_uint32)]
+        L.oic_dp_tables.argtypes = [ctypes.c_double, ctypes.c_double, vp, vp, ctypes.POINTER(ctypes.c_uint32)]
         _lib = L
     return _lib
+
+
+def dp_tables(maxerate, alignnoise=1.0):
+    """(Error_Bound, Edit_Match_Limit) as the engine builds them; needs no device."""
+    me = ctypes.c_uint32()
+    rc = lib().oic_dp_tables(maxerate, alignnoise, None, None, ctypes.byref(me))
+    if rc != 0:
+        raise OicError(rc, "oic_dp_tables")
+    eb = np.zeros(1 << 21, np.int32)
+    ml = np.zeros(me.value, np.int32)
+    lib().oic_dp_tables(maxerate, alignnoise, eb.ctypes.data, ml.ctypes.data, ctypes.byref(me))
+    return eb, ml
 
 
 @dataclasses.dataclass

@@ -185,6 +185,11 @@ int oic_prefix_dp_batch(oic_ctx *ctx, const oic_dp_task *tasks, uint32_t n, oic_
 int oic_tables(const oic_ctx *ctx, const int32_t **error_bound, const int32_t **match_limit,
                uint32_t *max_errors);
 
+/* The same tables without a context or device: error_bound needs 2^21 entries, match_limit
+ * *max_errors entries (call once with match_limit = NULL to learn the size). */
+int oic_dp_tables(double max_erate, double align_noise, int32_t *error_bound, int32_t *match_limit,
+                  uint32_t *max_errors);
+
 #ifdef __cplusplus
 }
 #endif

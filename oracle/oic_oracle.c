@@ -1551,7 +1551,8 @@ int oo_run(const oo_params *Pin, const char *bases, const uint64_t *start, const
   C.P = &P;
   C.hopeless = P.hopeless;
   if (P.minkmers)                                          /* overlapInCore.C:410-411 */
-    C.Filter_By_Kmer_Count = (uint64_t)(int)(floor(exp(-1.0 * (double)P.kmer * P.maxErate) * (P.minOlap - (int64_t)P.kmer + 1)));
+    C.Filter_By_Kmer_Count = (uint64_t)(int)(floor(exp(-1.0 * (double)P.kmer * P.maxErate) *
+                                                   (double)((uint64_t)(int64_t)P.minOlap - (uint64_t)P.kmer + 1)));
 
   oo_tables *tb = oo_tables_new(P.maxErate, P.alignNoise, P.partial);
   C.tb = tb;

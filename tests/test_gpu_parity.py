@@ -1,0 +1,242 @@
+"""GPU engine (liboic_hip.so through the C ABI) against the oracle, bit for bit.
+
+DP units: prefixEditDistance::forward / ::reverse / Extend_Alignment on random and mutated
+windows (with 'n', exact matches, zero error limits, long reads whose bands need the wide tile).
+Whole runs: sorted 24-byte records and the .stats counters on small versions of the configured
+workloads and on every option the engine supports.
+"""
+import numpy as np
+import pytest
+
+from canu_amd import oic, readsim
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def mutate(rng, s, rate):
+    out = []
+    for ch in s:
+        u = rng.random()
+        if u < rate / 3:
+            out.append(rng.choice([c for c in "ACGT" if c != ch]))
+        elif u < 2 * rate / 3:
+            continue
+        elif u < rate:
+            out.append(rng.choice(list("ACGT")))
+            out.append(ch)
+        else:
+            out.append(ch)
+    return "".join(out)
+
+
+def dp_cases(seed, n, lmin, lmax):
+    rng = np.random.default_rng(seed)
+    seqs = []
+    for _ in range(n):
+        L = int(rng.integers(lmin, lmax))
+        a = "".join(rng.choice(list("ACGT"), L))
+        b = mutate(rng, a, float(rng.choice([0.0, 0.01, 0.03, 0.06, 0.1, 0.25])))
+        if rng.random() < 0.3:
+            b += "".join(rng.choice(list("ACGT"), int(rng.integers(0, 300))))
+        if rng.random() < 0.1 and L > 4:
+            k = int(rng.integers(0, L))
+            a = a[:k] + "N" + a[k + 1:]
+        seqs += [a, b]
+    return seqs
+
+
+def check_dp(seqs, rate, limit_slack):
+    R = readsim.from_strings(seqs)
+    eng = oic.Engine(kmer=22, maxerate=rate)
+    eng.load_reads(R)
+    od = oracle.OracleDP(rate)
+    tasks = []
+    for i in range(0, len(seqs), 2):
+        ia, ib = i + 1, i + 2
+        if len(seqs[i]) > len(seqs[i + 1]):
+            ia, ib = ib, ia
+        m, n = len(seqs[ia - 1]), len(seqs[ib - 1])
+        lim = min(int(np.ceil(m * od.tables.maxErate)) + limit_slack, od.tables.MAX_ERRORS - 1)
+        tasks.append((ia, ib, 0, 0, 0, 0, m, n, lim, 0))
+        tasks.append((ia, ib, 0, 0, m - 1, n - 1, m, n, lim, 1))
+        tasks.append((ia, ib, 0, 0, 0, 0, m, n, 0, 0))
+    t = np.array(tasks, oic.DP_TASK)
+    res, delta = eng.prefix_dp(t, delta_stride=8192)
+    od.reset_counters()
+    for k, tk in enumerate(t):
+        A = seqs[tk["a_id"] - 1].lower().encode()
+        B = seqs[tk["b_id"] - 1].lower().encode()
+        r = res[k]
+        assert r["status"] == 0
+        if tk["dir"] == 0:
+            e, ae, te, mte, d = od.forward(A, B, int(tk["error_limit"]))
+        else:
+            e, ae, te, lo, mte, d = od.reverse(A, B, int(tk["error_limit"]))
+            assert r["leftover"] == lo, k
+        assert (r["errors"], r["a_end"], r["t_end"], bool(r["match_to_end"])) == (e, ae, te, mte), (k, tk)
+        assert r["delta_len"] == len(d) and np.array_equal(delta[k][:len(d)], d), k
+    assert int(res["cells"].sum()) == od.counters()["cells"]
+
+
+def test_dp_units_short():
+    check_dp(dp_cases(1, 300, 1, 600), 0.045, 3)
+
+
+def test_dp_units_long_wide_band():
+    # 8-30 kbp windows at up to 25 % divergence: bands grow past the small tile
+    check_dp(dp_cases(2, 24, 8000, 30000), 0.045, 3)
+
+
+def test_dp_units_other_rates():
+    check_dp(dp_cases(3, 120, 50, 3000), 0.01, 2)
+    check_dp(dp_cases(4, 120, 50, 3000), 0.12, 2)
+
+
+def test_extend_alignment_units():
+    rng = np.random.default_rng(5)
+    R = readsim.simulate(readsim.PRESETS["c2"].scaled(150_000, seed=31))
+    eng = oic.Engine(kmer=22, maxerate=0.045, minlength=500)
+    eng.load_reads(R)
+    od = oracle.OracleDP(0.045)
+    seqs = [None] + [R.read(i) for i in range(1, R.n + 1)]
+    comp = bytes.maketrans(b"ACGT", b"TGCA")
+    tasks = []
+    for _ in range(400):
+        s, t = (int(x) for x in rng.integers(1, R.n + 1, 2))
+        if s == t:
+            continue
+        sd = int(rng.integers(0, 2))
+        S = seqs[s] if sd == 0 else seqs[s].translate(comp)[::-1]
+        T = seqs[t]
+        ln = int(rng.integers(1, 40))
+        st = int(rng.integers(0, len(S) - ln))
+        of = int(rng.integers(0, len(T) - ln))
+        tasks.append((s, t, sd, st, of, ln, S, T))
+    arr = np.array([x[:6] for x in tasks], oic.EXT_TASK)
+    res, delta = eng.extend_alignment(arr, delta_stride=8192)
+    for k, (s, t, sd, st, of, ln, S, T) in enumerate(tasks):
+        kind, slo, shi, tlo, thi, err, d = od.extend(S.lower(), T.lower(), st, of, ln)
+        r = res[k]
+        assert r["status"] == 0
+        assert (r["kind"], r["s_lo"], r["s_hi"], r["t_lo"], r["t_hi"], r["errors"]) == (kind, slo, shi, tlo, thi, err), k
+        assert np.array_equal(delta[k][:r["delta_len"]], d), k
+
+
+def run_both(R, threads=4, schedule=oic.SCHED_PLAIN, **kw):
+    ekw = dict(kmer=22, maxerate=0.045, minlength=500, threads=threads, schedule=schedule)
+    okw = dict(kmer=22, maxErate=0.045, minOlap=500, nThreads=threads, ompThreads=16)
+    m = dict(partial=("partial", "partial"), unique=("unique", "unique"), hopeless=("hopeless", "hopeless"),
+             minkmers=("minkmers", "minkmers"), hashbits=("hashbits", "hashBits"), hashload=("hashload", "hashLoad"),
+             maxerate=("maxerate", "maxErate"), minlength=("minlength", "minOlap"))
+    ranges = {}
+    for k, v in kw.items():
+        if k in ("bgn_ref", "end_ref", "bgn_hash", "end_hash"):
+            ranges[k] = v
+            continue
+        ek, ok = m[k]
+        ekw[ek] = v
+        okw[ok] = int(v) if isinstance(v, bool) else v
+    eng = oic.Engine(**ekw)
+    eng.load_reads(R)
+    res = eng.run(ranges.get("bgn_hash", 1), ranges.get("end_hash", 0xFFFFFFFF), ranges.get("bgn_ref", 1),
+                  ranges.get("end_ref", 0xFFFFFFFF))
+    ref = oracle.run(R, bgnHash=ranges.get("bgn_hash", 1), endHash=ranges.get("end_hash", 0xFFFFFFFF),
+                     bgnRef=ranges.get("bgn_ref", 1), endRef=ranges.get("end_ref", 0xFFFFFFFF), **okw)
+    return res, ref
+
+
+def assert_same(res, ref):
+    a = oic.sort_records(res.records)
+    b = oracle.sort_records(ref["records"])
+    assert len(a) == len(b), (len(a), len(b))
+    for f in ("a_iid", "b_iid", "w0", "w1"):
+        assert np.array_equal(a[f], b[f]), f
+    assert list(res.stats) == list(ref["stats"])
+
+
+@pytest.fixture(scope="module")
+def c1s():
+    return readsim.simulate(readsim.PRESETS["c1"].scaled(200_000, seed=41))
+
+
+@pytest.fixture(scope="module")
+def c2s():
+    return readsim.simulate(readsim.PRESETS["c2"].scaled(120_000, seed=42))
+
+
+@pytest.fixture(scope="module")
+def c3s():
+    return readsim.simulate(readsim.PRESETS["c3"].scaled(300_000, seed=43))
+
+
+def test_run_c1_small(c1s):
+    assert_same(*run_both(c1s))
+
+
+def test_run_c2_small(c2s):
+    assert_same(*run_both(c2s))
+
+
+def test_run_c3_small_repeats(c3s):
+    assert_same(*run_both(c3s))
+
+
+@pytest.mark.parametrize("opt", [dict(unique=False), dict(partial=True), dict(hopeless=False), dict(minkmers=True),
+                                 dict(maxerate=0.01, minlength=1000), dict(maxerate=0.12)])
+def test_run_options(c1s, opt):
+    assert_same(*run_both(c1s, **opt))
+
+
+def test_run_multiple_hash_blocks(c1s):
+    res, ref = run_both(c1s, hashbits=12, hashload=0.3)
+    assert len(ref["hash_end"]) > 2
+    assert_same(res, ref)
+
+
+def test_run_ranges_and_thread_chunking(c1s):
+    n = c1s.n
+    for t in (1, 3, 7, 8):
+        assert_same(*run_both(c1s, threads=t, bgn_ref=5, end_ref=n - 7, bgn_hash=3, end_hash=n - 2))
+
+
+def test_run_schedules_identical(c2s):
+    base = run_both(c2s)[0]
+    for sch in (oic.SCHED_BATCHED, oic.SCHED_LEARNED):
+        eng = oic.Engine(kmer=22, maxerate=0.045, minlength=500, threads=4, schedule=sch)
+        eng.load_reads(c2s)
+        r = eng.run()
+        a, b = oic.sort_records(r.records), oic.sort_records(base.records)
+        assert len(a) == len(b) and all(np.array_equal(a[f], b[f]) for f in ("a_iid", "b_iid", "w0", "w1"))
+        assert r.stats == base.stats
+
+
+def test_run_reads_with_n_and_short_reads():
+    rng = np.random.default_rng(9)
+    R = readsim.simulate(readsim.PRESETS["c1"].scaled(60_000, seed=44))
+    seqs = [R.read(i).decode() for i in range(1, R.n + 1)]
+    for i in range(0, len(seqs), 3):
+        s = list(seqs[i])
+        for p in rng.integers(0, len(s), 5):
+            s[p] = "N"
+        seqs[i] = "".join(s)
+    seqs[1] = seqs[1][:300]     # shorter than --minlength: neither hashed nor searched
+    seqs.append("ACGT" * 3)     # shorter than k
+    R2 = readsim.from_strings(seqs)
+    assert_same(*run_both(R2))
+
+
+def test_run_single_and_two_reads():
+    R = readsim.simulate(readsim.PRESETS["c1"].scaled(60_000, seed=45))
+    one = R.subset(1)
+    res, ref = run_both(one)
+    assert len(res.records) == 0 and len(ref["records"]) == 0
+    two = R.subset(2)
+    assert_same(*run_both(two))
+
+
+def test_run_c1_full_bit_exact():
+    R = readsim.simulate(readsim.PRESETS["c1"])
+    res, ref = run_both(R, threads=8)
+    assert len(res.records) > 50000
+    assert_same(res, ref)

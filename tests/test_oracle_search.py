@@ -1,0 +1,202 @@
+"""The oracle's seed index, Find_Overlaps match lists, hash blocks and output records.
+
+Pinned against brute force on small read sets:
+  * seed hits of each (S, orientation, T) pair = windows of k ACGT bases shared by S and T with
+    S_ID < T_ID (Find_Overlaps.C:251-332 with Hash_Find's exact k-mer compare);
+  * match nodes = maximal runs of hits on one diagonal at consecutive S offsets, Len = k + run - 1
+    (Add_Match, Find_Overlaps.C:24-96);
+  * hash blocks stop before the read at which the distinct k-mer count reaches
+    hashload * 2^hashbits * 21 or the loaded length reaches hashdatalen + 2^21 - 1
+    (Build_Hash_Index.C:445,504-506), and the last hash read is never indexed when a block ends
+    one short of it (overlapInCore.C:204);
+  * records: the ovOverlap layout invariants and the simulator's truth.
+"""
+import numpy as np
+import pytest
+
+from canu_amd import readsim
+from oracle import oracle
+
+K = 22
+COMP = bytes.maketrans(b"ACGTacgt", b"TGCAtgca")
+
+
+def rc(s):
+    return s.translate(COMP)[::-1]
+
+
+@pytest.fixture(scope="module")
+def small():
+    cfg = readsim.PRESETS["c1"].scaled(30000, seed=21)
+    cfg.ln_mu, cfg.ln_sigma, cfg.min_len = 7.6, 0.3, 800
+    R = readsim.simulate(cfg)
+    out = oracle.run(R, collect_trace=True, kmer=K, maxErate=0.045, minOlap=500, nThreads=1)
+    return R, out
+
+
+def kmer_positions(s):
+    d = {}
+    for i in range(len(s) - K + 1):
+        w = s[i:i + K]
+        if all(c in b"ACGT" for c in w):
+            d.setdefault(w, []).append(i)
+    return d
+
+
+def brute_pairs(R):
+    seqs = [None] + [R.read(i).upper() for i in range(1, R.n + 1)]
+    idx = [None] + [kmer_positions(seqs[i]) for i in range(1, R.n + 1)]
+    res = {}
+    for s in range(1, R.n + 1):
+        for dr, S in ((0, seqs[s]), (1, rc(seqs[s]))):
+            for t in range(s + 1, R.n + 1):
+                hits = []
+                tk = idx[t]
+                for o in range(len(S) - K + 1):
+                    for toff in tk.get(S[o:o + K], ()):
+                        hits.append((o, toff))
+                if hits:
+                    res[(s, t, dr)] = hits
+    return res
+
+
+def runs(hits):
+    hs = set(hits)
+    starts = [h for h in hits if (h[0] - 1, h[1] - 1) not in hs]
+    lens = []
+    for o, t in starts:
+        n = 1
+        while (o + n, t + n) in hs:
+            n += 1
+        lens.append(n)
+    return lens
+
+
+def test_hits_and_nodes_match_brute_force(small):
+    R, out = small
+    tr = out["trace"]
+    bp = brute_pairs(R)
+    got = {(int(x["s_id"]), int(x["t_id"]), int(x["dir"])): x for x in tr}
+    assert set(got) == set(bp)
+    for key, hits in bp.items():
+        x = got[key]
+        assert x["n_hits"] == len(hits), key
+        lens = runs(hits)
+        assert x["n_nodes"] == len(lens), key
+        assert x["longest"] == K + max(lens) - 1, key
+        assert x["diag_min"] == min(t - o for o, t in hits)
+        assert x["diag_max"] == max(t - o for o, t in hits)
+    assert out["hits"] == sum(len(h) for h in bp.values())
+
+
+def test_stats_consistent(small):
+    R, out = small
+    st = out["stats"]
+    tr = out["trace"]
+    assert st[0] + st[1] == len(tr)           # every processed pair counted once
+    assert st[3] == len(out["records"]) == st[4] + st[5]
+    assert st[6] == st[7] == 0
+
+
+def decode(recs):
+    w0, w1 = recs["w0"].astype(np.uint64), recs["w1"].astype(np.uint64)
+    F = np.uint64(0x1FFFFF)
+    return dict(ahg5=w0 & F, ahg3=(w0 >> np.uint64(21)) & F, evalue=(w0 >> np.uint64(42)) & np.uint64(0xFFFF),
+                flipped=(w0 >> np.uint64(58)) & np.uint64(1), obt=(w0 >> np.uint64(59)) & np.uint64(1),
+                dup=(w0 >> np.uint64(60)) & np.uint64(1), utg=(w0 >> np.uint64(61)) & np.uint64(1),
+                bhg5=w1 & F, bhg3=(w1 >> np.uint64(21)) & F, span=(w1 >> np.uint64(42)) & F,
+                extra=(w0 >> np.uint64(62)) | (w1 >> np.uint64(63)))
+
+
+def test_record_layout_invariants(small):
+    R, out = small
+    recs = out["records"]
+    f = decode(recs)
+    L = R.length.astype(np.int64)
+    assert np.all(f["utg"] == 1) and np.all(f["obt"] == 0) and np.all(f["dup"] == 0) and np.all(f["extra"] == 0)
+    assert np.all(recs["a_iid"] != recs["b_iid"])
+    # at most one of ahg5/bhg5 and of ahg3/bhg3 is positive: dovetail or contained (Output.C:66-68)
+    assert np.all((f["ahg5"] == 0) | (f["bhg5"] == 0))
+    assert np.all((f["ahg3"] == 0) | (f["bhg3"] == 0))
+    # errors <= Error_Bound[len] = ceil(0.045 len) with len >= 500: evalue < 1e5 * (0.045 + 1/500)
+    assert np.all(f["evalue"] <= 4700)
+    # aligned spans fit the reads
+    alen = L[recs["a_iid"]] - f["ahg5"].astype(np.int64) - f["ahg3"].astype(np.int64)
+    blen = L[recs["b_iid"]] - f["bhg5"].astype(np.int64) - f["bhg3"].astype(np.int64)
+    assert np.all(alen >= 500) and np.all(blen >= 500)
+    assert np.all(np.abs(f["span"].astype(np.int64) - (alen + blen) // 2) <= 0.05 * (alen + blen))
+
+
+def test_overlaps_match_truth(small):
+    """Every reported pair overlaps on the genome, and long true overlaps are all found."""
+    R, out = small
+    recs = out["records"]
+    pos = R.pos.astype(np.int64)
+    tl = np.array([0] + [int(R.length[i]) for i in range(1, R.n + 1)], np.int64)
+    found = {(min(a, b), max(a, b)) for a, b in zip(recs["a_iid"].tolist(), recs["b_iid"].tolist())}
+    for a, b in found:
+        ov = min(pos[a] + tl[a], pos[b] + tl[b]) - max(pos[a], pos[b])
+        assert ov > 300, (a, b, ov)
+    true_long = []
+    for a in range(1, R.n + 1):
+        for b in range(a + 1, R.n + 1):
+            ov = min(pos[a] + tl[a], pos[b] + tl[b]) - max(pos[a], pos[b])
+            if ov >= 1500:
+                true_long.append((a, b))
+    hit = sum(p in found for p in true_long)
+    assert hit >= 0.97 * len(true_long), (hit, len(true_long))
+
+
+def test_hash_blocks_follow_distinct_kmer_limit():
+    R = readsim.simulate(readsim.PRESETS["c1"].scaled(20000, seed=3))
+    # distinct k-mers of reads in order, brute force
+    seen, cum = set(), [0]
+    for i in range(1, R.n + 1):
+        s = R.read(i).upper()
+        if len(s) >= 500:
+            for j in range(len(s) - K + 1):
+                w = s[j:j + K]
+                if all(c in b"ACGT" for c in w):
+                    seen.add(w)
+        cum.append(len(seen))
+    hb, load = 10, 0.2
+    limit = int(load * (1 << hb) * 21)
+    # block ends: first read r (after the block start) with distinct-before-r >= limit
+    ends, bgn = [], 1
+    while bgn < R.n:
+        base = cum[bgn - 1]
+        end = R.n
+        for r in range(bgn, R.n + 1):
+            if cum[r - 1] - base >= limit:
+                end = r - 1
+                break
+        # distinct counts restart per block: recompute from the block's own reads
+        seen_b, end_b = set(), R.n
+        for r in range(bgn, R.n + 1):
+            if len(seen_b) >= limit:
+                end_b = r - 1
+                break
+            s = R.read(r).upper()
+            if len(s) >= 500:
+                for j in range(len(s) - K + 1):
+                    w = s[j:j + K]
+                    if all(c in b"ACGT" for c in w):
+                        seen_b.add(w)
+        ends.append(end_b)
+        bgn = end_b + 1
+    out = oracle.run(R, kmer=K, maxErate=0.045, minOlap=500, hashBits=hb, hashLoad=load)
+    assert out["hash_end"] == ends
+    assert len(ends) > 2
+
+
+def test_thread_chunking_skips_chunk_at_end():
+    """-r 1-E with -t T: E is never searched when (E - 1) % perThread == 0 (perThread = 1 + (E-1)/T/8)."""
+    R = readsim.simulate(readsim.PRESETS["c1"].scaled(40000, seed=5))
+    E = R.n - 3
+    t_skip = next(t for t in range(1, 64) if (E - 1) % (1 + (E - 1) // t // 8) == 0)
+    t_keep = next(t for t in range(1, 64) if (E - 1) % (1 + (E - 1) // t // 8) != 0)
+    a = oracle.run(R, collect_trace=True, kmer=K, maxErate=0.045, minOlap=500, endRef=E, nThreads=t_skip)
+    b = oracle.run(R, collect_trace=True, kmer=K, maxErate=0.045, minOlap=500, endRef=E, nThreads=t_keep)
+    assert E not in set(a["trace"]["s_id"].tolist())
+    sb = set(b["trace"]["s_id"].tolist())
+    assert max(sb) <= E and (E in sb or not any((R.pos[E] < R.pos[t] + R.length[t]) and (R.pos[t] < R.pos[E] + R.length[E]) for t in range(E + 1, R.n + 1)))
