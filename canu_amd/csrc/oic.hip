@@ -20,6 +20,7 @@
 #include "oic_sort.h"
 #include "oic_search.h"
 #include "oic_sched.h"
+#include "oic_batch.h"
 
 namespace {
 
@@ -85,6 +86,7 @@ struct oic_ctx {
   DevBuf g_cnt, g_off, g_pbase, g_searched;
   DevBuf p_pairs, p_tmp, p_nodes, p_recs, p_pout, p_order, p_queue, p_stats, p_nout, p_noff, p_outrec, p_retry;
   DevBuf sc_rowL, sc_rowR, sc_rowOff, sc_rowVal, sc_dbuf, sc_ld, sc_rd;
+  DevBuf b_cache, b_ncells, b_nexit, b_probe, b_spec, b_cnt, b_off, b_stats;
   int32_t sc_rows = 0;
   int64_t sc_val = 0;
   uint32_t sc_waves = 0;
@@ -527,6 +529,73 @@ static uint32_t ref_end_searched(uint32_t bgn, uint32_t end, uint32_t nthreads) 
   return ((end - bgn) % per == 0) ? end - 1 : end;
 }
 
+template <int G, int CAP>
+static int run_fwd(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
+  if (n == 0) return OIC_OK;
+  FA.ntasks = n;
+  HIPCHK(c, hipMemsetAsync(FA.queue, 0, 4, c->st));
+  const uint32_t waves = (uint32_t)std::min<uint64_t>((uint64_t)c->ncu * 16, (n + (OIC_WAVE / G) - 1) / (OIC_WAVE / G));
+  HIPCHK(c, hipEventRecord(c->ev0, c->st));
+  hipLaunchKernelGGL((k_fwd_tasks<G, CAP>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev1, c->st));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float f = 0;
+  hipEventElapsedTime(&f, c->ev0, c->ev1);
+  *ms += f;
+  return OIC_OK;
+}
+
+// probe pass over every pair's longest node, then every other node of the pairs whose probe
+// ended at a branch point; results land in the per-node cache
+static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
+  ALLOC(c, c->b_cache, H);
+  ALLOC(c, c->b_ncells, H * 4);
+  ALLOC(c, c->b_nexit, H * 4);
+  ALLOC(c, c->b_probe, np * sizeof(FwdTask));
+  ALLOC(c, c->b_cnt, np * 4);
+  ALLOC(c, c->b_off, np * 8);
+  ALLOC(c, c->b_stats, 64);
+  HIPCHK(c, hipMemsetAsync(c->b_cache.p, 0, H, c->st));
+  HIPCHK(c, hipMemsetAsync(c->b_stats.p, 0, 64, c->st));
+  const unsigned g = (unsigned)((np + 255) / 256);
+  hipLaunchKernelGGL(k_probe_tasks, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->p_nodes.as<int4>(), c->b_probe.as<FwdTask>());
+  FwdArgs FA;
+  FA.R = dev_reads(c);
+  FA.tb = dev_tables(c);
+  FA.pairs = c->p_pairs.as<PairRec>();
+  FA.nodes = c->p_nodes.as<int4>();
+  FA.queue = c->p_queue.as<uint32_t>();
+  FA.cache = c->b_cache.as<uint8_t>();
+  FA.ncells = c->b_ncells.as<uint32_t>();
+  FA.nexit = c->b_nexit.as<int32_t>();
+  FA.stats = c->b_stats.as<unsigned long long>();
+  double ms = 0;
+  FA.tasks = c->b_probe.as<FwdTask>();
+  int rc = run_fwd<16, 256>(c, FA, np, &ms);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_spec_count, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
+                     c->b_cache.as<uint8_t>(), c->b_cnt.as<uint32_t>());
+  uint64_t nt = 0;
+  ALLOC(c, c->s_scan, scan_scratch_words(np) * 8);
+  HIPCHK(c, scan_excl<uint32_t>(c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), np, c->s_scan.as<uint64_t>(), c->st, &nt));
+  if (nt) {
+    ALLOC(c, c->b_spec, nt * sizeof(FwdTask));
+    hipLaunchKernelGGL(k_spec_write, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
+                       c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), c->b_spec.as<FwdTask>());
+    FA.tasks = c->b_spec.as<FwdTask>();
+    rc = run_fwd<16, 256>(c, FA, nt, &ms);
+    if (rc) return rc;
+  }
+  unsigned long long st[8];
+  HIPCHK(c, hipMemcpy(st, c->b_stats.p, sizeof st, hipMemcpyDeviceToHost));
+  c->ctr.spec_cells += st[0];
+  c->ctr.spec_tasks += np + nt;
+  c->ctr.ms_fwd_kernel += ms;
+  c->ctr.ms_dp_kernel += ms;
+  return OIC_OK;
+}
+
 static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<uint8_t> &searched, oic_sink *sink) {
   const int K = (int)c->P.kmer;
   const double t0 = now_ms();
@@ -621,9 +690,19 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   }
   HIPCHK(c, hipMemcpyAsync(c->p_order.p, order.data(), np * 4, hipMemcpyHostToDevice, c->st));
 
+  // batched schedules: forward DPs of probe and speculative nodes, packed 4 per wavefront
+  const bool batched = c->P.schedule != OIC_SCHED_PLAIN && !c->P.partial;
+  if (batched) {
+    int rc = run_forward_passes(c, np, H);
+    if (rc) return rc;
+  }
+
   PairKernelArgs A;
   A.R = dev_reads(c);
   A.C = search_const(c, true);
+  A.cache = batched ? c->b_cache.as<uint8_t>() : nullptr;
+  A.ncells = batched ? c->b_ncells.as<uint32_t>() : nullptr;
+  A.nexit = batched ? c->b_nexit.as<int32_t>() : nullptr;
   A.pairs = c->p_pairs.as<PairRec>();
   A.order = c->p_order.as<uint32_t>();
   A.nodes = c->p_nodes.as<int4>();
@@ -679,6 +758,7 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   c->ctr.dp_lane_slots += stats[STAT_SLOTS];
   c->ctr.extensions += stats[STAT_EXT];
   c->ctr.dp_calls += stats[STAT_DPCALLS];
+  c->ctr.dp_bases += stats[STAT_BASES];
   if (sink->count + nrec > sink->capacity || !sink->records) {
     sink->records_needed = sink->count + nrec;
     // keep counting so the caller learns the full size
@@ -826,8 +906,8 @@ __global__ void __launch_bounds__(OIC_WAVE) k_dp_batch(BatchArgs A) {
       a = read_view(A.R, t.a_id, 1 - t.a_strand); a.b0 += A.R.len[t.a_id] - 1 - t.a_pos;
       b = read_view(A.R, t.b_id, 1 - t.b_strand); b.b0 += A.R.len[t.b_id] - 1 - t.b_pos;
     }
-    uint64_t cells = 0, slots = 0;
-    DPOut o = wave_prefix_dp<CAP>(a, t.m, b, t.n, t.error_limit, t.dir == 0, true, A.tb, bufA, bufB, sc, cells, slots);
+    uint64_t cells = 0, slots = 0, lb = 0;
+    DPOut o = wave_prefix_dp<CAP>(a, t.m, b, t.n, t.error_limit, t.dir == 0, true, A.tb, bufA, bufB, sc, cells, slots, lb);
     const int32_t *src = (t.dir == 0) ? sc.rdelta : sc.ldelta;
     if (A.delta && !o.overflow) {
       const int32_t nd = o.ndelta < (int32_t)A.delta_stride ? o.ndelta : (int32_t)A.delta_stride;
@@ -858,9 +938,10 @@ __global__ void __launch_bounds__(OIC_WAVE) k_ext_batch(BatchArgs A) {
     const oic_ext_task t = A.etasks[i];
     const SeqV S = read_view(A.R, t.s_id, t.s_dir), Src = read_view(A.R, t.s_id, 1 - t.s_dir);
     const SeqV T = read_view(A.R, t.t_id, 0), Trc = read_view(A.R, t.t_id, 1);
-    uint64_t cells = 0, slots = 0;
+    uint64_t cells = 0, slots = 0, lb = 0;
     uint32_t calls = 0;
-    ExtOut x = wave_extend<CAP>(t.start, t.offset, t.len, S, Src, A.R.len[t.s_id], T, Trc, A.R.len[t.t_id], A.C, bufA, bufB, sc, cells, slots, calls);
+    ExtOut x = wave_extend<CAP>(t.start, t.offset, t.len, S, Src, A.R.len[t.s_id], T, Trc, A.R.len[t.t_id], A.C, bufA, bufB, sc, cells, slots,
+                                calls, lb);
     if (A.delta && !x.overflow) {
       const int32_t nd = x.ndelta < (int32_t)A.delta_stride ? x.ndelta : (int32_t)A.delta_stride;
       for (int32_t j = lane; j < nd; j += OIC_WAVE) A.delta[(uint64_t)i * A.delta_stride + j] = sc.ldelta[j];

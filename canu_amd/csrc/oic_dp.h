@@ -185,7 +185,7 @@ template <int CAP>
 __device__ DPOut wave_prefix_dp(const SeqV A, const int32_t m, const SeqV T, const int32_t n, const int32_t limit,
                                 const bool is_forward, const bool want_delta, const DPTables &tb,
                                 int32_t *__restrict__ bufA, int32_t *__restrict__ bufB, const DPScratch &sc,
-                                uint64_t &cells, uint64_t &slots) {
+                                uint64_t &cells, uint64_t &slots, uint64_t &lane_bases) {
   constexpr int32_t MASK = CAP - 1;
   const int lane = lane_id();
   DPOut out;
@@ -198,6 +198,7 @@ __device__ DPOut wave_prefix_dp(const SeqV A, const int32_t m, const SeqV T, con
   const int32_t row0 = wave_slide(A, 0, T, 0, m);
   cells += 1;
   slots += 1;
+  if (lane == 0) lane_bases += (uint64_t)row0 + 1;
   __syncthreads();
   if (lane == 0) pbuf[0] = row0;
   if (want_delta && lane == 0) {
@@ -249,7 +250,9 @@ __device__ DPOut wave_prefix_dp(const SeqV A, const int32_t m, const SeqV T, con
         Row = a > b ? a : b;
         Row = Row > c ? Row : c;
         const int32_t r1 = m - Row, r2 = n - Row - d;
-        Row += lane_slide(A, Row, T, (int64_t)Row + d, r1 < r2 ? r1 : r2);
+        const int32_t adv = lane_slide(A, Row, T, (int64_t)Row + d, r1 < r2 ? r1 : r2);
+        Row += adv;
+        lane_bases += (uint64_t)adv + 1;
         cbuf[d & MASK] = Row;
       }
       const bool reach = act && (Row == m || Row + d == n);

@@ -402,7 +402,8 @@ struct SearchConst {
 template <int CAP>
 __device__ ExtOut wave_extend(int32_t Start, int32_t Offset, int32_t Len, const SeqV S, const SeqV Src, int32_t S_Len,
                               const SeqV T, const SeqV Trc, int32_t T_Len, const SearchConst &C, int32_t *bufA,
-                              int32_t *bufB, const DPScratch &sc, uint64_t &cells, uint64_t &slots, uint32_t &dpcalls) {
+                              int32_t *bufB, const DPScratch &sc, uint64_t &cells, uint64_t &slots, uint32_t &dpcalls,
+                              uint64_t &lane_bases) {
   const int lane = lane_id();
   ExtOut r = {};
   const int32_t S_Left_Begin = Start - 1, S_Right_Begin = Start + Len, S_Right_Len = S_Len - S_Right_Begin;
@@ -426,7 +427,7 @@ __device__ ExtOut wave_extend(int32_t Start, int32_t Offset, int32_t Len, const 
     } else {
       a = T; b = S; a.b0 += T_Right_Begin; am = T_Right_Len; b.b0 += S_Right_Begin; bn = S_Right_Len;
     }
-    DPOut o = wave_prefix_dp<CAP>(a, am, b, bn, Error_Limit, true, need_delta, C.tb, bufA, bufB, sc, cells, slots);
+    DPOut o = wave_prefix_dp<CAP>(a, am, b, bn, Error_Limit, true, need_delta, C.tb, bufA, bufB, sc, cells, slots, lane_bases);
     dpcalls++;
     if (o.overflow) { r.overflow = 1; return r; }
     Right_Errors = o.e; rmte = o.mte; nr = o.ndelta;
@@ -460,7 +461,7 @@ __device__ ExtOut wave_extend(int32_t Start, int32_t Offset, int32_t Len, const 
       b = Src; b.b0 += S_Len - 1 - S_Left_Begin; bn = S_Left_Begin + 1;
       swapped = true;
     }
-    DPOut o = wave_prefix_dp<CAP>(a, am, b, bn, Error_Limit - Right_Errors, false, need_delta, C.tb, bufA, bufB, sc, cells, slots);
+    DPOut o = wave_prefix_dp<CAP>(a, am, b, bn, Error_Limit - Right_Errors, false, need_delta, C.tb, bufA, bufB, sc, cells, slots, lane_bases);
     dpcalls++;
     if (o.overflow) { r.overflow = 1; return r; }
     Left_Errors = o.e; lmte = o.mte; nl = o.ndelta; Leftover = o.leftover;
@@ -476,7 +477,9 @@ __device__ ExtOut wave_extend(int32_t Start, int32_t Offset, int32_t Len, const 
   r.s_lo = S_Lo; r.s_hi = S_Hi; r.t_lo = T_Lo; r.t_hi = T_Hi;
 
   if (!need_delta && r.kind == 3) {
-    // both sides matched to the end: rerun the two DPs with row storage for the deltas
+    // both sides matched to the end: rerun the two DPs with row storage for the deltas (their
+    // cells repeat work already counted)
+    uint64_t rc_cells = 0, rc_slots = 0;
     int32_t re_e = 0, rr = 0;
     {
       SeqV a = S, b = T;
@@ -485,8 +488,8 @@ __device__ ExtOut wave_extend(int32_t Start, int32_t Offset, int32_t Len, const 
       else if (S_Right_Len <= T_Right_Len) { a.b0 += S_Right_Begin; am = S_Right_Len; b.b0 += T_Right_Begin; bn = T_Right_Len; }
       else { a = T; b = S; a.b0 += T_Right_Begin; am = T_Right_Len; b.b0 += S_Right_Begin; bn = S_Right_Len; }
       if (am > 0) {
-        DPOut o = wave_prefix_dp<CAP>(a, am, b, bn, Error_Limit, true, true, C.tb, bufA, bufB, sc, cells, slots);
-        dpcalls++;
+        uint64_t lb = 0;
+        DPOut o = wave_prefix_dp<CAP>(a, am, b, bn, Error_Limit, true, true, C.tb, bufA, bufB, sc, rc_cells, rc_slots, lb);
         if (o.overflow) { r.overflow = 1; return r; }
         rr = o.ndelta; re_e = o.e;
       }
@@ -509,8 +512,8 @@ __device__ ExtOut wave_extend(int32_t Start, int32_t Offset, int32_t Len, const 
         b = Src; b.b0 += S_Len - 1 - S_Left_Begin; bn = S_Left_Begin + 1;
         swapped = true;
       }
-      DPOut o = wave_prefix_dp<CAP>(a, am, b, bn, Error_Limit - Right_Errors, false, true, C.tb, bufA, bufB, sc, cells, slots);
-      dpcalls++;
+      uint64_t lb = 0;
+      DPOut o = wave_prefix_dp<CAP>(a, am, b, bn, Error_Limit - Right_Errors, false, true, C.tb, bufA, bufB, sc, rc_cells, rc_slots, lb);
       if (o.overflow) { r.overflow = 1; return r; }
       nl = o.ndelta; Leftover = o.leftover;
       // Set_Left_Delta may move T_End by one (reverse.C:76-92): take the coordinates from this run
@@ -673,6 +676,7 @@ struct PairOut {
 #define STAT_SLOTS   9
 #define STAT_EXT     10
 #define STAT_DPCALLS 11
+#define STAT_BASES   12
 #define STAT_NSTATS  16
 
 struct PairKernelArgs {
@@ -686,6 +690,9 @@ struct PairKernelArgs {
   PairOut *pout;             // [np]
   uint32_t *queue;           // work counter
   unsigned long long *stats; // [STAT_NSTATS]
+  const uint8_t *cache;      // per node forward outcome (1 = ended at a branch point), or nullptr
+  const uint32_t *ncells;    // per node cells of that forward run
+  const int32_t *nexit;      // per node right-end diagonal of that run
   DPScratch sc0;             // scratch of wave 0; wave w adds w * stride
   int64_t row_stride, val_stride, delta_stride;
 };
@@ -716,7 +723,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_process_pairs(PairKernelArgs A) {
   const DPScratch sc = wave_scratch<CAP>(A, w);
   const SearchConst &C = A.C;
 
-  uint64_t cells = 0, slots = 0;
+  uint64_t cells = 0, slots = 0, lane_bases = 0;
   uint32_t dpcalls = 0, exts = 0;
   while (true) {
     if (lane == 0) next_item = atomicAdd(A.queue, 1u);
@@ -779,7 +786,16 @@ __global__ void __launch_bounds__(OIC_WAVE) k_process_pairs(PairKernelArgs A) {
       wave_argmax_first(bl, bi);
       if (bl < 0) break;
       const int4 lm = nodes[bi];
-      ExtOut x = wave_extend<CAP>(lm.z, lm.x, lm.y, S, Src, S_Len, T, Trc, T_Len, C, bufA, bufB, sc, cells, slots, dpcalls);
+      ExtOut x;
+      if (A.cache && A.cache[pr.hb + bi] == 1) {
+        // forward already known to end at a branch point: only this node goes (:473-490)
+        x = ExtOut{};
+        x.kind = 0;
+        x.s_hi = 0; x.t_hi = A.nexit[pr.hb + bi];
+        cells += A.ncells[pr.hb + bi];
+      } else {
+        x = wave_extend<CAP>(lm.z, lm.x, lm.y, S, Src, S_Len, T, Trc, T_Len, C, bufA, bufB, sc, cells, slots, dpcalls, lane_bases);
+      }
       exts++;
       po.n_ext++;
       if (x.overflow) { overflow = true; break; }
@@ -902,7 +918,9 @@ __global__ void __launch_bounds__(OIC_WAVE) k_process_pairs(PairKernelArgs A) {
     }
     __syncthreads();
   }
+  for (int o = 32; o > 0; o >>= 1) lane_bases += __shfl_xor(lane_bases, o);
   if (lane == 0) {
+    atomicAdd(&A.stats[STAT_BASES], (unsigned long long)lane_bases);
     atomicAdd(&A.stats[STAT_CELLS], (unsigned long long)cells);
     atomicAdd(&A.stats[STAT_SLOTS], (unsigned long long)slots);
     atomicAdd(&A.stats[STAT_EXT], (unsigned long long)exts);

@@ -59,8 +59,10 @@ class Sink(ctypes.Structure):
 
 class Counters(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("kmer_probes", "hits", "pairs", "nodes", "extensions", "dp_calls",
-                                               "dp_cells", "dp_lane_slots", "overflow_retries")] + \
-               [(n, ctypes.c_double) for n in ("ms_index", "ms_gather", "ms_lists", "ms_extend", "ms_total", "ms_dp_kernel")]
+                                               "dp_cells", "dp_lane_slots", "dp_bases", "overflow_retries",
+                                               "spec_tasks", "spec_cells")] + \
+               [(n, ctypes.c_double) for n in ("ms_index", "ms_gather", "ms_lists", "ms_extend", "ms_total", "ms_dp_kernel",
+                                               "ms_fwd_kernel")]
 
 
 EXT_TASK = np.dtype([("s_id", "<u4"), ("t_id", "<u4"), ("s_dir", "<i4"), ("start", "<i4"), ("offset", "<i4"), ("len", "<i4")])

@@ -110,9 +110,13 @@ typedef struct oic_counters {
   uint64_t dp_calls;           /* forward / reverse DP runs executed */
   uint64_t dp_cells;           /* (e, d) cell updates executed, row-0 slide counted as one */
   uint64_t dp_lane_slots;      /* lane-cells issued (cells plus idle lanes of partial chunks) */
+  uint64_t dp_bases;           /* bases compared in slides, plus one terminating compare per cell */
   uint64_t overflow_retries;   /* DP tasks re-run with a wider band tile */
+  uint64_t spec_tasks;         /* forward DPs run ahead of Process_Matches (batched schedules) */
+  uint64_t spec_cells;         /* cells of those runs, used or not */
   double   ms_index, ms_gather, ms_lists, ms_extend, ms_total;
-  double   ms_dp_kernel;       /* device time of the extension kernels (HIP events) */
+  double   ms_dp_kernel;       /* device time of all DP kernels (HIP events) */
+  double   ms_fwd_kernel;      /* of which the packed forward-DP kernels */
 } oic_counters;
 
 int         oic_create(const oic_params *p, oic_ctx **out);

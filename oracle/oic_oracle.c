@@ -173,6 +173,7 @@ struct oo_dp {
   int32_t  *Left_Delta, *Right_Delta, *Delta_Stack;
   int32_t   Left_Delta_Len, Right_Delta_Len;
   uint64_t  cells, bases, calls, rows;
+  uint64_t  elided;            /* cells of reverse() runs whose result Process_Matches cannot use */
   int       poison;
   int       poison_hits;
 };
@@ -615,13 +616,17 @@ int32_t oo_extend(oo_dp *W, const oo_match *Match,
     *T_Lo = 0;
     lMatchToEnd = 1;
   } else if (S_Right_Begin <= T_Right_Begin) {
+    uint64_t c0 = W->cells;
     Left_Errors = oo_reverse(W, S + S_Left_Begin, S_Left_Begin + 1, T + T_Left_Begin, T_Left_Begin + 1,
                              Error_Limit - Right_Errors, S_Lo, T_Lo, &Leftover, &lMatchToEnd);
+    if (!rMatchToEnd && !W->tb->partial) W->elided += W->cells - c0;
   } else {
+    uint64_t c0 = W->cells;
     Left_Errors = oo_reverse(W, T + T_Left_Begin, T_Left_Begin + 1, S + S_Left_Begin, S_Left_Begin + 1,
                              Error_Limit - Right_Errors, T_Lo, S_Lo, &Leftover, &lMatchToEnd);
     for (int32_t i = 0; i < W->Left_Delta_Len; i++)
       W->Left_Delta[i] *= -1;
+    if (!rMatchToEnd && !W->tb->partial) W->elided += W->cells - c0;
   }
 
   *S_Lo += S_Left_Begin + 1;
@@ -1625,6 +1630,7 @@ int oo_run(const oo_params *Pin, const char *bases, const uint64_t *start, const
     out->stats[4] += WA->Contained_Overlap_Ct;
     out->stats[5] += WA->Dovetail_Overlap_Ct;
     out->cells += WA->ed->cells; out->bases += WA->ed->bases; out->dp_calls += WA->ed->calls;
+    out->cells_used += WA->ed->cells - WA->ed->elided;
     out->extensions += WA->extensions; out->hits += WA->hits; out->pairs += WA->pairs;
     wa_free(WA);
   }

@@ -114,6 +114,7 @@ typedef struct oo_result {
   oo_record     *rec;      uint64_t nrec;
   uint64_t       stats[8]; /* overlapInCore.C:550-558 order */
   uint64_t       cells, bases, dp_calls, extensions, hits, pairs;
+  uint64_t       cells_used;     /* cells minus reverse() runs after a forward() that did not reach the end */
   oo_pair_trace *trace;    uint64_t ntrace;   /* only when collect_trace */
   uint32_t       hashEnd[64]; uint32_t nblocks;   /* hash block boundaries actually used */
   double         t_index, t_search;

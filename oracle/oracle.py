@@ -54,6 +54,7 @@ class Result(ctypes.Structure):
         ("rec", ctypes.POINTER(Record)), ("nrec", ctypes.c_uint64), ("stats", ctypes.c_uint64 * 8),
         ("cells", ctypes.c_uint64), ("bases", ctypes.c_uint64), ("dp_calls", ctypes.c_uint64),
         ("extensions", ctypes.c_uint64), ("hits", ctypes.c_uint64), ("pairs", ctypes.c_uint64),
+        ("cells_used", ctypes.c_uint64),
         ("trace", ctypes.c_void_p), ("ntrace", ctypes.c_uint64),
         ("hashEnd", ctypes.c_uint32 * 64), ("nblocks", ctypes.c_uint32),
         ("t_index", ctypes.c_double), ("t_search", ctypes.c_double),
@@ -249,7 +250,8 @@ def run(reads, collect_trace=False, **kw):
         if collect_trace and res.ntrace:
             buf = ctypes.string_at(res.trace, res.ntrace * TRACE_DTYPE.itemsize)
             trace = np.frombuffer(buf, TRACE_DTYPE).copy()
-        out = dict(records=recs, stats=list(res.stats), cells=res.cells, bases=res.bases, dp_calls=res.dp_calls,
+        out = dict(records=recs, stats=list(res.stats), cells=res.cells, cells_used=res.cells_used, bases=res.bases,
+                   dp_calls=res.dp_calls,
                    extensions=res.extensions, hits=res.hits, pairs=res.pairs, trace=trace,
                    hash_end=list(res.hashEnd[:min(res.nblocks, 64)]), t_index=res.t_index, t_search=res.t_search)
     finally:

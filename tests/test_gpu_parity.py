@@ -171,15 +171,26 @@ def c3s():
 
 
 def test_run_c1_small(c1s):
-    assert_same(*run_both(c1s))
+    res, ref = run_both(c1s)
+    assert_same(res, ref)
+    # the DP work Process_Matches consumes, cell for cell
+    assert res.counters["dp_cells"] == ref["cells_used"]
+    assert res.counters["extensions"] == ref["extensions"]
 
 
-def test_run_c2_small(c2s):
-    assert_same(*run_both(c2s))
+@pytest.mark.parametrize("schedule", [oic.SCHED_PLAIN, oic.SCHED_BATCHED])
+def test_run_c2_small(c2s, schedule):
+    res, ref = run_both(c2s, schedule=schedule)
+    assert_same(res, ref)
+    assert res.counters["dp_cells"] == ref["cells_used"]
+    assert res.counters["extensions"] == ref["extensions"]
 
 
-def test_run_c3_small_repeats(c3s):
-    assert_same(*run_both(c3s))
+@pytest.mark.parametrize("schedule", [oic.SCHED_PLAIN, oic.SCHED_BATCHED])
+def test_run_c3_small_repeats(c3s, schedule):
+    res, ref = run_both(c3s, schedule=schedule)
+    assert_same(res, ref)
+    assert res.counters["dp_cells"] == ref["cells_used"]
 
 
 @pytest.mark.parametrize("opt", [dict(unique=False), dict(partial=True), dict(hopeless=False), dict(minkmers=True),
