@@ -276,7 +276,9 @@ extern "C" int oic_load_reads(oic_ctx *c, const oic_reads *rd) {
   }
   if (n == 0) { lo = hi = 0; }
   for (uint32_t r = 1; r <= n; r++) start[r] = rd->start[r] - lo;
-  const uint64_t nwords = (uint64_t)acc / 16 + 4, nmw = (uint64_t)acc / 32 + 4;
+  // slack after the last strand: the packed forward-DP kernels stage OIC_SEQ_WIN words from a
+  // tail's start whatever its length
+  const uint64_t nwords = (uint64_t)acc / 16 + 4 + 2 * OIC_SEQ_WIN, nmw = (uint64_t)acc / 32 + 4;
   ALLOC(c, c->d_words, nwords * 4);
   ALLOC(c, c->d_nmask, nmw * 4);
   ALLOC(c, c->d_b0f, ((size_t)n + 1) * 8);

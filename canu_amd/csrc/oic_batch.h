@@ -6,11 +6,12 @@
 // the coordinates are never used.  A node's forward DP depends on nothing but the node, so the
 // forward DPs of many nodes run here ahead of Process_Matches, as independent tasks, and their
 // outcome is cached per node:
-//   cache[node] = 1  forward ended at a branch point / error limit (Process_Matches skips the DP)
-//                 2  forward reached the end (Process_Matches runs the whole extension)
+//   cache[node] = 1  not a dovetail extension (Process_Matches skips the DP, removes the node)
+//                 2  dovetail (Process_Matches runs the whole extension for the deltas)
 //                 0  not computed (band outgrew the tile)
 // The DP below is wave_prefix_dp's row loop without row storage or traceback, on a G-lane group,
-// with the first 1024 bases of both tails staged in LDS.
+// with the first 1024 bases of both tails staged in LDS.  When the forward DP reaches the end, the
+// reverse DP on the left tails runs too, so the cached outcome is the extension's kind.
 #pragma once
 
 #include "oic_search.h"
@@ -56,10 +57,153 @@ __device__ __forceinline__ uint64_t group_ballot(bool p, int g) {
   return (G == 64) ? b : ((b >> (g * G)) & ((1ull << G) - 1));
 }
 
+struct GroupDP {
+  bool mte;
+  bool overflow;
+  int32_t e;          // errors at the end (mte) or of the best branch point
+  int32_t exit_d;     // diagonal of the end cell / best branch point
+  uint64_t cells;
+};
+
+// One prefix DP (forward() row loop, no row storage) on a G-lane group; sa/st are the group's LDS
+// copies of the first OIC_SEQ_WIN words of a and b.
+template <int G, int CAP>
+__device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, const int32_t n, const int32_t limit,
+                                   const DPTables &tb, int32_t *pbuf, int32_t *cbuf, uint32_t *sa, uint32_t *sb, int g, int gl) {
+  constexpr int32_t MASK = CAP - 1;
+  GroupDP r;
+  r.mte = false; r.overflow = false; r.e = 0; r.exit_d = 0; r.cells = 1;
+  const bool nA = a.nm != nullptr, nB = b.nm != nullptr;
+  const int64_t wa0 = a.b0 >> 4, wb0 = b.b0 >> 4;
+  for (int i = gl; i < OIC_SEQ_WIN; i += G) {
+    sa[i] = a.w[wa0 + i];
+    sb[i] = b.w[wb0 + i];
+  }
+  wave_fence();
+  int32_t row0 = 0;
+  for (int32_t base = 0;; base += 16 * G) {
+    const int32_t off = base + 16 * gl;
+    const int32_t rem = m - off;
+    int32_t k = 0;
+    bool stop = true;
+    if (rem > 0) {
+      uint32_t x = win16_staged(a, sa, wa0, off) ^ win16_staged(b, sb, wb0, off);
+      x = (x | (x >> 1)) & 0x55555555u;
+      if (nA) x &= ~nwin16(a.nm, a.b0 + off);
+      if (nB) x &= ~nwin16(b.nm, b.b0 + off);
+      const int32_t kk = x ? (int32_t)(__builtin_ctz(x) >> 1) : 16;
+      if (kk >= rem) { k = rem; stop = true; } else { k = kk; stop = kk < 16; }
+    }
+    const uint64_t bl = group_ballot<G>(stop, g);
+    if (bl) {
+      const int f = __builtin_ctzll(bl);
+      row0 = base + 16 * f + __shfl(k, g * G + f);
+      break;
+    }
+  }
+  if (row0 == m) { r.mte = true; return r; }
+  if (gl == 0) pbuf[0] = row0;
+  int32_t Left = 0, Right = 0, Longest = 0, Best_d = 0, Best_e = 0, MSL = 0, MSd = 0, MSe = 0;
+  double MaxScore = 0.0;
+  for (int32_t e = 1; e <= limit; e++) {
+    Left = (Left - 1 > -e) ? Left - 1 : -e;
+    Right = (Right + 1 < e) ? Right + 1 : e;
+    if (Right - Left + 3 > CAP) { r.overflow = true; return r; }
+    wave_fence();
+    if (gl < 4) {
+      const int32_t sd = (gl == 0) ? Left : (gl == 1) ? Left - 1 : (gl == 2) ? Right : Right + 1;
+      pbuf[sd & MASK] = -2;
+    }
+    wave_fence();
+    for (int32_t c0 = Left; c0 <= Right; c0 += G) {
+      const int32_t d = c0 + gl;
+      const bool act = d <= Right;
+      int32_t Row = 0;
+      if (act) {
+        const int32_t x1 = 1 + pbuf[d & MASK], x2 = pbuf[(d - 1) & MASK], x3 = 1 + pbuf[(d + 1) & MASK];
+        Row = x1 > x2 ? x1 : x2;
+        Row = Row > x3 ? Row : x3;
+        const int32_t r1 = m - Row, r2 = n - Row - d;
+        int32_t rem = r1 < r2 ? r1 : r2;
+        int32_t adv = 0;
+        while (rem > 0) {
+          uint32_t x = win16_staged(a, sa, wa0, Row + adv) ^ win16_staged(b, sb, wb0, (int64_t)Row + d + adv);
+          x = (x | (x >> 1)) & 0x55555555u;
+          if (nA) x &= ~nwin16(a.nm, a.b0 + Row + adv);
+          if (nB) x &= ~nwin16(b.nm, b.b0 + Row + d + adv);
+          const int32_t kk = x ? (int32_t)(__builtin_ctz(x) >> 1) : 16;
+          if (kk >= rem) { adv += rem; break; }
+          adv += kk;
+          if (kk < 16) break;
+          rem -= 16;
+        }
+        Row += adv;
+        cbuf[d & MASK] = Row;
+      }
+      const bool reach = act && (Row == m || Row + d == n);
+      const uint64_t bl = group_ballot<G>(reach, g);
+      if (bl) {
+        const int f = __builtin_ctzll(bl);
+        r.cells += (uint64_t)(f + 1);
+        const int32_t rd = c0 + f;
+        const int32_t rrow = __shfl(Row, g * G + f);
+        const double Score = (double)rrow * tb.bmv - (double)e;
+        const int32_t Tail_Len = rrow - MSL;
+        const double slope = (double)(MaxScore - Score) / (double)Tail_Len;
+        const bool abort_ = ((uint32_t)e > tb.min_branch_end_dist / 2u) && ((uint32_t)Tail_Len >= tb.min_branch_end_dist) &&
+                            (slope >= tb.slope_min);
+        if (abort_) { r.mte = false; r.e = MSe; r.exit_d = MSd; }
+        else        { r.mte = true;  r.e = e;   r.exit_d = rd; }
+        return r;
+      }
+      r.cells += (uint64_t)((Right - c0 + 1) < G ? (Right - c0 + 1) : G);
+    }
+    wave_fence();
+    const int32_t eml = tb.match_limit[e];
+    int32_t nL = Right + 1;
+    for (int32_t c0 = Left; c0 <= Right; c0 += G) {
+      const int32_t d = c0 + gl;
+      const bool keep = (d <= Right) && (cbuf[d & MASK] + (d > 0 ? d : 0) >= eml);
+      const uint64_t bl = group_ballot<G>(keep, g);
+      if (bl) { nL = c0 + __builtin_ctzll(bl); break; }
+    }
+    if (nL > Right) break;
+    int32_t nR = nL;
+    for (int32_t c1 = Right; c1 >= nL; c1 -= G) {
+      const int32_t d = c1 - gl;
+      const bool keep = (d >= nL) && (cbuf[d & MASK] + (d > 0 ? d : 0) >= eml);
+      const uint64_t bl = group_ballot<G>(keep, g);
+      if (bl) { nR = c1 - __builtin_ctzll(bl); break; }
+    }
+    Left = nL;
+    Right = nR;
+    int32_t bv = -1, bd = 0x7fffffff;
+    for (int32_t c0 = Left; c0 <= Right; c0 += G) {
+      const int32_t d = c0 + gl;
+      if (d <= Right) {
+        const int32_t v = cbuf[d & MASK];
+        if (v > bv) { bv = v; bd = d; }
+      }
+    }
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+      const int32_t ov = __shfl_xor(bv, o), od = __shfl_xor(bd, o);
+      if (ov > bv || (ov == bv && od < bd)) { bv = ov; bd = od; }
+    }
+    if (bv > Longest) { Longest = bv; Best_d = bd; Best_e = e; }
+    const double Score = (double)Longest * tb.bmv - (double)e;
+    if (Score > MaxScore) { MaxScore = Score; MSL = Longest; MSd = Best_d; MSe = Best_e; }
+    int32_t *t = pbuf; pbuf = cbuf; cbuf = t;
+  }
+  r.mte = false; r.e = MSe; r.exit_d = MSd;
+  return r;
+}
+
+// Extend_Alignment's outcome for one node without deltas: forward on the right tails, then (only
+// if it reached the end) reverse on the left tails with the remaining error budget.
 template <int G, int CAP>
 __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
   constexpr int NG = OIC_WAVE / G;
-  constexpr int32_t MASK = CAP - 1;
   __shared__ int32_t tile[NG][2][CAP];
   __shared__ uint32_t seqA[NG][OIC_SEQ_WIN], seqT[NG][OIC_SEQ_WIN];
   __shared__ uint32_t fetched[NG];
@@ -68,7 +212,6 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
   uint64_t cells_acc = 0;
 
   while (true) {
-    // ---- fetch a task for this group
     if (gl == 0) fetched[g] = atomicAdd(A.queue, 1u);
     wave_fence();
     const uint32_t ti = fetched[g];
@@ -79,172 +222,52 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
     const int4 nd = A.nodes[pr.hb + tk.node];
     const int32_t S_Len = A.R.len[pr.s_id], T_Len = A.R.len[pr.t_id];
     const int32_t Start = nd.z, Offset = nd.x, Len = nd.y;
+    const int32_t S_Left_Begin = Start - 1, T_Left_Begin = Offset - 1;
     const int32_t S_Right_Begin = Start + Len, S_Right_Len = S_Len - S_Right_Begin;
     const int32_t T_Right_Begin = Offset + Len, T_Right_Len = T_Len - T_Right_Begin;
     const uint64_t gnode = pr.hb + tk.node;
-    if (S_Right_Len == 0 || T_Right_Len == 0) {   // nothing to the right: matches to the end
-      if (gl == 0) { A.cache[gnode] = 2; A.ncells[gnode] = 0; A.nexit[gnode] = 0; }
-      continue;
-    }
     const int32_t Total_Olap = (Start < Offset ? Start : Offset) + Len + (S_Right_Len < T_Right_Len ? S_Right_Len : T_Right_Len);
     const int32_t limit = A.tb.error_bound[Total_Olap];
-    SeqV S = read_view(A.R, pr.s_id, pr.dir), T = read_view(A.R, pr.t_id, 0);
-    SeqV a, b;
-    int32_t m, n;
-    const bool s_short = S_Right_Len <= T_Right_Len;
-    if (s_short) { a = S; a.b0 += S_Right_Begin; m = S_Right_Len; b = T; b.b0 += T_Right_Begin; n = T_Right_Len; }
-    else         { a = T; a.b0 += T_Right_Begin; m = T_Right_Len; b = S; b.b0 += S_Right_Begin; n = S_Right_Len; }
-    const bool nA = a.nm != nullptr, nB = b.nm != nullptr;
-    // stage the first OIC_SEQ_WIN words of each tail
-    const int64_t wa0 = a.b0 >> 4, wb0 = b.b0 >> 4;
-    for (int i = gl; i < OIC_SEQ_WIN; i += G) {
-      seqA[g][i] = a.w[wa0 + i];
-      seqT[g][i] = b.w[wb0 + i];
+    const SeqV S = read_view(A.R, pr.s_id, pr.dir), T = read_view(A.R, pr.t_id, 0);
+    uint64_t cells = 0;
+    bool rmte = true, overflow = false;
+    int32_t right_e = 0, exit_d = T_Right_Begin - S_Right_Begin;
+    if (!(S_Right_Len == 0 || T_Right_Len == 0)) {
+      SeqV a, b;
+      int32_t m, n;
+      const bool s_short = S_Right_Len <= T_Right_Len;
+      if (s_short) { a = S; a.b0 += S_Right_Begin; m = S_Right_Len; b = T; b.b0 += T_Right_Begin; n = T_Right_Len; }
+      else         { a = T; a.b0 += T_Right_Begin; m = T_Right_Len; b = S; b.b0 += S_Right_Begin; n = S_Right_Len; }
+      const GroupDP f = group_prefix_dp<G, CAP>(a, m, b, n, limit, A.tb, tile[g][0], tile[g][1], seqA[g], seqT[g], g, gl);
+      cells += f.cells;
+      overflow = f.overflow;
+      rmte = f.mte;
+      right_e = f.e;
+      exit_d += s_short ? f.exit_d : -f.exit_d;
     }
-    wave_fence();
-    int32_t *pbuf = tile[g][0], *cbuf = tile[g][1];
-
-    // row 0: group-cooperative slide of 16 * G bases per step
-    int32_t row0 = 0;
-    for (int32_t base = 0;; base += 16 * G) {
-      const int32_t off = base + 16 * gl;
-      const int32_t rem = m - off;
-      int32_t k = 0;
-      bool stop = true;
-      if (rem > 0) {
-        uint32_t x = win16_staged(a, seqA[g], wa0, off) ^ win16_staged(b, seqT[g], wb0, off);
-        x = (x | (x >> 1)) & 0x55555555u;
-        if (nA) x &= ~nwin16(a.nm, a.b0 + off);
-        if (nB) x &= ~nwin16(b.nm, b.b0 + off);
-        const int32_t kk = x ? (int32_t)(__builtin_ctz(x) >> 1) : 16;
-        if (kk >= rem) { k = rem; stop = true; } else { k = kk; stop = kk < 16; }
+    bool lmte = true;
+    if (!overflow && rmte && !(S_Left_Begin < 0 || T_Left_Begin < 0)) {
+      const SeqV Src = read_view(A.R, pr.s_id, 1 - pr.dir), Trc = read_view(A.R, pr.t_id, 1);
+      SeqV a, b;
+      int32_t m, n;
+      if (S_Right_Begin <= T_Right_Begin) {
+        a = Src; a.b0 += S_Len - 1 - S_Left_Begin; m = S_Left_Begin + 1;
+        b = Trc; b.b0 += T_Len - 1 - T_Left_Begin; n = T_Left_Begin + 1;
+      } else {
+        a = Trc; a.b0 += T_Len - 1 - T_Left_Begin; m = T_Left_Begin + 1;
+        b = Src; b.b0 += S_Len - 1 - S_Left_Begin; n = S_Left_Begin + 1;
       }
-      const uint64_t bl = group_ballot<G>(stop, g);
-      if (bl) {
-        const int f = __builtin_ctzll(bl);
-        row0 = base + 16 * f + __shfl(k, g * G + f);
-        break;
-      }
-    }
-    uint64_t cells = 1;
-    bool mte = false;
-    int32_t exit_d = 0;
-    if (row0 == m) {
-      mte = true;
-    } else {
-      if (gl == 0) pbuf[0] = row0;
-      int32_t Left = 0, Right = 0, Longest = 0, Best_d = 0, MSL = 0, MSd = 0;
-      double MaxScore = 0.0;
-      bool ended = false, overflow = false;
-      for (int32_t e = 1; e <= limit; e++) {
-        Left = (Left - 1 > -e) ? Left - 1 : -e;
-        Right = (Right + 1 < e) ? Right + 1 : e;
-        if (Right - Left + 3 > CAP) { overflow = true; break; }
-        wave_fence();
-        if (gl < 4) {
-          const int32_t sd = (gl == 0) ? Left : (gl == 1) ? Left - 1 : (gl == 2) ? Right : Right + 1;
-          pbuf[sd & MASK] = -2;
-        }
-        wave_fence();
-        int32_t rd = 0, rrow = 0;
-        bool reached = false;
-        for (int32_t c0 = Left; c0 <= Right; c0 += G) {
-          const int32_t d = c0 + gl;
-          const bool act = d <= Right;
-          int32_t Row = 0;
-          if (act) {
-            const int32_t x1 = 1 + pbuf[d & MASK], x2 = pbuf[(d - 1) & MASK], x3 = 1 + pbuf[(d + 1) & MASK];
-            Row = x1 > x2 ? x1 : x2;
-            Row = Row > x3 ? Row : x3;
-            const int32_t r1 = m - Row, r2 = n - Row - d;
-            int32_t rem = r1 < r2 ? r1 : r2;
-            int32_t adv = 0;
-            while (rem > 0) {
-              uint32_t x = win16_staged(a, seqA[g], wa0, Row + adv) ^ win16_staged(b, seqT[g], wb0, (int64_t)Row + d + adv);
-              x = (x | (x >> 1)) & 0x55555555u;
-              if (nA) x &= ~nwin16(a.nm, a.b0 + Row + adv);
-              if (nB) x &= ~nwin16(b.nm, b.b0 + Row + d + adv);
-              const int32_t kk = x ? (int32_t)(__builtin_ctz(x) >> 1) : 16;
-              if (kk >= rem) { adv += rem; break; }
-              adv += kk;
-              if (kk < 16) break;
-              rem -= 16;
-            }
-            Row += adv;
-            cbuf[d & MASK] = Row;
-          }
-          const bool reach = act && (Row == m || Row + d == n);
-          const uint64_t bl = group_ballot<G>(reach, g);
-          if (bl) {
-            const int f = __builtin_ctzll(bl);
-            cells += (uint64_t)(f + 1);
-            rd = c0 + f;
-            rrow = __shfl(Row, g * G + f);
-            reached = true;
-            break;
-          }
-          cells += (uint64_t)((Right - c0 + 1) < G ? (Right - c0 + 1) : G);
-        }
-        if (reached) {
-          const double Score = (double)rrow * A.tb.bmv - (double)e;
-          const int32_t Tail_Len = rrow - MSL;
-          const double slope = (double)(MaxScore - Score) / (double)Tail_Len;
-          const bool abort_ = ((uint32_t)e > A.tb.min_branch_end_dist / 2u) && ((uint32_t)Tail_Len >= A.tb.min_branch_end_dist) &&
-                              (slope >= A.tb.slope_min);
-          if (abort_) { mte = false; exit_d = MSd; }
-          else        { mte = true; exit_d = rd; }
-          ended = true;
-          break;
-        }
-        wave_fence();
-        const int32_t eml = A.tb.match_limit[e];
-        int32_t nL = Right + 1;
-        for (int32_t c0 = Left; c0 <= Right; c0 += G) {
-          const int32_t d = c0 + gl;
-          const bool keep = (d <= Right) && (cbuf[d & MASK] + (d > 0 ? d : 0) >= eml);
-          const uint64_t bl = group_ballot<G>(keep, g);
-          if (bl) { nL = c0 + __builtin_ctzll(bl); break; }
-        }
-        if (nL > Right) break;
-        int32_t nR = nL;
-        for (int32_t c1 = Right; c1 >= nL; c1 -= G) {
-          const int32_t d = c1 - gl;
-          const bool keep = (d >= nL) && (cbuf[d & MASK] + (d > 0 ? d : 0) >= eml);
-          const uint64_t bl = group_ballot<G>(keep, g);
-          if (bl) { nR = c1 - __builtin_ctzll(bl); break; }
-        }
-        Left = nL;
-        Right = nR;
-        int32_t bv = -1, bd = 0x7fffffff;
-        for (int32_t c0 = Left; c0 <= Right; c0 += G) {
-          const int32_t d = c0 + gl;
-          if (d <= Right) {
-            const int32_t v = cbuf[d & MASK];
-            if (v > bv) { bv = v; bd = d; }
-          }
-        }
-#pragma unroll
-        for (int o = 1; o < G; o <<= 1) {
-          const int32_t ov = __shfl_xor(bv, o), od = __shfl_xor(bd, o);
-          if (ov > bv || (ov == bv && od < bd)) { bv = ov; bd = od; }
-        }
-        if (bv > Longest) { Longest = bv; Best_d = bd; }
-        const double Score = (double)Longest * A.tb.bmv - (double)e;
-        if (Score > MaxScore) { MaxScore = Score; MSL = Longest; MSd = Best_d; }
-        int32_t *t = pbuf; pbuf = cbuf; cbuf = t;
-      }
-      if (!ended && !overflow) { mte = false; exit_d = MSd; }
-      if (overflow) {
-        if (gl == 0) A.cache[gnode] = 0;
-        continue;
-      }
+      const GroupDP l = group_prefix_dp<G, CAP>(a, m, b, n, limit - right_e, A.tb, tile[g][0], tile[g][1], seqA[g], seqT[g], g, gl);
+      cells += l.cells;
+      overflow = l.overflow;
+      lmte = l.mte;
     }
     if (gl == 0) {
-      A.cache[gnode] = mte ? 2 : 1;
+      // 1: not a dovetail (Process_Matches only removes the node), 2: dovetail, 0: not computed
+      A.cache[gnode] = overflow ? 0 : ((rmte && lmte) ? 2 : 1);
       A.ncells[gnode] = (uint32_t)(cells > 0xffffffffull ? 0xffffffffull : cells);
-      // right-end diagonal T_Hi - S_Hi of the run, in S/T orientation
-      A.nexit[gnode] = (T_Right_Begin - S_Right_Begin) + (s_short ? exit_d : -exit_d);
-      cells_acc += cells;
+      A.nexit[gnode] = exit_d;
+      if (!overflow) cells_acc += cells;
     }
   }
   for (int o = 32; o > 0; o >>= 1) cells_acc += __shfl_xor(cells_acc, o);

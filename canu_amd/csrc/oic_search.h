@@ -776,6 +776,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_process_pairs(PairKernelArgs A) {
     __syncthreads();
     bool overflow = false;
     bool first = true;
+    const uint8_t *cache = A.cache ? A.cache + pr.hb : nullptr;
     while (true) {
       // first node with the maximal Len (Process_String_Overlaps.C:424-431)
       int32_t bl = -1, bi = 0x7fffffff;
@@ -785,17 +786,40 @@ __global__ void __launch_bounds__(OIC_WAVE) k_process_pairs(PairKernelArgs A) {
       }
       wave_argmax_first(bl, bi);
       if (bl < 0) break;
-      const int4 lm = nodes[bi];
-      ExtOut x;
-      if (A.cache && A.cache[pr.hb + bi] == 1) {
-        // forward already known to end at a branch point: only this node goes (:473-490)
-        x = ExtOut{};
-        x.kind = 0;
-        x.s_hi = 0; x.t_hi = A.nexit[pr.hb + bi];
-        cells += A.ncells[pr.hb + bi];
-      } else {
-        x = wave_extend<CAP>(lm.z, lm.x, lm.y, S, Src, S_Len, T, Trc, T_Len, C, bufA, bufB, sc, cells, slots, dpcalls, lane_bases);
+      if (cache && cache[bi] == 1) {
+        // A node whose extension is known not to be a dovetail is only removed (:473-490); so are
+        // all such nodes that would be picked before the first node without that knowledge.
+        int32_t ul = -1, ui = 0x7fffffff;
+        for (int32_t i = lane; i < nn; i += OIC_WAVE) {
+          const int4 nd = nodes[i];
+          if (!nd.w && cache[i] != 1 && nd.y > ul) { ul = nd.y; ui = i; }
+        }
+        wave_argmax_first(ul, ui);
+        uint64_t cnt = 0, cs = 0;
+        if (pr.consistent) {
+          if (lane == 0) { cnt = 1; cs = A.ncells[pr.hb + bi]; nodes[bi].w = 1; }
+        } else {
+          for (int32_t i = lane; i < nn; i += OIC_WAVE) {
+            const int4 nd = nodes[i];
+            if (!nd.w && cache[i] == 1 && (nd.y > ul || (nd.y == ul && i < ui))) {
+              nodes[i].w = 1;
+              cnt++;
+              cs += A.ncells[pr.hb + i];
+            }
+          }
+        }
+        for (int o = 32; o > 0; o >>= 1) { cnt += __shfl_xor(cnt, o); cs += __shfl_xor(cs, o); }
+        exts += (uint32_t)cnt;
+        po.n_ext += (int32_t)cnt;
+        cells += cs;
+        if (first) { po.exit_d = A.nexit[pr.hb + bi]; first = false; }
+        __threadfence_block();
+        __syncthreads();
+        if (pr.consistent) break;
+        continue;
       }
+      const int4 lm = nodes[bi];
+      ExtOut x = wave_extend<CAP>(lm.z, lm.x, lm.y, S, Src, S_Len, T, Trc, T_Len, C, bufA, bufB, sc, cells, slots, dpcalls, lane_bases);
       exts++;
       po.n_ext++;
       if (x.overflow) { overflow = true; break; }
