@@ -1,10 +1,17 @@
 """overlapInCore throughput on one MI355X: DP cells/s and overlaps/s, learned-binned vs plain.
 
-Workload (default c2): 50 Mbp uniform genome, 30x lognormal reads (12 kbp mean, 5 kbp sd,
-<= 60 kbp), 1.5/1.5/1.0 % sub/ins/del; k=22, --maxerate 0.045, --minlength 500; one hash block
-over all reads (--hashbits 26 --hashload 0.8 --hashdatalen 2e9).  Reads are packed into HBM and
-the k-mer table is built before timing; a step is one overlapInCore search of a batch of ref
-reads (both orientations) against the whole table, records copied back to host memory.
+Workload (default c3): 50 Mbp genome, 25 % of it interspersed repeats (300 families of 3-8 kbp,
+copies 1-5 % diverged) and 20 tandem arrays of a 171 bp unit (50-200 kbp, 0.5 % unit divergence);
+30x lognormal reads (12 kbp mean, 5 kbp sd, <= 60 kbp), 1.5/1.5/1.0 % sub/ins/del; k=22,
+--maxerate 0.045, --minlength 500; one hash block over all reads (--hashbits 26 --hashload 0.8
+--hashdatalen 2e9).  Reads are packed into HBM and the k-mer table is built before timing; a step
+is one overlapInCore search of a batch of ref reads (both orientations) against the whole table,
+records copied back to host memory.
+
+DP cells are the (e, d) updates of the forward / reverse DP calls whose results Process_Matches
+uses (the reference's count without the reverse DPs it runs after a forward DP that did not reach
+the end); the engine's count equals the oracle's exactly.  The learned schedule's cost model is
+trained first, on the GPU, on traces of a fresh c3-generator instance (seed 4).
 
 Prints one JSON line.  --gpus must be 1: this build runs on one GPU.
 """
@@ -47,18 +54,19 @@ def traffic_from_profiles(kernel):
     return None
 
 
-def cpu_baseline(genome_len, threads):
-    """The oracle ('port') on the same read/error model at a smaller genome, timed on host cores."""
+def cpu_baseline(preset, genome_len, threads):
+    """The oracle ('port') on the same generator at a smaller genome, timed on host cores."""
     from canu_amd import readsim
     from oracle import oracle
-    cfg = readsim.PRESETS["c2"].scaled(genome_len, seed=2)
+    cfg = readsim.PRESETS[preset].scaled(genome_len, seed=readsim.PRESETS[preset].seed)
     R = readsim.simulate(cfg)
     out = oracle.run(R, kmer=22, maxErate=0.045, minOlap=500, nThreads=threads, ompThreads=threads)
-    return dict(value=out["cells"] / out["t_search"], unit="DP cells/s", cores=threads, kind="port",
-                sample=f"oracle on the c2 read/error model at a {genome_len/1e6:g} Mbp genome ({R.n} reads, all-vs-all): "
-                       f"{out['cells']:.3g} cells in {out['t_search']:.1f} s of search "
-                       f"(+{out['t_index']:.1f} s single-thread index build, not counted); {len(out['records'])} overlaps",
-                overlaps_per_s=len(out["records"]) / (out["t_search"] + out["t_index"]))
+    return dict(value=out["cells_used"] / out["t_search"], unit="DP cells/s", cores=threads, kind="port",
+                sample=f"oracle on the {preset} generator at a {genome_len/1e6:g} Mbp genome ({R.n} reads, all-vs-all): "
+                       f"{out['cells_used']:.3g} used cells ({out['cells']:.3g} incl. unused reverse DPs) in "
+                       f"{out['t_search']:.1f} s of search (+{out['t_index']:.1f} s single-thread index build, not counted); "
+                       f"{len(out['records'])} overlaps",
+                overlaps_per_s=len(out["records"]) / out["t_search"])
 
 
 def run_steps(eng, n, batch, steps, warmup, first):
@@ -89,12 +97,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--batch", type=int, default=2500, help="ref reads per step")
     ap.add_argument("--schedule", default="learned", choices=["plain", "batched", "learned"])
     ap.add_argument("--no-plain", action="store_true", help="skip the plain one-pair-per-wavefront comparison")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-genome", type=int, default=1_000_000)
+    ap.add_argument("--cpu-genome", type=int, default=500_000)
+    ap.add_argument("--train-genome", type=int, default=12_000_000, help="genome of the c3 instance the model learns from")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -119,11 +128,12 @@ def main():
     last = eng.build_hash_index(1, R.n)
     t_index = time.perf_counter() - t0
     log(f"load {t_load:.2f} s, index over reads 1-{last} in {t_index:.2f} s")
+    info = None
     if args.schedule == "learned":
         from canu_amd import learned
         t0 = time.perf_counter()
-        info = learned.train_for(eng, R, kw)
-        log(f"cost model: {info}")
+        info = learned.train_for(eng, engine_kw=kw, genome_len=args.train_genome)
+        log(f"cost model: {info} ({time.perf_counter() - t0:.1f} s)")
 
     first = 1
     per, recs, d = run_steps(eng, R.n, args.batch, args.steps, args.warmup, first)
@@ -144,6 +154,7 @@ def main():
         "overlaps_per_s": olap_s,
         "dp_cells_per_step": d["dp_cells"] / args.steps,
         "index_build_s": t_index,
+        "cost_model": info if args.schedule == "learned" else None,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT_TOPS, "unit": "TOPS",
                      "frac": achieved / PEAK_INT_TOPS, "traffic": traffic_from_profiles("k_process_pairs"),
                      "kernel": "k_process_pairs", "kernel_ms_per_step": d["ms_dp_kernel"] / launches,
@@ -164,7 +175,7 @@ def main():
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         log(f"cpu baseline: oracle, {threads} threads")
-        line["cpu_baseline"] = cpu_baseline(args.cpu_genome, threads)
+        line["cpu_baseline"] = cpu_baseline(W["preset"], args.cpu_genome, threads)
     print(json.dumps(line), flush=True)
 
 

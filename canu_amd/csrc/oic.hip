@@ -21,6 +21,9 @@
 #include "oic_search.h"
 #include "oic_sched.h"
 #include "oic_batch.h"
+#include "oic_mlp.h"
+#include <random>
+#include <numeric>
 
 namespace {
 
@@ -57,6 +60,7 @@ int ceil_log2(uint64_t x) {
 }
 
 constexpr int kCapSmall = 512;
+constexpr uint64_t kHitCap = 600000000ull;   // seed hits per search pass (about 50 GB of pass buffers)
 constexpr int kCapLarge = 8192;
 
 }  // namespace
@@ -83,19 +87,20 @@ struct oic_ctx {
   // sort / scan temporaries
   DevBuf s_k1, s_v1, s_k2, s_v2, s_hist, s_offs, s_scan, s_flag, s_idx, s_small;
   // search
-  DevBuf g_cnt, g_off, g_pbase, g_searched;
+  DevBuf g_cnt, g_off, g_pbase, g_searched, g_sdsum;
+  DevBuf l_scr;
   DevBuf p_pairs, p_tmp, p_nodes, p_recs, p_pout, p_order, p_queue, p_stats, p_nout, p_noff, p_outrec, p_retry;
   DevBuf sc_rowL, sc_rowR, sc_rowOff, sc_rowVal, sc_dbuf, sc_ld, sc_rd;
-  DevBuf b_cache, b_ncells, b_nexit, b_probe, b_spec, b_cnt, b_off, b_stats;
+  DevBuf b_cache, b_ncells, b_nexit, b_probe, b_spec, b_cnt, b_off, b_stats, b_order;
+  // learned cost model
+  bool tracing = false;
+  std::vector<float> tr_feat, tr_targ;
+  float mparam[MLP_NPARAM] = {0}, mmu[MLP_IN] = {0}, msd[MLP_IN] = {0};
+  DevBuf m_param, m_mu, m_sd, m_feat, m_pred;
   int32_t sc_rows = 0;
   int64_t sc_val = 0;
   uint32_t sc_waves = 0;
   oic_counters ctr{};
-  // last batch's pair table (host copies) for the scheduler / trace export
-  std::vector<PairFeat> last_feat;
-  std::vector<PairOut> last_pout;
-  bool keep_trace = false;
-  oic_sched_model model{};
   bool have_model = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -531,6 +536,24 @@ static uint32_t ref_end_searched(uint32_t bgn, uint32_t end, uint32_t nthreads) 
   return ((end - bgn) % per == 0) ? end - 1 : end;
 }
 
+// model prediction (log1p cells) for every pair of the batch, on the device
+static int predict_pairs(oic_ctx *c, uint64_t np, std::vector<float> &pred) {
+  ALLOC(c, c->m_feat, np * MLP_IN * 4);
+  ALLOC(c, c->m_pred, np * MLP_OUT * 4);
+  const unsigned g = (unsigned)((np + 255) / 256);
+  hipLaunchKernelGGL(k_pair_features, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->d_len.as<int32_t>(),
+                     c->m_feat.as<float>());
+  hipLaunchKernelGGL(k_mlp_forward, dim3(g), dim3(256), 0, c->st, c->m_feat.as<float>(), np, c->m_param.as<float>(),
+                     c->m_mu.as<float>(), c->m_sd.as<float>(), c->m_pred.as<float>());
+  HIPCHK(c, hipGetLastError());
+  std::vector<float> y(np * MLP_OUT);
+  HIPCHK(c, hipMemcpyAsync(y.data(), c->m_pred.p, np * MLP_OUT * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  pred.resize(np);
+  for (uint64_t p = 0; p < np; p++) pred[p] = y[p * MLP_OUT];
+  return OIC_OK;
+}
+
 template <int G, int CAP>
 static int run_fwd(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
   if (n == 0) return OIC_OK;
@@ -577,14 +600,14 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
   int rc = run_fwd<16, 256>(c, FA, np, &ms);
   if (rc) return rc;
   hipLaunchKernelGGL(k_spec_count, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
-                     c->b_cache.as<uint8_t>(), c->b_cnt.as<uint32_t>());
+                     c->b_cache.as<uint8_t>(), c->p_order.as<uint32_t>(), c->b_cnt.as<uint32_t>());
   uint64_t nt = 0;
   ALLOC(c, c->s_scan, scan_scratch_words(np) * 8);
   HIPCHK(c, scan_excl<uint32_t>(c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), np, c->s_scan.as<uint64_t>(), c->st, &nt));
   if (nt) {
     ALLOC(c, c->b_spec, nt * sizeof(FwdTask));
     hipLaunchKernelGGL(k_spec_write, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
-                       c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), c->b_spec.as<FwdTask>());
+                       c->p_order.as<uint32_t>(), c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), c->b_spec.as<FwdTask>());
     FA.tasks = c->b_spec.as<FwdTask>();
     rc = run_fwd<16, 256>(c, FA, nt, &ms);
     if (rc) return rc;
@@ -634,11 +657,33 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   const int sbits = std::max(1, ceil_log2(2ull * ns));
   if (GA.tbits + sbits > 64) return fail(c, OIC_EINVAL, "pair key needs %d bits", GA.tbits + sbits);
   if (c->nu == 0) return OIC_OK;
-  hipLaunchKernelGGL(k_hit_count, dim3(2 * ns), dim3(256), 0, c->st, GA, c->g_cnt.as<uint32_t>(), c->g_pbase.as<uint64_t>());
+  ALLOC(c, c->g_sdsum, 2 * (size_t)ns * 8);
+  hipLaunchKernelGGL(k_hit_count, dim3(2 * ns), dim3(256), 0, c->st, GA, c->g_cnt.as<uint32_t>(), c->g_pbase.as<uint64_t>(),
+                     c->g_sdsum.as<unsigned long long>());
   HIPCHK(c, hipGetLastError());
   ALLOC(c, c->s_scan, scan_scratch_words(P) * 8);
   uint64_t H = 0;
   HIPCHK(c, scan_excl<uint32_t>(c->g_cnt.as<uint32_t>(), c->g_off.as<uint64_t>(), P, c->s_scan.as<uint64_t>(), c->st, &H));
+  if (H > kHitCap && ns > 1) {
+    // too many seed hits for one pass (repeat reads): split at read boundaries
+    std::vector<uint64_t> sds(2 * (size_t)ns);
+    HIPCHK(c, hipMemcpy(sds.data(), c->g_sdsum.p, sds.size() * 8, hipMemcpyDeviceToHost));
+    uint32_t a = 0;
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < ns; i++) {
+      const uint64_t h = sds[2 * i] + sds[2 * i + 1];
+      if (i > a && acc + h > kHitCap) {
+        std::vector<uint8_t> sub(searched.begin() + a, searched.begin() + i);
+        int rc = search_batch(c, s0 + a, i - a, sub, sink);
+        if (rc) return rc;
+        a = i;
+        acc = 0;
+      }
+      acc += h;
+    }
+    std::vector<uint8_t> sub(searched.begin() + a, searched.end());
+    return search_batch(c, s0 + a, ns - a, sub, sink);
+  }
   c->ctr.kmer_probes += P;
   c->ctr.hits += H;
   if (H == 0) { c->ctr.ms_gather += now_ms() - t0; return OIC_OK; }
@@ -670,8 +715,16 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   const double t1 = now_ms();
   ALLOC(c, c->p_tmp, H * 16);
   ALLOC(c, c->p_nodes, H * 16);
-  hipLaunchKernelGGL(k_build_lists, dim3((unsigned)((np + 127) / 128)), dim3(128), 0, c->st, c->p_pairs.as<PairRec>(), np, hv,
-                     c->p_tmp.as<int4>(), c->p_nodes.as<int4>(), K);
+  ALLOC(c, c->l_scr, H * (8 * 4 + 1));
+  {
+    ListScratch LS;
+    int32_t *b = c->l_scr.as<int32_t>();
+    LS.sA = b; LS.sB = b + H; LS.oA = b + 2 * H; LS.oB = b + 3 * H; LS.fr = b + 4 * H; LS.dq = b + 5 * H; LS.db = b + 6 * H;
+    LS.pos = b + 7 * H;
+    LS.fl = (uint8_t *)(b + 8 * H);
+    hipLaunchKernelGGL(k_build_lists, dim3((unsigned)((np + 127) / 128)), dim3(128), 0, c->st, c->p_pairs.as<PairRec>(), np, hv,
+                       c->p_tmp.as<int4>(), c->p_nodes.as<int4>(), K, LS);
+  }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->st));
   c->ctr.ms_lists += now_ms() - t1;
@@ -687,7 +740,12 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   HIPCHK(c, hipMemsetAsync(c->p_pout.p, 0, np * sizeof(PairOut), c->st));
   std::vector<uint32_t> order;
   {
-    int rc = make_order(c->P.schedule, c->p_pairs.as<PairRec>(), np, c->h_len.data(), c->have_model ? &c->model : nullptr, c->st, order);
+    std::vector<float> pred;
+    if (c->P.schedule == OIC_SCHED_LEARNED && c->have_model) {
+      int rc = predict_pairs(c, np, pred);
+      if (rc) return rc;
+    }
+    int rc = make_order(c->P.schedule, c->p_pairs.as<PairRec>(), np, c->h_len.data(), pred.empty() ? nullptr : pred.data(), c->st, order);
     if (rc) return fail(c, OIC_EHIP, "scheduling failed");
   }
   HIPCHK(c, hipMemcpyAsync(c->p_order.p, order.data(), np * 4, hipMemcpyHostToDevice, c->st));
@@ -771,16 +829,16 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     HIPCHK(c, hipMemcpy(sink->records + sink->count, c->p_outrec.p, nrec * sizeof(OutRec), hipMemcpyDeviceToHost));
   sink->count += nrec;
   sink->records_needed = sink->count;
-  if (c->keep_trace) {
-    std::vector<PairRec> pr(np);
-    HIPCHK(c, hipMemcpy(pr.data(), c->p_pairs.p, np * sizeof(PairRec), hipMemcpyDeviceToHost));
+  if (c->tracing) {
+    ALLOC(c, c->m_feat, np * MLP_IN * 4);
+    hipLaunchKernelGGL(k_pair_features, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np,
+                       c->d_len.as<int32_t>(), c->m_feat.as<float>());
+    const size_t o = c->tr_feat.size();
+    c->tr_feat.resize(o + np * MLP_IN);
+    HIPCHK(c, hipMemcpy(c->tr_feat.data() + o, c->m_feat.p, np * MLP_IN * 4, hipMemcpyDeviceToHost));
     for (uint64_t p = 0; p < np; p++) {
-      PairFeat f;
-      pair_features(pr[p], c->h_len.data(), &f);
-      f.cells = (double)pout[p].cells;
-      f.exit_d = pout[p].exit_d;
-      f.n_ext = pout[p].n_ext;
-      c->last_feat.push_back(f);
+      c->tr_targ.push_back((float)std::log1p((double)pout[p].cells));
+      c->tr_targ.push_back((float)(pout[p].exit_d / 1000.0));
     }
   }
   return OIC_OK;
@@ -827,7 +885,6 @@ extern "C" int oic_run(oic_ctx *c, uint32_t bgn_hash, uint32_t end_hash, uint32_
   if (!c || !sink) return OIC_EINVAL;
   const double t0 = now_ms();
   c->ctr = oic_counters{};
-  c->last_feat.clear();
   sink->count = 0;
   sink->records_needed = 0;
   memset(sink->stats, 0, sizeof sink->stats);
@@ -1046,5 +1103,166 @@ extern "C" int oic_extend_batch(oic_ctx *c, const oic_ext_task *tasks, uint32_t 
   if (rc) return rc;
   HIPCHK(c, hipMemcpy(res, dr.p, (size_t)n * sizeof(oic_ext_result), hipMemcpyDeviceToHost));
   if (delta) HIPCHK(c, hipMemcpy(delta, dd.p, (size_t)n * stride * 4, hipMemcpyDeviceToHost));
+  return OIC_OK;
+}
+
+// ------------------------------------------------------------------------------------ cost model
+
+extern "C" int oic_set_tracing(oic_ctx *c, int on) {
+  if (!c) return OIC_EINVAL;
+  c->tracing = on != 0;
+  c->tr_feat.clear();
+  c->tr_targ.clear();
+  return OIC_OK;
+}
+
+extern "C" int oic_get_traces(oic_ctx *c, float *feat, float *targ, uint64_t cap, uint64_t *count) {
+  if (!c || !count) return OIC_EINVAL;
+  const uint64_t n = c->tr_targ.size() / MLP_OUT;
+  *count = n;
+  if (!feat && !targ) return OIC_OK;
+  if (cap < n) return fail(c, OIC_ECAPACITY, "%lu traces, capacity %lu", (unsigned long)n, (unsigned long)cap);
+  if (feat) memcpy(feat, c->tr_feat.data(), n * MLP_IN * 4);
+  if (targ) memcpy(targ, c->tr_targ.data(), n * MLP_OUT * 4);
+  return OIC_OK;
+}
+
+static int upload_model(oic_ctx *c) {
+  ALLOC(c, c->m_param, MLP_NPARAM * 4);
+  ALLOC(c, c->m_mu, MLP_IN * 4);
+  ALLOC(c, c->m_sd, MLP_IN * 4);
+  HIPCHK(c, hipMemcpy(c->m_param.p, c->mparam, MLP_NPARAM * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->m_mu.p, c->mmu, MLP_IN * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->m_sd.p, c->msd, MLP_IN * 4, hipMemcpyHostToDevice));
+  c->have_model = true;
+  return OIC_OK;
+}
+
+extern "C" int oic_model_get(const oic_ctx *c, float *w, float *mu, float *sd) {
+  if (!c || !c->have_model) return OIC_ESTATE;
+  if (w) memcpy(w, c->mparam, MLP_NPARAM * 4);
+  if (mu) memcpy(mu, c->mmu, MLP_IN * 4);
+  if (sd) memcpy(sd, c->msd, MLP_IN * 4);
+  return OIC_OK;
+}
+
+extern "C" int oic_model_set(oic_ctx *c, const float *w, const float *mu, const float *sd) {
+  if (!c || !w || !mu || !sd) return OIC_EINVAL;
+  memcpy(c->mparam, w, MLP_NPARAM * 4);
+  memcpy(c->mmu, mu, MLP_IN * 4);
+  memcpy(c->msd, sd, MLP_IN * 4);
+  return upload_model(c);
+}
+
+extern "C" int oic_model_predict(oic_ctx *c, const float *feat, uint64_t n, float *out) {
+  if (!c || (!feat && n) || (!out && n)) return OIC_EINVAL;
+  if (!c->have_model) return fail(c, OIC_ESTATE, "no model");
+  if (n == 0) return OIC_OK;
+  DevBuf x, y;
+  ALLOC(c, x, n * MLP_IN * 4);
+  ALLOC(c, y, n * MLP_OUT * 4);
+  HIPCHK(c, hipMemcpy(x.p, feat, n * MLP_IN * 4, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->st, x.as<float>(), n, c->m_param.as<float>(),
+                     c->m_mu.as<float>(), c->m_sd.as<float>(), y.as<float>());
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy(out, y.p, n * MLP_OUT * 4, hipMemcpyDeviceToHost));
+  return OIC_OK;
+}
+
+static uint32_t grad_waves(oic_ctx *c, uint64_t n) {
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 63) / 64, (uint64_t)c->ncu * 2));
+}
+
+extern "C" int oic_model_grad(oic_ctx *c, const float *w, const float *z, const float *t, uint64_t n, float w_exit, float *loss,
+                              float *grad) {
+  if (!c || !w || !z || !t || !n || !grad) return OIC_EINVAL;
+  DevBuf dp, dz, dt, dg, dl;
+  ALLOC(c, dp, MLP_NPARAM * 4);
+  ALLOC(c, dz, n * MLP_IN * 4);
+  ALLOC(c, dt, n * MLP_OUT * 4);
+  ALLOC(c, dg, MLP_NPARAM * 4);
+  ALLOC(c, dl, 4);
+  HIPCHK(c, hipMemcpy(dp.p, w, MLP_NPARAM * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(dz.p, z, n * MLP_IN * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(dt.p, t, n * MLP_OUT * 4, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_fill_f, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, c->st, dg.as<float>(), (uint64_t)MLP_NPARAM, 0.f);
+  hipLaunchKernelGGL(k_fill_f, dim3(1), dim3(64), 0, c->st, dl.as<float>(), (uint64_t)1, 0.f);
+  hipLaunchKernelGGL(k_mlp_grad, dim3(grad_waves(c, n)), dim3(64), 0, c->st, dz.as<float>(), dt.as<float>(), (const uint32_t *)nullptr, n,
+                     dp.as<float>(), w_exit, 1.0f / (float)n, dg.as<float>(), dl.as<float>());
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  HIPCHK(c, hipMemcpy(grad, dg.p, MLP_NPARAM * 4, hipMemcpyDeviceToHost));
+  float l = 0;
+  HIPCHK(c, hipMemcpy(&l, dl.p, 4, hipMemcpyDeviceToHost));
+  if (loss) *loss = l / (float)n;
+  return OIC_OK;
+}
+
+extern "C" int oic_model_train(oic_ctx *c, const float *feat, const float *targ, uint64_t n, const oic_train_params *tp, float *final_loss) {
+  if (!c || !feat || !targ || n < 2 || !tp || tp->batch == 0 || tp->epochs == 0) return OIC_EINVAL;
+  // standardisation
+  for (int i = 0; i < MLP_IN; i++) {
+    double s = 0, s2 = 0;
+    for (uint64_t r = 0; r < n; r++) { const double v = feat[r * MLP_IN + i]; s += v; s2 += v * v; }
+    const double m = s / n, var = s2 / n - m * m;
+    c->mmu[i] = (float)m;
+    c->msd[i] = (float)(var > 1e-12 ? std::sqrt(var) : 1.0);
+  }
+  std::vector<float> z(n * MLP_IN);
+  for (uint64_t r = 0; r < n; r++)
+    for (int i = 0; i < MLP_IN; i++) z[r * MLP_IN + i] = (feat[r * MLP_IN + i] - c->mmu[i]) / c->msd[i];
+  // Glorot-uniform initial weights from a fixed seed, zero biases
+  std::mt19937_64 rng(tp->seed);
+  auto init = [&](int off, int fan_out, int fan_in) {
+    const double a = std::sqrt(6.0 / (fan_in + fan_out));
+    std::uniform_real_distribution<double> u(-a, a);
+    for (int i = 0; i < fan_out * fan_in; i++) c->mparam[off + i] = (float)u(rng);
+  };
+  memset(c->mparam, 0, sizeof c->mparam);
+  init(MLP_OFF_W1, MLP_H, MLP_IN);
+  init(MLP_OFF_W2, MLP_H, MLP_H);
+  init(MLP_OFF_W3, MLP_OUT, MLP_H);
+  int rc = upload_model(c);
+  if (rc) return rc;
+  DevBuf dz, dt, dg, dm, dv, dl, didx;
+  ALLOC(c, dz, n * MLP_IN * 4);
+  ALLOC(c, dt, n * MLP_OUT * 4);
+  ALLOC(c, dg, MLP_NPARAM * 4);
+  ALLOC(c, dm, MLP_NPARAM * 4);
+  ALLOC(c, dv, MLP_NPARAM * 4);
+  ALLOC(c, dl, 4);
+  ALLOC(c, didx, n * 4);
+  HIPCHK(c, hipMemcpy(dz.p, z.data(), n * MLP_IN * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(dt.p, targ, n * MLP_OUT * 4, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_fill_f, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, c->st, dm.as<float>(), (uint64_t)MLP_NPARAM, 0.f);
+  hipLaunchKernelGGL(k_fill_f, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, c->st, dv.as<float>(), (uint64_t)MLP_NPARAM, 0.f);
+  std::vector<uint32_t> perm(n);
+  std::iota(perm.begin(), perm.end(), 0u);
+  int step = 0;
+  float last = 0;
+  for (uint32_t ep = 0; ep < tp->epochs; ep++) {
+    std::shuffle(perm.begin(), perm.end(), rng);
+    HIPCHK(c, hipMemcpyAsync(didx.p, perm.data(), n * 4, hipMemcpyHostToDevice, c->st));
+    hipLaunchKernelGGL(k_fill_f, dim3(1), dim3(64), 0, c->st, dl.as<float>(), (uint64_t)1, 0.f);
+    for (uint64_t b0 = 0; b0 < n; b0 += tp->batch) {
+      const uint64_t nb = std::min<uint64_t>(tp->batch, n - b0);
+      hipLaunchKernelGGL(k_fill_f, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, c->st, dg.as<float>(), (uint64_t)MLP_NPARAM, 0.f);
+      hipLaunchKernelGGL(k_mlp_grad, dim3(grad_waves(c, nb)), dim3(64), 0, c->st, dz.as<float>(), dt.as<float>(),
+                         didx.as<uint32_t>() + b0, nb, c->m_param.as<float>(), tp->w_exit, 1.0f / (float)nb, dg.as<float>(),
+                         dl.as<float>());
+      step++;
+      const float c1 = 1.f - std::pow(tp->beta1, (float)step), c2 = 1.f - std::pow(tp->beta2, (float)step);
+      hipLaunchKernelGGL(k_adam, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, c->st, c->m_param.as<float>(), dg.as<float>(),
+                         dm.as<float>(), dv.as<float>(), MLP_NPARAM, tp->lr, tp->beta1, tp->beta2, tp->eps, c1, c2);
+    }
+    HIPCHK(c, hipGetLastError());
+    float l = 0;
+    HIPCHK(c, hipMemcpyAsync(&l, dl.p, 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    last = l / (float)n;
+  }
+  HIPCHK(c, hipMemcpy(c->mparam, c->m_param.p, MLP_NPARAM * 4, hipMemcpyDeviceToHost));
+  if (final_loss) *final_loss = last;
   return OIC_OK;
 }

@@ -288,23 +288,26 @@ __global__ void k_probe_tasks(const PairRec *__restrict__ pairs, uint64_t np, co
   tasks[p].node = (uint32_t)best;
 }
 
-// pairs whose probe ended at a branch point: count their remaining nodes
+// pairs whose probe was not a dovetail: count their remaining nodes, in processing order
 __global__ void k_spec_count(const PairRec *__restrict__ pairs, uint64_t np, const FwdTask *__restrict__ probe,
-                             const uint8_t *__restrict__ cache, uint32_t *__restrict__ cnt) {
-  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
+                             const uint8_t *__restrict__ cache, const uint32_t *__restrict__ order, uint32_t *__restrict__ cnt) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const uint32_t p = order[i];
   const PairRec pr = pairs[p];
   const bool spec = !pr.consistent && pr.nnodes > 1 && cache[pr.hb + probe[p].node] == 1;
-  cnt[p] = spec ? (uint32_t)(pr.nnodes - 1) : 0u;
+  cnt[i] = spec ? (uint32_t)(pr.nnodes - 1) : 0u;
 }
 
 __global__ void k_spec_write(const PairRec *__restrict__ pairs, uint64_t np, const FwdTask *__restrict__ probe,
-                             const uint32_t *__restrict__ cnt, const uint64_t *__restrict__ off, FwdTask *__restrict__ tasks) {
-  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np || cnt[p] == 0) return;
+                             const uint32_t *__restrict__ order, const uint32_t *__restrict__ cnt, const uint64_t *__restrict__ off,
+                             FwdTask *__restrict__ tasks) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np || cnt[i] == 0) return;
+  const uint32_t p = order[i];
   const int32_t nn = pairs[p].nnodes;
   const uint32_t skip = probe[p].node;
-  uint64_t w = off[p];
+  uint64_t w = off[i];
   for (int32_t i = 0; i < nn; i++) {
     if ((uint32_t)i == skip) continue;
     tasks[w].pair = (uint32_t)p;

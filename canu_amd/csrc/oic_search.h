@@ -201,11 +201,19 @@ struct GatherArgs {
   int      tbits;           // bits for t_local in the pair key
 };
 
-// one block per (S, orientation): count hits with S_ID < T_ID for every window
-__global__ void __launch_bounds__(256) k_hit_count(GatherArgs A, uint32_t *__restrict__ cnt, const uint64_t *__restrict__ pbase) {
+// one block per (S, orientation): count hits with S_ID < T_ID for every window; sdsum gets the
+// block's total
+__global__ void __launch_bounds__(256) k_hit_count(GatherArgs A, uint32_t *__restrict__ cnt, const uint64_t *__restrict__ pbase,
+                                                   unsigned long long *__restrict__ sdsum) {
+  __shared__ unsigned long long bsum;
+  if (threadIdx.x == 0) bsum = 0;
+  __syncthreads();
   const uint32_t sd = blockIdx.x;
   const uint32_t si = sd >> 1, dir = sd & 1;
-  if (si >= A.ns || !A.searched[si]) return;
+  if (si >= A.ns || !A.searched[si]) {
+    if (threadIdx.x == 0) sdsum[sd] = 0;
+    return;
+  }
   const uint32_t S = A.s0 + si;
   const int32_t L = A.R.len[S];
   const int32_t npos = L - A.k + 1;
@@ -213,6 +221,7 @@ __global__ void __launch_bounds__(256) k_hit_count(GatherArgs A, uint32_t *__res
   const bool hn = A.R.hasN[S];
   const uint64_t pb = pbase[sd];
   const uint32_t tl_max = (S >= A.hbgn) ? (S - A.hbgn) : 0xffffffffu;   // T_local must exceed this
+  unsigned long long mine = 0;
   for (int32_t o = threadIdx.x; o < npos; o += blockDim.x) {
     uint32_t c = 0;
     if (!(hn && kmer_has_n(A.R.nmask, b0 + o, A.k))) {
@@ -224,7 +233,11 @@ __global__ void __launch_bounds__(256) k_hit_count(GatherArgs A, uint32_t *__res
       }
     }
     cnt[pb + o] = c;
+    mine += c;
   }
+  atomicAdd(&bsum, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) sdsum[sd] = bsum;
 }
 
 // same walk; write the hits in (o asc, T desc, T offset desc) order at the scanned offsets
@@ -300,65 +313,114 @@ __global__ void k_pair_ends(PairRec *__restrict__ pairs, uint64_t np, uint64_t n
   pairs[p].he = (p + 1 < np) ? pairs[p + 1].hb : nhits;
 }
 
-// Add_Ref + Add_Match state machine (overlapInCore-Find_Overlaps.C:24-96, :104-163), one thread
-// per candidate pair.  The linked list lives in tmp (Offset, Len, Start, Next; -1 = end); the
-// finished list is written head-to-tail into nodes.
+// Add_Ref + Add_Match (overlapInCore-Find_Overlaps.C:24-96, :104-163), one thread per candidate
+// pair, in O(1) per hit.
+//
+// Add_Match walks the match list from its head.  With hits in (S offset asc, T offset desc) order,
+// the list at S offset o is [nodes touched at o] ++ [nodes touched at o-1, in list order] ++ [dead
+// nodes]: a node touched at o-1 expects offset o, anything older ends the walk.  So a hit at o
+// extends the node of its diagonal among those touched at o-1 (at most one), which moves to the
+// front iff an untouched node of o-1 precedes it; otherwise a new node goes to the front, and
+// the pair stops being consistent if any live node was passed, or (none live) if the list's head
+// is more than 3 diagonals away or does not end near o.  After offset o the list is
+//   reverse(created/moved at o) ++ (o-1 nodes extended in place) ++ (o-1 nodes not hit) ++ older dead,
+// and the hits of o, taken in order, are also the new live nodes sorted by diagonal (descending),
+// which lets the next offset find diagonals by a merge instead of a walk.
+//
+// Scratch, all indexed from the pair's first hit: tmp (node records), and the int arrays
+// sA/sB (live nodes by diagonal), oA/oB (live nodes in list order), fr (created / moved at o),
+// dq (dead nodes, blocks in order of death), db (block starts), pos (list position of a live node),
+// fl (per live position: 1 touched, 2 moved).
+struct ListScratch {
+  int32_t *sA, *sB, *oA, *oB, *fr, *dq, *db, *pos;
+  uint8_t *fl;
+};
+
 __global__ void k_build_lists(PairRec *__restrict__ pairs, uint64_t np, const uint64_t *__restrict__ hval, int4 *__restrict__ tmp,
-                              int4 *__restrict__ nodes, int32_t K) {
+                              int4 *__restrict__ nodes, int32_t K, ListScratch X) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   PairRec pr = pairs[p];
   const uint64_t hb = pr.hb, he = pr.he;
-  int4 *T = tmp + hb;
-  int32_t head = -1, navail = 0;
+  int4 *N = tmp + hb;                     // node records: x Offset, y Len, z Start
+  int32_t *sCur = X.sA + hb, *sNew = X.sB + hb, *oCur = X.oA + hb, *oNew = X.oB + hb;
+  int32_t *fr = X.fr + hb, *dq = X.dq + hb, *db = X.db + hb, *pos = X.pos + hb;
+  uint8_t *fl = X.fl + hb;
+  int32_t nA = 0, nnode = 0, ndq = 0, ndb = 0;
   int32_t consistent = 1, maxm = 0;
-  for (uint64_t h = hb; h < he; h++) {
-    const uint64_t v = hval[h];
-    const int32_t offset = (int32_t)(v & 0x1fffff);
-    const int32_t toff = (int32_t)((v >> 21) & 0x1fffff);
-    const int32_t mult = (int32_t)(v >> 42);
-    if (mult > maxm) maxm = mult;
-    const int32_t new_diag = toff - offset;
-    int32_t diag = 0, expected_start = 0, num_checked = 0;
-    bool move_to_front = false, done = false;
-    int32_t prev = -1, q = head;
-    while (q != -1) {
-      const int4 nd = T[q];   // x Offset, y Len, z Start, w Next
-      expected_start = nd.z + nd.y - K + 1;
-      diag = nd.x - nd.z;
-      if (expected_start < offset) break;
-      if (expected_start == offset) {
-        if (new_diag == diag) {
-          T[q].y = nd.y + 1;
-          if (move_to_front) {
-            T[prev].w = nd.w;       // prev != -1 here: a node was passed before q
-            T[q].w = head;
-            head = q;
-          }
-          done = true;
-          break;
-        } else
-          move_to_front = true;
-      }
-      num_checked++;
-      prev = q;
-      q = nd.w;
+  int32_t prev_o = -2;
+  uint64_t h = hb;
+  while (h < he) {
+    const int32_t o = (int32_t)(hval[h] & 0x1fffff);
+    if (o != prev_o + 1 && nA > 0) {       // a gap: every live node is dead now
+      db[ndb++] = ndq;
+      for (int32_t i = 0; i < nA; i++) dq[ndq++] = oCur[i];
+      nA = 0;
     }
-    if (done) continue;
-    if (head != -1 && (num_checked > 0 || abs(diag - new_diag) > 3 || offset < expected_start + K - 2))
-      consistent = 0;
-    T[navail] = make_int4(toff, K, offset, head);
-    head = navail++;
+    for (int32_t i = 0; i < nA; i++) fl[i] = 0;
+    int32_t f = 0, nfr = 0, nmoved = 0, nnew = 0, q = 0;
+    for (; h < he && (int32_t)(hval[h] & 0x1fffff) == o; h++) {
+      const uint64_t v = hval[h];
+      const int32_t toff = (int32_t)((v >> 21) & 0x1fffff);
+      const int32_t mult = (int32_t)(v >> 42);
+      if (mult > maxm) maxm = mult;
+      const int32_t g = toff - o;
+      while (q < nA && (N[sCur[q]].x - N[sCur[q]].z) > g) q++;
+      if (q < nA && (N[sCur[q]].x - N[sCur[q]].z) == g) {
+        const int32_t x = sCur[q], i = pos[x];
+        N[x].y += 1;
+        fl[i] = 1;
+        if (f < i) { fl[i] = 2; fr[nfr++] = x; nmoved++; }
+        while (f < nA && fl[f]) f++;
+        sNew[nnew++] = x;
+        q++;
+      } else {
+        if (nnode > 0 && consistent) {
+          const int32_t live = nfr + nA - nmoved;
+          if (live > 0) consistent = 0;
+          else {
+            const int4 hd = N[dq[db[ndb - 1]]];          // head: first node of the latest dead block
+            const int32_t es = hd.z + hd.y - K + 1, dg = hd.x - hd.z;
+            if (abs(dg - g) > 3 || o < es + K - 2) consistent = 0;
+          }
+        }
+        N[nnode] = make_int4(toff, K, o, 0);
+        fr[nfr++] = nnode;
+        sNew[nnew++] = nnode;
+        nnode++;
+      }
+    }
+    // new list order of live nodes; the o-1 nodes not hit are dead (block prepended to the dead list)
+    int32_t n2 = 0;
+    for (int32_t i = nfr - 1; i >= 0; i--) { oNew[n2] = fr[i]; pos[fr[i]] = n2; n2++; }
+    bool dead_block = false;
+    for (int32_t i = 0; i < nA; i++) {
+      if (fl[i] == 1) { oNew[n2] = oCur[i]; pos[oCur[i]] = n2; n2++; }
+      else if (fl[i] == 0) {
+        if (!dead_block) { db[ndb++] = ndq; dead_block = true; }
+        dq[ndq++] = oCur[i];
+      }
+    }
+    nA = n2;
+    int32_t *t = sCur; sCur = sNew; sNew = t;
+    t = oCur; oCur = oNew; oNew = t;
+    prev_o = o;
   }
+  // list: live nodes, then dead blocks newest first
   int32_t n = 0, longest = 0, dmin = 0x7fffffff, dmax = -0x7fffffff;
-  for (int32_t q = head; q != -1; q = T[q].w) {
-    const int4 nd = T[q];
+  auto emit = [&](int32_t x) {
+    const int4 nd = N[x];
     nodes[hb + n] = make_int4(nd.x, nd.y, nd.z, 0);
     n++;
     if (nd.y > longest) longest = nd.y;
     const int32_t dg = nd.x - nd.z;
     if (dg < dmin) dmin = dg;
     if (dg > dmax) dmax = dg;
+  };
+  for (int32_t i = 0; i < nA; i++) emit(oCur[i]);
+  for (int32_t b = ndb - 1; b >= 0; b--) {
+    const int32_t e = (b + 1 < ndb) ? db[b + 1] : ndq;
+    for (int32_t i = db[b]; i < e; i++) emit(dq[i]);
   }
   pr.nnodes = n;
   pr.consistent = consistent;

@@ -26,8 +26,17 @@ SCHED_PLAIN, SCHED_BATCHED, SCHED_LEARNED = 0, 1, 2
 EXPORTS = [
     "oic_params_default", "oic_create", "oic_destroy", "oic_last_error", "oic_load_reads", "oic_index_build",
     "oic_search", "oic_run", "oic_get_counters", "oic_extend_batch", "oic_prefix_dp_batch", "oic_tables",
-    "oic_dp_tables",
+    "oic_dp_tables", "oic_set_tracing", "oic_get_traces", "oic_model_train", "oic_model_get", "oic_model_set",
+    "oic_model_predict", "oic_model_grad",
 ]
+
+NFEAT, NOUT = 12, 2
+NPARAM = 64 * NFEAT + 64 + 64 * 64 + 64 + NOUT * 64 + NOUT
+
+
+class TrainParams(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_uint32), ("batch", ctypes.c_uint32), ("lr", ctypes.c_float), ("beta1", ctypes.c_float),
+                ("beta2", ctypes.c_float), ("eps", ctypes.c_float), ("w_exit", ctypes.c_float), ("seed", ctypes.c_uint64)]
 
 RECORD_DTYPE = np.dtype([("a_iid", "<u4"), ("b_iid", "<u4"), ("w0", "<u8"), ("w1", "<u8")])
 
@@ -99,6 +108,13 @@ def lib():
         L.oic_tables.argtypes = [vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
                                  ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)), ctypes.POINTER(ctypes.c_uint32)]
         L.oic_dp_tables.argtypes = [ctypes.c_double, ctypes.c_double, vp, vp, ctypes.POINTER(ctypes.c_uint32)]
+        L.oic_set_tracing.argtypes = [vp, ctypes.c_int]
+        L.oic_get_traces.argtypes = [vp, vp, vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+        L.oic_model_train.argtypes = [vp, vp, vp, ctypes.c_uint64, ctypes.POINTER(TrainParams), ctypes.POINTER(ctypes.c_float)]
+        L.oic_model_get.argtypes = [vp, vp, vp, vp]
+        L.oic_model_set.argtypes = [vp, vp, vp, vp]
+        L.oic_model_predict.argtypes = [vp, vp, ctypes.c_uint64, vp]
+        L.oic_model_grad.argtypes = [vp, vp, vp, vp, ctypes.c_uint64, ctypes.c_float, ctypes.POINTER(ctypes.c_float), vp]
         _lib = L
     return _lib
 
@@ -210,6 +226,52 @@ class Engine:
         delta = np.zeros((len(tasks), delta_stride), np.int32)
         self._check(lib().oic_prefix_dp_batch(self._ctx, tasks.ctypes.data, len(tasks), res.ctypes.data, delta.ctypes.data, delta_stride))
         return res, delta
+
+    # ---- learned cost model
+    def set_tracing(self, on=True):
+        self._check(lib().oic_set_tracing(self._ctx, int(on)))
+
+    def traces(self):
+        """(features n x 12, targets n x 2) of the pairs searched since set_tracing(True)."""
+        n = ctypes.c_uint64()
+        self._check(lib().oic_get_traces(self._ctx, None, None, 0, ctypes.byref(n)))
+        f = np.zeros((n.value, NFEAT), np.float32)
+        t = np.zeros((n.value, NOUT), np.float32)
+        self._check(lib().oic_get_traces(self._ctx, f.ctypes.data, t.ctypes.data, n.value, ctypes.byref(n)))
+        return f, t
+
+    def train_model(self, feat, targ, epochs=12, batch=4096, lr=2e-3, w_exit=0.05, seed=1):
+        feat = np.ascontiguousarray(feat, np.float32)
+        targ = np.ascontiguousarray(targ, np.float32)
+        tp = TrainParams(epochs, batch, lr, 0.9, 0.999, 1e-8, w_exit, seed)
+        loss = ctypes.c_float()
+        self._check(lib().oic_model_train(self._ctx, feat.ctypes.data, targ.ctypes.data, len(feat), ctypes.byref(tp), ctypes.byref(loss)))
+        return loss.value
+
+    def model(self):
+        w = np.zeros(NPARAM, np.float32)
+        mu = np.zeros(NFEAT, np.float32)
+        sd = np.zeros(NFEAT, np.float32)
+        self._check(lib().oic_model_get(self._ctx, w.ctypes.data, mu.ctypes.data, sd.ctypes.data))
+        return w, mu, sd
+
+    def set_model(self, w, mu, sd):
+        w, mu, sd = (np.ascontiguousarray(a, np.float32) for a in (w, mu, sd))
+        self._check(lib().oic_model_set(self._ctx, w.ctypes.data, mu.ctypes.data, sd.ctypes.data))
+
+    def predict(self, feat):
+        feat = np.ascontiguousarray(feat, np.float32)
+        out = np.zeros((len(feat), NOUT), np.float32)
+        self._check(lib().oic_model_predict(self._ctx, feat.ctypes.data, len(feat), out.ctypes.data))
+        return out
+
+    def model_grad(self, w, z, t, w_exit=0.05):
+        w, z, t = (np.ascontiguousarray(a, np.float32) for a in (w, z, t))
+        g = np.zeros(NPARAM, np.float32)
+        loss = ctypes.c_float()
+        self._check(lib().oic_model_grad(self._ctx, w.ctypes.data, z.ctypes.data, t.ctypes.data, len(z), w_exit,
+                                         ctypes.byref(loss), g.ctypes.data))
+        return loss.value, g
 
     def tables(self):
         eb = ctypes.POINTER(ctypes.c_int32)()

@@ -189,6 +189,42 @@ int oic_prefix_dp_batch(oic_ctx *ctx, const oic_dp_task *tasks, uint32_t n, oic_
 int oic_tables(const oic_ctx *ctx, const int32_t **error_bound, const int32_t **match_limit,
                uint32_t *max_errors);
 
+/* ---- learned cost model (no counterpart in the reference: it schedules work, never changes a result) ----
+ *
+ * A 12 -> 64 -> 64 -> 2 ReLU MLP with FP32 weights predicts, from a candidate pair's seed
+ * features, log(1 + DP cells) and the exit diagonal of its first extension.  OIC_SCHED_LEARNED
+ * orders pairs and packed DP tasks by the prediction; OIC_SCHED_BATCHED uses a length-based
+ * estimate instead.  Training runs on the device with the library's own forward, backward and
+ * Adam kernels. */
+#define OIC_NFEAT   12
+#define OIC_NOUT    2
+#define OIC_NPARAM  (64 * OIC_NFEAT + 64 + 64 * 64 + 64 + OIC_NOUT * 64 + OIC_NOUT)
+
+/* keep per-pair traces (features, targets) of the following searches */
+int oic_set_tracing(oic_ctx *ctx, int on);
+/* features: count x 12, targets: count x 2 (log1p cells, exit diagonal / 1000) */
+int oic_get_traces(oic_ctx *ctx, float *features, float *targets, uint64_t capacity, uint64_t *count);
+
+typedef struct oic_train_params {
+  uint32_t epochs, batch;
+  float    lr, beta1, beta2, eps;
+  float    w_exit;              /* loss weight of the exit-diagonal output */
+  uint64_t seed;
+} oic_train_params;
+
+/* Train the context's model on (features, targets); returns the final mean loss. */
+int oic_model_train(oic_ctx *ctx, const float *features, const float *targets, uint64_t n,
+                    const oic_train_params *tp, float *final_loss);
+/* weights: OIC_NPARAM floats (layout W1[64][12] b1 W2[64][64] b2 W3[2][64] b3) plus the
+ * input standardisation mu[12], sd[12] */
+int oic_model_get(const oic_ctx *ctx, float *weights, float *mu, float *sd);
+int oic_model_set(oic_ctx *ctx, const float *weights, const float *mu, const float *sd);
+/* device inference: out = n x 2 */
+int oic_model_predict(oic_ctx *ctx, const float *features, uint64_t n, float *out);
+/* loss and gradient of the (standardised-input) MLP on one batch, for checking the kernels */
+int oic_model_grad(oic_ctx *ctx, const float *weights, const float *features, const float *targets, uint64_t n,
+                   float w_exit, float *loss, float *grad);
+
 /* The same tables without a context or device: error_bound needs 2^21 entries, match_limit
  * *max_errors entries (call once with match_limit = NULL to learn the size). */
 int oic_dp_tables(double max_erate, double align_noise, int32_t *error_bound, int32_t *match_limit,
