@@ -79,8 +79,6 @@ def run_steps(eng, n, batch, steps, warmup, first):
         t0 = time.perf_counter()
         res = eng.search(bgn, end)
         dt = time.perf_counter() - t0
-        if i == warmup - 1 or (warmup == 0 and i == 0 and not per):
-            pass
         if i >= warmup:
             per.append(dt)
             recs += len(res.records)
@@ -92,6 +90,16 @@ def run_steps(eng, n, batch, steps, warmup, first):
     return per, recs, d
 
 
+def timed_sample(eng, bgn, end):
+    """One search over a fixed ref range: (seconds, records, counter deltas)."""
+    c0 = eng.counters()
+    t0 = time.perf_counter()
+    res = eng.search(bgn, end)
+    dt = time.perf_counter() - t0
+    c1 = eng.counters()
+    return dt, res.records, {k: c1[k] - c0[k] for k in c1}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,14 +109,17 @@ def main():
     ap.add_argument("--batch", type=int, default=2500, help="ref reads per step")
     ap.add_argument("--schedule", default="learned", choices=["plain", "batched", "learned"])
     ap.add_argument("--no-plain", action="store_true", help="skip the plain one-pair-per-wavefront comparison")
+    ap.add_argument("--plain-refs", type=int, default=200, help="ref reads of the plain-vs-scheduled comparison sample")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tune-refs", type=int, default=200, help="ref reads the autotuner times each launch shape on")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-genome", type=int, default=500_000)
+    ap.add_argument("--cpu-genome", type=int, default=300_000)
     ap.add_argument("--train-genome", type=int, default=12_000_000, help="genome of the c3 instance the model learns from")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the overlap jobs shard by hash block x ref range with no exchange, see DESIGN.md)", file=sys.stderr)
+              f"(overlap jobs shard by hash block x ref range with no exchange between GPUs)", file=sys.stderr)
         sys.exit(2)
 
     from canu_amd import oic, readsim
@@ -136,41 +147,65 @@ def main():
         log(f"cost model: {info} ({time.perf_counter() - t0:.1f} s)")
 
     first = 1
+    timed_end = first + (args.warmup + args.steps) * args.batch - 1
+    tuning = eng.tuning()
+    if not args.no_autotune and args.schedule != "plain":
+        tb = min(R.n - 1, timed_end + 1)
+        t0 = time.perf_counter()
+        tuning, trials = eng.autotune(tb, min(R.n, tb + args.tune_refs - 1))
+        log(f"autotune over refs {tb}-{tb + args.tune_refs - 1}: {tuning} ({time.perf_counter() - t0:.1f} s)")
+        for t in trials:
+            log(f"    {t}")
+
     per, recs, d = run_steps(eng, R.n, args.batch, args.steps, args.warmup, first)
     tot = sum(per)
     cells_s = d["dp_cells"] / tot
     olap_s = recs / tot
-    kern_s = d["ms_dp_kernel"] / 1e3
-    ops = 4 * d["dp_cells"] + 2 * d["dp_bases"] / 16.0
-    achieved = ops / kern_s / 1e12 if kern_s > 0 else 0.0
-    launches = max(1, args.steps)
+    steps = args.steps
+    if args.schedule == "plain":
+        kname, kms, ops, kbases = "k_process_pairs<512>", d["ms_dp_kernel"], 4 * d["dp_cells"], d["dp_bases"]
+    else:
+        # the packed forward-DP kernel does almost all of the DP work of the scheduled paths
+        kname = f"k_fwd_tasks<{tuning['fwd_lanes']}, {tuning['fwd_tile']}>"
+        kms, ops, kbases = d["ms_fwd_kernel"], 4 * d["spec_cells"], d["spec_bases"]
+    ops += 2 * kbases / 16.0
+    achieved = ops / (kms / 1e3) / 1e12 if kms > 0 else 0.0
     line = {
         "metric": "overlaps/s + DP cells/s on 1 GPU, learned-binned vs plain, bit-exact",
-        "value": cells_s, "unit": "DP cells/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * tot / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": cells_s, "unit": "DP cells/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * tot / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int32", "data": f"synthetic ({W['desc']})",
         "config": {"workload": f"{args.workload}: {W['desc']}; k=22 --maxerate 0.045 --minlength 500",
                    "ref_reads_per_step": args.batch, "reads": R.n, "schedule": args.schedule},
         "overlaps_per_s": olap_s,
-        "dp_cells_per_step": d["dp_cells"] / args.steps,
+        "dp_cells_per_step": d["dp_cells"] / steps,
         "index_build_s": t_index,
-        "cost_model": info if args.schedule == "learned" else None,
+        "tuning": tuning,
+        "cost_model": info,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT_TOPS, "unit": "TOPS",
-                     "frac": achieved / PEAK_INT_TOPS, "traffic": traffic_from_profiles("k_process_pairs"),
-                     "kernel": "k_process_pairs", "kernel_ms_per_step": d["ms_dp_kernel"] / launches,
-                     "ops_per_step": ops / launches},
+                     "frac": achieved / PEAK_INT_TOPS, "traffic": traffic_from_profiles(kname),
+                     "kernel": kname, "kernel_ms_per_step": kms / steps, "int_ops_per_step": ops / steps},
+        "counters_per_step": {k: (v / steps) for k, v in d.items()},
     }
     if not args.no_plain and args.schedule != "plain":
+        # bounded comparison: the same ref reads through the scheduled engine and the plain one
+        pb, pe = first, min(R.n, first + args.plain_refs - 1)
+        ts, rs, ds = timed_sample(eng, pb, pe)
         eng.close()
         eng = oic.Engine(schedule=oic.SCHED_PLAIN, **kw)
         eng.load_reads(R)
         eng.build_hash_index(1, R.n)
-        log("plain one-pair-per-wavefront path:")
-        per_p, recs_p, dp = run_steps(eng, R.n, args.batch, args.steps, args.warmup, first)
-        tp = sum(per_p)
-        line["plain"] = {"dp_cells_per_s": dp["dp_cells"] / tp, "overlaps_per_s": recs_p / tp, "ms_per_step": 1e3 * tp / args.steps}
-        line["learned_vs_plain"] = cells_s / (dp["dp_cells"] / tp) if dp["dp_cells"] else None
-        assert recs_p == recs, "plain and scheduled paths disagree"
+        log(f"plain one-pair-per-wavefront path on refs {pb}-{pe}:")
+        tp, rp, dp = timed_sample(eng, pb, pe)
+        assert ds["dp_cells"] == dp["dp_cells"], "plain and scheduled paths disagree on DP cells"
+        a, b = oic.sort_records(rs), oic.sort_records(rp)
+        assert len(a) == len(b) and all((a[f] == b[f]).all() for f in ("a_iid", "b_iid", "w0", "w1")), \
+            "plain and scheduled paths disagree"
+        line["plain"] = {"sample": f"ref reads {pb}-{pe} against the whole index", "dp_cells": dp["dp_cells"],
+                         "plain_s": tp, "scheduled_s": ts, "plain_dp_cells_per_s": dp["dp_cells"] / tp,
+                         "scheduled_dp_cells_per_s": ds["dp_cells"] / ts}
+        line["learned_vs_plain" if args.schedule == "learned" else "batched_vs_plain"] = tp / ts
+        log(f"  plain {tp:.2f} s, scheduled {ts:.2f} s: {tp / ts:.2f}x")
     eng.close()
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))

@@ -88,10 +88,11 @@ struct oic_ctx {
   DevBuf s_k1, s_v1, s_k2, s_v2, s_hist, s_offs, s_scan, s_flag, s_idx, s_small;
   // search
   DevBuf g_cnt, g_off, g_pbase, g_searched, g_sdsum;
-  DevBuf l_scr;
+  DevBuf l_scr, l_done, l_big, l_cnt;
+  oic_tuning tune{};
   DevBuf p_pairs, p_tmp, p_nodes, p_recs, p_pout, p_order, p_queue, p_stats, p_nout, p_noff, p_outrec, p_retry;
   DevBuf sc_rowL, sc_rowR, sc_rowOff, sc_rowVal, sc_dbuf, sc_ld, sc_rd;
-  DevBuf b_cache, b_ncells, b_nexit, b_probe, b_spec, b_cnt, b_off, b_stats, b_order;
+  DevBuf b_cache, b_ncells, b_nexit, b_probe, b_spec, b_cnt, b_off, b_stats, b_order, b_ovf, b_novf;
   // learned cost model
   bool tracing = false;
   std::vector<float> tr_feat, tr_targ;
@@ -187,6 +188,9 @@ extern "C" int oic_create(const oic_params *p, oic_ctx **out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, p->device) != hipSuccess) { delete c; return OIC_EHIP; }
   c->ncu = prop.multiProcessorCount;
+  c->tune.list_wave_min_hits = 2048;
+  c->tune.fwd_lanes = 16;
+  c->tune.fwd_tile = 256;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; return OIC_EHIP; }
   hipEventCreate(&c->ev0);
   hipEventCreate(&c->ev1);
@@ -246,6 +250,22 @@ extern "C" int oic_dp_tables(double max_erate, double align_noise, int32_t *eb, 
   if (eb) memcpy(eb, T.error_bound.data(), T.error_bound.size() * 4);
   if (ml) memcpy(ml, T.match_limit.data(), (size_t)T.max_errors * 4);
   *max_errors = T.max_errors;
+  return OIC_OK;
+}
+
+extern "C" int oic_get_tuning(const oic_ctx *c, oic_tuning *out) {
+  if (!c || !out) return OIC_EINVAL;
+  *out = c->tune;
+  return OIC_OK;
+}
+
+extern "C" int oic_set_tuning(oic_ctx *c, const oic_tuning *t) {
+  if (!c || !t) return OIC_EINVAL;
+  const bool lanes_ok = t->fwd_lanes == 4 || t->fwd_lanes == 8 || t->fwd_lanes == 16 || t->fwd_lanes == 32;
+  const bool tile_ok = t->fwd_tile == 128 || t->fwd_tile == 256 || (t->fwd_tile == 512 && t->fwd_lanes != 4);
+  if (!lanes_ok || !tile_ok)
+    return fail(c, OIC_EINVAL, "fwd_lanes must be 4, 8, 16 or 32 and fwd_tile 128, 256 or 512 (not 512 with 4 lanes)");
+  c->tune = *t;
   return OIC_OK;
 }
 
@@ -571,6 +591,31 @@ static int run_fwd(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
   return OIC_OK;
 }
 
+// packed narrow-band pass, then the tasks whose band outgrew its tile, one per wavefront
+static int run_fwd_narrow(oic_ctx *c, const FwdArgs &FA, uint64_t n, double *ms) {
+  const uint32_t g = c->tune.fwd_lanes, t = c->tune.fwd_tile;
+#define FWD_CASE(G_, T_) if (g == G_ && t == T_) return run_fwd<G_, T_>(c, FA, n, ms);
+  FWD_CASE(4, 128) FWD_CASE(4, 256)
+  FWD_CASE(8, 128) FWD_CASE(8, 256) FWD_CASE(8, 512)
+  FWD_CASE(16, 128) FWD_CASE(16, 256) FWD_CASE(16, 512)
+  FWD_CASE(32, 128) FWD_CASE(32, 256) FWD_CASE(32, 512)
+#undef FWD_CASE
+  return fail(c, OIC_EINVAL, "no forward-DP kernel for %u lanes x %u-cell tile", g, t);
+}
+
+static int run_fwd_binned(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
+  HIPCHK(c, hipMemsetAsync(FA.novf, 0, 4, c->st));
+  int rc = run_fwd_narrow(c, FA, n, ms);
+  if (rc) return rc;
+  uint32_t nov = 0;
+  HIPCHK(c, hipMemcpy(&nov, FA.novf, 4, hipMemcpyDeviceToHost));
+  if (nov == 0) return OIC_OK;
+  c->ctr.overflow_retries += nov;
+  FA.tasks = FA.ovf;
+  FA.ovf = nullptr;
+  return run_fwd<64, 8192>(c, FA, nov, ms);
+}
+
 // probe pass over every pair's longest node, then every other node of the pairs whose probe
 // ended at a branch point; results land in the per-node cache
 static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
@@ -596,8 +641,12 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
   FA.nexit = c->b_nexit.as<int32_t>();
   FA.stats = c->b_stats.as<unsigned long long>();
   double ms = 0;
+  ALLOC(c, c->b_ovf, np * sizeof(FwdTask));
+  ALLOC(c, c->b_novf, 16);
+  FA.ovf = c->b_ovf.as<FwdTask>();
+  FA.novf = c->b_novf.as<uint32_t>();
   FA.tasks = c->b_probe.as<FwdTask>();
-  int rc = run_fwd<16, 256>(c, FA, np, &ms);
+  int rc = run_fwd_binned(c, FA, np, &ms);
   if (rc) return rc;
   hipLaunchKernelGGL(k_spec_count, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
                      c->b_cache.as<uint8_t>(), c->p_order.as<uint32_t>(), c->b_cnt.as<uint32_t>());
@@ -609,12 +658,18 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
     hipLaunchKernelGGL(k_spec_write, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
                        c->p_order.as<uint32_t>(), c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), c->b_spec.as<FwdTask>());
     FA.tasks = c->b_spec.as<FwdTask>();
-    rc = run_fwd<16, 256>(c, FA, nt, &ms);
+    ALLOC(c, c->b_ovf, nt * sizeof(FwdTask));
+    FA.ovf = c->b_ovf.as<FwdTask>();
+    rc = run_fwd_binned(c, FA, nt, &ms);
     if (rc) return rc;
   }
   unsigned long long st[8];
   HIPCHK(c, hipMemcpy(st, c->b_stats.p, sizeof st, hipMemcpyDeviceToHost));
   c->ctr.spec_cells += st[0];
+  c->ctr.spec_rows += st[1];
+  c->ctr.spec_lane_slots += st[2];
+  c->ctr.spec_wave_rows += st[3];
+  c->ctr.spec_bases += st[4];
   c->ctr.spec_tasks += np + nt;
   c->ctr.ms_fwd_kernel += ms;
   c->ctr.ms_dp_kernel += ms;
@@ -722,11 +777,42 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     LS.sA = b; LS.sB = b + H; LS.oA = b + 2 * H; LS.oB = b + 3 * H; LS.fr = b + 4 * H; LS.dq = b + 5 * H; LS.db = b + 6 * H;
     LS.pos = b + 7 * H;
     LS.fl = (uint8_t *)(b + 8 * H);
+    // pairs with many hits: one wavefront each; the rest (and any handed back): one thread each
+    ALLOC(c, c->l_done, np);
+    ALLOC(c, c->l_big, np * 4);
+    ALLOC(c, c->l_cnt, 32);
+    HIPCHK(c, hipMemsetAsync(c->l_cnt.p, 0, 32, c->st));
+    uint32_t *cnt = c->l_cnt.as<uint32_t>();
+    hipLaunchKernelGGL(k_big_select, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np,
+                       (uint64_t)c->tune.list_wave_min_hits, c->l_done.as<uint8_t>(), c->l_big.as<uint32_t>(), cnt);
+    BigListArgs BA;
+    BA.pairs = c->p_pairs.as<PairRec>();
+    BA.big = c->l_big.as<uint32_t>();
+    BA.nbig = cnt;
+    BA.next = cnt + 1;
+    BA.handed_back = cnt + 2;
+    BA.done = c->l_done.as<uint8_t>();
+    BA.nnodes_total = (unsigned long long *)(cnt + 4);
+    BA.hval = hv;
+    BA.tmp = c->p_tmp.as<int4>();
+    BA.nodes = c->p_nodes.as<int4>();
+    BA.K = K;
+    const unsigned gw = (unsigned)std::min<uint64_t>(np, 8ull * (uint64_t)std::max(c->ncu, 1));
+    hipLaunchKernelGGL(k_build_lists_wave, dim3(gw), dim3(64), 0, c->st, BA);
     hipLaunchKernelGGL(k_build_lists, dim3((unsigned)((np + 127) / 128)), dim3(128), 0, c->st, c->p_pairs.as<PairRec>(), np, hv,
-                       c->p_tmp.as<int4>(), c->p_nodes.as<int4>(), K, LS);
+                       c->p_tmp.as<int4>(), c->p_nodes.as<int4>(), K, LS, c->l_done.as<uint8_t>(), BA.nnodes_total);
   }
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipStreamSynchronize(c->st));
+  {
+    uint32_t lc[8];
+    HIPCHK(c, hipMemcpyAsync(lc, c->l_cnt.p, 32, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->ctr.list_wave_pairs += lc[0];
+    c->ctr.list_handed_back += lc[2];
+    uint64_t nn;
+    memcpy(&nn, lc + 4, 8);
+    c->ctr.nodes += nn;
+  }
   c->ctr.ms_lists += now_ms() - t1;
 
   // processing order
@@ -910,6 +996,74 @@ extern "C" int oic_run(oic_ctx *c, uint32_t bgn_hash, uint32_t end_hash, uint32_
   c->ctr.ms_total = now_ms() - t0;
   if (sink->count > sink->capacity) return fail(c, OIC_ECAPACITY, "sink holds %lu records, %lu needed", (unsigned long)sink->capacity,
                                                  (unsigned long)sink->count);
+  return OIC_OK;
+}
+
+// ------------------------------------------------------------------------------------ autotuner
+
+// Times the search of a sample of ref reads under each launch shape and keeps the fastest: the
+// packed forward-DP kernel over lanes per DP x band tile (device time of that kernel), then the
+// list-builder threshold (device + host time of the list stage).  Results never depend on the
+// shape, so the sample's records are discarded.
+extern "C" int oic_autotune(oic_ctx *c, uint32_t bgn_ref, uint32_t end_ref, oic_tuning *best, oic_tune_trial *trials,
+                            uint32_t cap, uint32_t *ntrials) {
+  if (!c) return OIC_EINVAL;
+  if (!c->have_index) return fail(c, OIC_ESTATE, "no index built");
+  if (bgn_ref < 1) bgn_ref = 1;
+  if (end_ref > c->nreads) end_ref = c->nreads;
+  if (end_ref < bgn_ref) return fail(c, OIC_EINVAL, "empty sample range");
+  const oic_counters saved = c->ctr;
+  oic_tuning cur = c->tune;
+  uint32_t nt = 0;
+  auto trial = [&](const oic_tuning &t, int what, double *ms) -> int {
+    c->tune = t;
+    c->ctr = oic_counters{};
+    oic_sink sk{};
+    const int rc = search_range(c, bgn_ref, end_ref, &sk);
+    if (rc) return rc;
+    *ms = (what == 0) ? c->ctr.ms_fwd_kernel : c->ctr.ms_lists;
+    if (trials && nt < cap) {
+      trials[nt].tuning = t;
+      trials[nt].stage = what;
+      trials[nt].ms = *ms;
+    }
+    nt++;
+    return OIC_OK;
+  };
+  double ms = 0;
+  int rc = trial(cur, 1, &ms);   // first run pays for buffer growth; not compared
+  if (rc) { c->ctr = saved; return rc; }
+  if (c->P.schedule != OIC_SCHED_PLAIN && !c->P.partial) {
+    double bms = 1e300;
+    oic_tuning b = cur;
+    for (uint32_t g : {4u, 8u, 16u, 32u})
+      for (uint32_t t : {128u, 256u, 512u}) {
+        if (g == 4 && t == 512) continue;
+        oic_tuning x = cur;
+        x.fwd_lanes = g;
+        x.fwd_tile = t;
+        rc = trial(x, 0, &ms);
+        if (rc) { c->ctr = saved; return rc; }
+        if (ms < bms) { bms = ms; b = x; }
+      }
+    cur = b;
+  }
+  {
+    double bms = 1e300;
+    oic_tuning b = cur;
+    for (uint32_t h : {256u, 1024u, 4096u, 16384u}) {
+      oic_tuning x = cur;
+      x.list_wave_min_hits = h;
+      rc = trial(x, 1, &ms);
+      if (rc) { c->ctr = saved; return rc; }
+      if (ms < bms) { bms = ms; b = x; }
+    }
+    cur = b;
+  }
+  c->tune = cur;
+  c->ctr = saved;
+  if (best) *best = cur;
+  if (ntrials) *ntrials = nt;
   return OIC_OK;
 }
 

@@ -9,6 +9,9 @@
 //   cache[node] = 1  not a dovetail extension (Process_Matches skips the DP, removes the node)
 //                 2  dovetail (Process_Matches runs the whole extension for the deltas)
 //                 0  not computed (band outgrew the tile)
+// Tasks run 4 to a wavefront (16 lanes, 256-cell band tile); those whose band outgrows the tile are
+// queued and re-run one to a wavefront with an 8192-cell tile, which holds any band the error
+// limit allows (2 x 2700 + 3 cells at 60 kbp and 4.5 %).
 // The DP below is wave_prefix_dp's row loop without row storage or traceback, on a G-lane group,
 // with the first 1024 bases of both tails staged in LDS.  When the forward DP reaches the end, the
 // reverse DP on the left tails runs too, so the cached outcome is the extension's kind.
@@ -17,6 +20,7 @@
 #include "oic_search.h"
 
 #define OIC_SEQ_WIN 64      // words (16 bases each) of each tail staged in LDS
+#define FWD_CHUNK 8         // consecutive tasks a group takes from the queue at once
 
 struct FwdTask {
   uint32_t pair;
@@ -35,6 +39,8 @@ struct FwdArgs {
   uint32_t *ncells;         // per node: DP cells of the cached forward run
   int32_t *nexit;           // per node: right-end diagonal (T_Hi - S_Hi) of the cached run
   unsigned long long *stats;
+  FwdTask *ovf;             // tasks whose band outgrew the tile are appended here (if not null)
+  uint32_t *novf;
 };
 
 __device__ __forceinline__ void wave_fence() {
@@ -63,6 +69,10 @@ struct GroupDP {
   int32_t e;          // errors at the end (mte) or of the best branch point
   int32_t exit_d;     // diagonal of the end cell / best branch point
   uint64_t cells;
+  uint32_t rows;      // rows e >= 1 entered
+  uint32_t slots;     // lane-cells issued (G per band chunk)
+  uint32_t wrows;     // row iterations the whole wavefront executed (counted on its first active lane)
+  uint64_t bases;     // bases compared: slide lengths plus one terminating compare per cell
 };
 
 // One prefix DP (forward() row loop, no row storage) on a G-lane group; sa/st are the group's LDS
@@ -72,7 +82,7 @@ __device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, 
                                    const DPTables &tb, int32_t *pbuf, int32_t *cbuf, uint32_t *sa, uint32_t *sb, int g, int gl) {
   constexpr int32_t MASK = CAP - 1;
   GroupDP r;
-  r.mte = false; r.overflow = false; r.e = 0; r.exit_d = 0; r.cells = 1;
+  r.mte = false; r.overflow = false; r.e = 0; r.exit_d = 0; r.cells = 1; r.rows = 0; r.slots = 0; r.wrows = 0; r.bases = 0;
   const bool nA = a.nm != nullptr, nB = b.nm != nullptr;
   const int64_t wa0 = a.b0 >> 4, wb0 = b.b0 >> 4;
   for (int i = gl; i < OIC_SEQ_WIN; i += G) {
@@ -106,8 +116,11 @@ __device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, 
   int32_t Left = 0, Right = 0, Longest = 0, Best_d = 0, Best_e = 0, MSL = 0, MSd = 0, MSe = 0;
   double MaxScore = 0.0;
   for (int32_t e = 1; e <= limit; e++) {
+    const int32_t eml = tb.match_limit[e];
     Left = (Left - 1 > -e) ? Left - 1 : -e;
     Right = (Right + 1 < e) ? Right + 1 : e;
+    r.rows++;
+    if (lane_id() == __builtin_ctzll(__ballot(1))) r.wrows++;
     if (Right - Left + 3 > CAP) { r.overflow = true; return r; }
     wave_fence();
     if (gl < 4) {
@@ -118,6 +131,7 @@ __device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, 
     for (int32_t c0 = Left; c0 <= Right; c0 += G) {
       const int32_t d = c0 + gl;
       const bool act = d <= Right;
+      r.slots += G;
       int32_t Row = 0;
       if (act) {
         const int32_t x1 = 1 + pbuf[d & MASK], x2 = pbuf[(d - 1) & MASK], x3 = 1 + pbuf[(d + 1) & MASK];
@@ -138,6 +152,7 @@ __device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, 
           rem -= 16;
         }
         Row += adv;
+        r.bases += (uint64_t)adv + 1;
         cbuf[d & MASK] = Row;
       }
       const bool reach = act && (Row == m || Row + d == n);
@@ -159,7 +174,6 @@ __device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, 
       r.cells += (uint64_t)((Right - c0 + 1) < G ? (Right - c0 + 1) : G);
     }
     wave_fence();
-    const int32_t eml = tb.match_limit[e];
     int32_t nL = Right + 1;
     for (int32_t c0 = Left; c0 <= Right; c0 += G) {
       const int32_t d = c0 + gl;
@@ -209,14 +223,18 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
   __shared__ uint32_t fetched[NG];
   const int lane = lane_id();
   const int g = lane / G, gl = lane % G;
-  uint64_t cells_acc = 0;
+  uint64_t cells_acc = 0, rows_acc = 0, slots_acc = 0, wrows_acc = 0, bases_acc = 0;
+  uint64_t ti = 0, tend = 0;
 
   while (true) {
-    if (gl == 0) fetched[g] = atomicAdd(A.queue, 1u);
-    wave_fence();
-    const uint32_t ti = fetched[g];
-    wave_fence();
-    if ((uint64_t)ti >= A.ntasks) break;   // group-uniform
+    if (ti == tend) {                        // group-uniform: take the next FWD_CHUNK tasks
+      if (gl == 0) fetched[g] = atomicAdd(A.queue, (uint32_t)FWD_CHUNK);
+      wave_fence();
+      ti = fetched[g];
+      wave_fence();
+      tend = ti + FWD_CHUNK < A.ntasks ? ti + FWD_CHUNK : A.ntasks;
+    }
+    if (ti >= A.ntasks) break;               // group-uniform
     const FwdTask tk = A.tasks[ti];
     const PairRec pr = A.pairs[tk.pair];
     const int4 nd = A.nodes[pr.hb + tk.node];
@@ -229,7 +247,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
     const int32_t Total_Olap = (Start < Offset ? Start : Offset) + Len + (S_Right_Len < T_Right_Len ? S_Right_Len : T_Right_Len);
     const int32_t limit = A.tb.error_bound[Total_Olap];
     const SeqV S = read_view(A.R, pr.s_id, pr.dir), T = read_view(A.R, pr.t_id, 0);
-    uint64_t cells = 0;
+    uint64_t cells = 0, rows = 0, slots = 0, wrows = 0, bases = 0;
     bool rmte = true, overflow = false;
     int32_t right_e = 0, exit_d = T_Right_Begin - S_Right_Begin;
     if (!(S_Right_Len == 0 || T_Right_Len == 0)) {
@@ -240,6 +258,10 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
       else         { a = T; a.b0 += T_Right_Begin; m = T_Right_Len; b = S; b.b0 += S_Right_Begin; n = S_Right_Len; }
       const GroupDP f = group_prefix_dp<G, CAP>(a, m, b, n, limit, A.tb, tile[g][0], tile[g][1], seqA[g], seqT[g], g, gl);
       cells += f.cells;
+      rows += f.rows;
+      slots += f.slots;
+      wrows += f.wrows;
+      bases += f.bases;
       overflow = f.overflow;
       rmte = f.mte;
       right_e = f.e;
@@ -259,19 +281,41 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
       }
       const GroupDP l = group_prefix_dp<G, CAP>(a, m, b, n, limit - right_e, A.tb, tile[g][0], tile[g][1], seqA[g], seqT[g], g, gl);
       cells += l.cells;
+      rows += l.rows;
+      slots += l.slots;
+      wrows += l.wrows;
+      bases += l.bases;
       overflow = l.overflow;
       lmte = l.mte;
     }
     if (gl == 0) {
       // 1: not a dovetail (Process_Matches only removes the node), 2: dovetail, 0: not computed
       A.cache[gnode] = overflow ? 0 : ((rmte && lmte) ? 2 : 1);
+      if (overflow && A.ovf) A.ovf[atomicAdd(A.novf, 1u)] = tk;
       A.ncells[gnode] = (uint32_t)(cells > 0xffffffffull ? 0xffffffffull : cells);
       A.nexit[gnode] = exit_d;
       if (!overflow) cells_acc += cells;
+      rows_acc += rows;
+      slots_acc += slots;
     }
+    wrows_acc += wrows;
+    bases_acc += bases;
+    ti++;
   }
-  for (int o = 32; o > 0; o >>= 1) cells_acc += __shfl_xor(cells_acc, o);
-  if (lane == 0 && A.stats) atomicAdd(&A.stats[0], (unsigned long long)cells_acc);
+  for (int o = 32; o > 0; o >>= 1) {
+    cells_acc += __shfl_xor(cells_acc, o);
+    rows_acc += __shfl_xor(rows_acc, o);
+    slots_acc += __shfl_xor(slots_acc, o);
+    wrows_acc += __shfl_xor(wrows_acc, o);
+    bases_acc += __shfl_xor(bases_acc, o);
+  }
+  if (lane == 0 && A.stats) {
+    atomicAdd(&A.stats[3], (unsigned long long)wrows_acc);
+    atomicAdd(&A.stats[4], (unsigned long long)bases_acc);
+    atomicAdd(&A.stats[0], (unsigned long long)cells_acc);
+    atomicAdd(&A.stats[1], (unsigned long long)rows_acc);
+    atomicAdd(&A.stats[2], (unsigned long long)slots_acc);
+  }
 }
 
 // probe tasks: each pair's first-longest node (Process_String_Overlaps.C:424-431)
@@ -288,14 +332,20 @@ __global__ void k_probe_tasks(const PairRec *__restrict__ pairs, uint64_t np, co
   tasks[p].node = (uint32_t)best;
 }
 
-// pairs whose probe was not a dovetail: count their remaining nodes, in processing order
+// Pairs whose other nodes are all worth a forward DP ahead of Process_Matches: those whose probe
+// was not a dovetail (each later node is reached unless it is itself the next one removed), and
+// those with a dovetail probe but at least SPEC_DOVE_MIN nodes (repeat copies on shifted
+// diagonals survive the alignment's removal; a few nodes near it are computed in vain).  Counts
+// are written in processing order.
+#define SPEC_DOVE_MIN 16
 __global__ void k_spec_count(const PairRec *__restrict__ pairs, uint64_t np, const FwdTask *__restrict__ probe,
                              const uint8_t *__restrict__ cache, const uint32_t *__restrict__ order, uint32_t *__restrict__ cnt) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const uint32_t p = order[i];
   const PairRec pr = pairs[p];
-  const bool spec = !pr.consistent && pr.nnodes > 1 && cache[pr.hb + probe[p].node] == 1;
+  const uint8_t pc = cache[pr.hb + probe[p].node];
+  const bool spec = !pr.consistent && pr.nnodes > 1 && (pc == 1 || (pc == 2 && pr.nnodes >= SPEC_DOVE_MIN));
   cnt[i] = spec ? (uint32_t)(pr.nnodes - 1) : 0u;
 }
 

@@ -240,12 +240,20 @@ __global__ void __launch_bounds__(256) k_hit_count(GatherArgs A, uint32_t *__res
   if (threadIdx.x == 0) sdsum[sd] = bsum;
 }
 
-// same walk; write the hits in (o asc, T desc, T offset desc) order at the scanned offsets
+// same walk; write the hits in (o asc, T desc, T offset desc) order at the scanned offsets.
+// The hits of 64 consecutive offsets are one contiguous output range, written by the whole
+// wavefront: output slot s comes from the lane whose range holds s (binary search over the 64
+// range starts in LDS), so stores are coalesced and an offset with 10^4 occurrences (satellite
+// k-mers) is spread over all lanes instead of serialised on one.
 __global__ void __launch_bounds__(256) k_hit_emit(GatherArgs A, const uint64_t *__restrict__ off, const uint64_t *__restrict__ pbase,
                                                   uint64_t *__restrict__ hkey, uint64_t *__restrict__ hval) {
+  __shared__ uint32_t s_start[4][64];
+  __shared__ uint64_t s_hi[4][64];
+  __shared__ uint64_t s_val[4][64];     // o | mult << 42
   const uint32_t sd = blockIdx.x;
   const uint32_t si = sd >> 1, dir = sd & 1;
   if (si >= A.ns || !A.searched[si]) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint32_t S = A.s0 + si;
   const int32_t L = A.R.len[S];
   const int32_t npos = L - A.k + 1;
@@ -254,22 +262,41 @@ __global__ void __launch_bounds__(256) k_hit_emit(GatherArgs A, const uint64_t *
   const uint64_t pb = pbase[sd];
   const uint32_t tl_max = (S >= A.hbgn) ? (S - A.hbgn) : 0xffffffffu;
   const uint64_t pref = (uint64_t)sd << A.tbits;
-  for (int32_t o = threadIdx.x; o < npos; o += blockDim.x) {
-    if (hn && kmer_has_n(A.R.nmask, b0 + o, A.k)) continue;
-    const uint64_t key = kmer_at(A.R.words, b0 + o, A.k);
-    uint64_t lo, hi;
-    if (!table_find(A.X, key, &lo, &hi)) continue;
-    const uint64_t f = (tl_max == 0xffffffffu) ? lo : occ_upper(A.X.occ, lo, hi, tl_max);
-    const uint64_t mult = hi - lo;
-    const uint64_t mcap = mult > 0x3fffffull ? 0x3fffffull : mult;
-    uint64_t w = off[pb + o];
-    for (uint64_t q = hi; q > f; q--) {
-      const uint64_t oc = A.X.occ[q - 1];
-      const uint32_t tl = (uint32_t)(oc >> 32), toff = (uint32_t)oc;
-      hkey[w] = pref | tl;
-      hval[w] = (uint64_t)o | ((uint64_t)toff << 21) | (mcap << 42);
-      w++;
+  for (int32_t o0 = wv * 64; o0 < npos; o0 += 256) {
+    const int32_t o = o0 + lane;
+    uint64_t hi = 0, cnt = 0, mcap = 0;
+    if (o < npos && !(hn && kmer_has_n(A.R.nmask, b0 + o, A.k))) {
+      const uint64_t key = kmer_at(A.R.words, b0 + o, A.k);
+      uint64_t lo;
+      if (table_find(A.X, key, &lo, &hi)) {
+        const uint64_t f = (tl_max == 0xffffffffu) ? lo : occ_upper(A.X.occ, lo, hi, tl_max);
+        cnt = hi - f;
+        const uint64_t mult = hi - lo;
+        mcap = mult > 0x3fffffull ? 0x3fffffull : mult;
+      }
     }
+    const uint64_t w0 = off[pb + o0];               // first output slot of these 64 offsets
+    const uint64_t mine = (o < npos) ? off[pb + o] : 0;
+    const uint64_t last = __shfl(mine + cnt, 63 < (npos - 1 - o0) ? 63 : (npos - 1 - o0));
+    const uint32_t total = (uint32_t)(last - w0);
+    s_start[wv][lane] = (o < npos) ? (uint32_t)(mine - w0) : total;
+    s_hi[wv][lane] = hi;
+    s_val[wv][lane] = (uint64_t)(o < npos ? o : 0) | (mcap << 42);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t s = lane; s < total; s += 64) {
+      int32_t l = 0;
+#pragma unroll
+      for (int32_t step = 32; step > 0; step >>= 1)
+        if (s_start[wv][l + step] <= s) l += step;
+      const uint64_t q = s_hi[wv][l] - 1 - (s - s_start[wv][l]);
+      const uint64_t oc = A.X.occ[q];
+      const uint32_t tl = (uint32_t)(oc >> 32), toff = (uint32_t)oc;
+      hkey[w0 + s] = pref | tl;
+      hval[w0 + s] = s_val[wv][l] | ((uint64_t)toff << 21);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -337,9 +364,10 @@ struct ListScratch {
 };
 
 __global__ void k_build_lists(PairRec *__restrict__ pairs, uint64_t np, const uint64_t *__restrict__ hval, int4 *__restrict__ tmp,
-                              int4 *__restrict__ nodes, int32_t K, ListScratch X) {
+                              int4 *__restrict__ nodes, int32_t K, ListScratch X, const uint8_t *__restrict__ done,
+                              unsigned long long *__restrict__ nnodes_total) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
+  if (p >= np || done[p]) return;
   PairRec pr = pairs[p];
   const uint64_t hb = pr.hb, he = pr.he;
   int4 *N = tmp + hb;                     // node records: x Offset, y Len, z Start
@@ -423,6 +451,7 @@ __global__ void k_build_lists(PairRec *__restrict__ pairs, uint64_t np, const ui
     for (int32_t i = db[b]; i < e; i++) emit(dq[i]);
   }
   pr.nnodes = n;
+  atomicAdd(nnodes_total, (unsigned long long)n);
   pr.consistent = consistent;
   pr.diag_bgn = (int32_t)(hval[hb] & 0x1fffff);
   pr.diag_end = (int32_t)(hval[he - 1] & 0x1fffff);
@@ -431,6 +460,276 @@ __global__ void k_build_lists(PairRec *__restrict__ pairs, uint64_t np, const ui
   pr.dmin = dmin;
   pr.dmax = dmax;
   pairs[p] = pr;
+}
+
+// The same list, built by one wavefront per pair for pairs with many hits (satellite and repeat
+// pairs reach millions).  Offsets are still taken in order, but the hits of one offset are handled
+// by all lanes at once, with the live nodes in LDS:
+//   - a hit's node is found by binary search over the live diagonals (descending);
+//   - with t(k) = index of the hit touching the node at list position k (INF if none), the node
+//     of hit j moves to the front iff max(t(0..k-1)) > j, i.e. some node before it is not yet
+//     touched when hit j is taken;
+//   - the new list is reverse(created/moved hits) ++ (nodes extended in place) ++ (untouched
+//     nodes, now dead), positions found by ballot ranks;
+//   - a created node makes the pair inconsistent unless it is the offset's first hit with no
+//     live node, in which case the head-of-list test of Add_Match applies to the first node of the
+//     latest dead block.
+// Dead nodes go to tmp in blocks as they die, each record tagged so that its final position
+// (live nodes, then blocks newest first, each in list order) is nlive + ndead - tag.  A pair whose
+// offset has more than LW_CAP hits is handed back (done[p] = 0) to the one-thread builder.
+#define LW_CAP 512
+
+__device__ __forceinline__ int32_t wave_incl_max_i32(int32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t t = __shfl_up(v, o);
+    if (lane >= o) v = max(v, t);
+  }
+  return v;
+}
+
+__device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+__device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+
+struct BigListArgs {
+  PairRec *pairs;
+  const uint32_t *big;      // indices of pairs with many hits
+  const uint32_t *nbig;
+  uint32_t *next;           // work counter, starts at 0
+  uint32_t *handed_back;    // pairs left to the one-thread builder
+  uint8_t *done;            // per pair: 1 = list built by k_build_lists_wave
+  unsigned long long *nnodes_total;
+  const uint64_t *hval;
+  int4 *tmp;
+  int4 *nodes;
+  int32_t K;
+};
+
+__global__ void k_big_select(const PairRec *__restrict__ pairs, uint64_t np, uint64_t min_hits, uint8_t *__restrict__ done,
+                             uint32_t *__restrict__ big, uint32_t *__restrict__ nbig) {
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const bool b = pairs[p].he - pairs[p].hb >= min_hits;
+  done[p] = b ? 1 : 0;
+  if (b) big[atomicAdd(nbig, 1u)] = (uint32_t)p;
+}
+
+__global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
+  __shared__ int32_t sLd[2][LW_CAP];   // live diagonals in hit order of the last offset (descending)
+  __shared__ int4 sR[2][LW_CAP];       // live nodes: Offset, Len, Start, list position
+  __shared__ int32_t sLq[2][LW_CAP];   // list position -> live index
+  __shared__ int32_t sQ[LW_CAP];       // per hit: live index of its node, or -1
+  __shared__ int32_t sT[LW_CAP];       // per list position: the hit touching it, or INF
+  __shared__ int32_t sM[LW_CAP];       // exclusive prefix max of sT
+  __shared__ int32_t sRk[LW_CAP];      // per hit: 2 x rank among created / moved hits, +1 if it is one
+  __shared__ int32_t sIp[LW_CAP];      // per list position: rank among nodes extended in place
+  __shared__ uint32_t sWork;
+  const int lane = threadIdx.x;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int32_t INF = 0x7fffffff;
+  const int32_t K = A.K;
+  const uint32_t nbig = *A.nbig;
+  for (;;) {
+    if (lane == 0) sWork = atomicAdd(A.next, 1u);
+    __syncthreads();
+    const uint32_t w = sWork;
+    __syncthreads();
+    if (w >= nbig) break;
+    const uint32_t p = A.big[w];
+    PairRec pr = A.pairs[p];
+    const uint64_t hb = pr.hb, he = pr.he;
+    int4 *dq = A.tmp + hb;
+    int cur = 0;
+    int32_t nA = 0, nnode = 0, ndq = 0, maxm = 0, prev_o = -2;
+    bool consistent = true, over = false;
+    int4 head = make_int4(0, 0, 0, 0);
+    uint64_t h = hb;
+    while (h < he) {
+      const int32_t o = (int32_t)(A.hval[h] & 0x1fffff);
+      int32_t c = 0;
+      for (;;) {
+        const uint64_t hh = h + (uint64_t)c + (uint64_t)lane;
+        const bool same = hh < he && (int32_t)(A.hval[hh] & 0x1fffff) == o;
+        const uint64_t b = __ballot(same);
+        const int32_t run = (b == ~0ull) ? 64 : __builtin_ctzll(~b);
+        c += run;
+        if (run < 64 || c > LW_CAP) break;
+      }
+      if (c > LW_CAP) { over = true; break; }
+      int4 *R = sR[cur], *R2 = sR[cur ^ 1];
+      int32_t *Ld = sLd[cur], *Ld2 = sLd[cur ^ 1], *Lq = sLq[cur], *Lq2 = sLq[cur ^ 1];
+      if (o != prev_o + 1 && nA > 0) {               // a gap: every live node dies, in list order
+        for (int32_t k = lane; k < nA; k += 64) {
+          int4 r = R[Lq[k]];
+          r.w = ndq + nA - k;
+          dq[ndq + k] = r;
+        }
+        head = R[Lq[0]];
+        ndq += nA;
+        nA = 0;
+      }
+      // hits -> live nodes
+      for (int32_t j = lane; j < c; j += 64) {
+        const uint64_t v = A.hval[h + j];
+        const int32_t g = (int32_t)((v >> 21) & 0x1fffff) - o;
+        maxm = max(maxm, (int32_t)(v >> 42));
+        int32_t lo = 0, hi = nA;
+        while (lo < hi) {
+          const int32_t mid = (lo + hi) >> 1;
+          if (Ld[mid] > g) lo = mid + 1;
+          else hi = mid;
+        }
+        sQ[j] = (lo < nA && Ld[lo] == g) ? lo : -1;
+        Ld2[j] = g;
+      }
+      for (int32_t k = lane; k < nA; k += 64) sT[k] = INF;
+      __syncthreads();
+      for (int32_t j = lane; j < c; j += 64) {
+        const int32_t q = sQ[j];
+        if (q >= 0) sT[R[q].w] = j;
+      }
+      __syncthreads();
+      {
+        int32_t carry = -1;
+        for (int32_t k0 = 0; k0 < nA; k0 += 64) {
+          const int32_t k = k0 + lane;
+          const int32_t t = (k < nA) ? sT[k] : -1;
+          const int32_t inc = wave_incl_max_i32(t);
+          const int32_t up = __shfl_up(inc, 1);
+          if (k < nA) sM[k] = (lane == 0) ? carry : max(up, carry);
+          carry = max(carry, __shfl(inc, 63));
+        }
+      }
+      __syncthreads();
+      // created / moved hits
+      int32_t nfr = 0, ncre = 0;
+      bool bad = false;
+      for (int32_t j0 = 0; j0 < c; j0 += 64) {
+        const int32_t j = j0 + lane;
+        bool cre = false, cf = false;
+        if (j < c) {
+          const int32_t q = sQ[j];
+          cre = q < 0;
+          cf = cre || sM[R[q].w] > j;
+        }
+        const uint64_t bcf = __ballot(cf), bcre = __ballot(cre);
+        if (j < c) sRk[j] = (nfr + (int32_t)__popcll(bcf & lt)) * 2 + (cf ? 1 : 0);
+        if (cre && (j > 0 || nA > 0)) bad = true;
+        nfr += (int32_t)__popcll(bcf);
+        ncre += (int32_t)__popcll(bcre);
+      }
+      if (nA == 0 && nnode > 0) {                    // first hit makes a node with no live node
+        const int32_t es = head.z + head.y - K + 1, dg = head.x - head.z;
+        if (abs(dg - Ld2[0]) > 3 || o < es + K - 2) bad = true;
+      }
+      if (__ballot(bad)) consistent = false;
+      // list positions: extended in place, dead
+      const int32_t sz = nA - (c - ncre);
+      {
+        int32_t ni = 0, nd = 0;
+        for (int32_t k0 = 0; k0 < nA; k0 += 64) {
+          const int32_t k = k0 + lane;
+          bool inpl = false, dead = false;
+          if (k < nA) {
+            const int32_t t = sT[k];
+            dead = t == INF;
+            inpl = !dead && sM[k] < t;
+          }
+          const uint64_t bi = __ballot(inpl), bd = __ballot(dead);
+          if (k < nA) sIp[k] = ni + (int32_t)__popcll(bi & lt);
+          if (dead) {
+            const int32_t r = nd + (int32_t)__popcll(bd & lt);
+            int4 rec = R[Lq[k]];
+            rec.w = ndq + sz - r;
+            dq[ndq + r] = rec;
+          }
+          ni += (int32_t)__popcll(bi);
+          nd += (int32_t)__popcll(bd);
+        }
+        ndq += sz;
+      }
+      __syncthreads();
+      // the new live set, in hit order, with list positions
+      for (int32_t j = lane; j < c; j += 64) {
+        const int32_t q = sQ[j], rk = sRk[j];
+        int4 rec;
+        int32_t P;
+        if (q < 0) {
+          rec = make_int4(Ld2[j] + o, K, o, 0);
+          P = nfr - 1 - (rk >> 1);
+        } else {
+          rec = R[q];
+          rec.y += 1;
+          P = (rk & 1) ? nfr - 1 - (rk >> 1) : nfr + sIp[rec.w];
+        }
+        rec.w = P;
+        R2[j] = rec;
+        Lq2[P] = j;
+      }
+      __syncthreads();
+      nnode += ncre;
+      nA = c;
+      prev_o = o;
+      h += (uint64_t)c;
+      cur ^= 1;
+    }
+    if (over) {
+      if (lane == 0) {
+        A.done[p] = 0;
+        atomicAdd(A.handed_back, 1u);
+      }
+      continue;
+    }
+    __threadfence();
+    __syncthreads();
+    int4 *out = A.nodes + hb;
+    const int4 *R = sR[cur];
+    const int32_t *Lq = sLq[cur];
+    int32_t longest = 0, dmin = 0x7fffffff, dmax = -0x7fffffff;
+    for (int32_t k = lane; k < nA; k += 64) {
+      int4 r = R[Lq[k]];
+      r.w = 0;
+      out[k] = r;
+      longest = max(longest, r.y);
+      dmin = min(dmin, r.x - r.z);
+      dmax = max(dmax, r.x - r.z);
+    }
+    for (int32_t i = lane; i < ndq; i += 64) {
+      int4 r = dq[i];
+      const int32_t d = nA + ndq - r.w;
+      r.w = 0;
+      out[d] = r;
+      longest = max(longest, r.y);
+      dmin = min(dmin, r.x - r.z);
+      dmax = max(dmax, r.x - r.z);
+    }
+    longest = wave_max_i32(longest);
+    dmin = wave_min_i32(dmin);
+    dmax = wave_max_i32(dmax);
+    maxm = wave_max_i32(maxm);
+    if (lane == 0) {
+      pr.nnodes = nA + ndq;
+      atomicAdd(A.nnodes_total, (unsigned long long)(nA + ndq));
+      pr.consistent = consistent ? 1 : 0;
+      pr.diag_bgn = (int32_t)(A.hval[hb] & 0x1fffff);
+      pr.diag_end = (int32_t)(A.hval[he - 1] & 0x1fffff);
+      pr.max_mult = maxm;
+      pr.longest = longest;
+      pr.dmin = dmin;
+      pr.dmax = dmax;
+      A.pairs[p] = pr;
+    }
+  }
 }
 
 // ------------------------------------------------------------------ per-pair Process_Matches

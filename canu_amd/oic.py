@@ -27,11 +27,19 @@ EXPORTS = [
     "oic_params_default", "oic_create", "oic_destroy", "oic_last_error", "oic_load_reads", "oic_index_build",
     "oic_search", "oic_run", "oic_get_counters", "oic_extend_batch", "oic_prefix_dp_batch", "oic_tables",
     "oic_dp_tables", "oic_set_tracing", "oic_get_traces", "oic_model_train", "oic_model_get", "oic_model_set",
-    "oic_model_predict", "oic_model_grad",
+    "oic_model_predict", "oic_model_grad", "oic_get_tuning", "oic_set_tuning", "oic_autotune",
 ]
 
 NFEAT, NOUT = 12, 2
 NPARAM = 64 * NFEAT + 64 + 64 * 64 + 64 + NOUT * 64 + NOUT
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("list_wave_min_hits", ctypes.c_uint32), ("fwd_lanes", ctypes.c_uint32), ("fwd_tile", ctypes.c_uint32)]
+
+
+class TuneTrial(ctypes.Structure):
+    _fields_ = [("tuning", Tuning), ("stage", ctypes.c_int32), ("ms", ctypes.c_double)]
 
 
 class TrainParams(ctypes.Structure):
@@ -69,7 +77,9 @@ class Sink(ctypes.Structure):
 class Counters(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("kmer_probes", "hits", "pairs", "nodes", "extensions", "dp_calls",
                                                "dp_cells", "dp_lane_slots", "dp_bases", "overflow_retries",
-                                               "spec_tasks", "spec_cells")] + \
+                                               "spec_tasks", "spec_cells", "spec_rows", "spec_lane_slots", "spec_wave_rows",
+                                               "spec_bases", "list_wave_pairs",
+                                               "list_handed_back")] + \
                [(n, ctypes.c_double) for n in ("ms_index", "ms_gather", "ms_lists", "ms_extend", "ms_total", "ms_dp_kernel",
                                                "ms_fwd_kernel")]
 
@@ -115,6 +125,10 @@ def lib():
         L.oic_model_set.argtypes = [vp, vp, vp, vp]
         L.oic_model_predict.argtypes = [vp, vp, ctypes.c_uint64, vp]
         L.oic_model_grad.argtypes = [vp, vp, vp, vp, ctypes.c_uint64, ctypes.c_float, ctypes.POINTER(ctypes.c_float), vp]
+        L.oic_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+        L.oic_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+        L.oic_autotune.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Tuning), ctypes.POINTER(TuneTrial),
+                                   ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
         _lib = L
     return _lib
 
@@ -226,6 +240,34 @@ class Engine:
         delta = np.zeros((len(tasks), delta_stride), np.int32)
         self._check(lib().oic_prefix_dp_batch(self._ctx, tasks.ctypes.data, len(tasks), res.ctypes.data, delta.ctypes.data, delta_stride))
         return res, delta
+
+    def tuning(self):
+        t = Tuning()
+        self._check(lib().oic_get_tuning(self._ctx, ctypes.byref(t)))
+        return {n: getattr(t, n) for n, _ in Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        """Change launch-shape knobs (oic_tuning); results never depend on them."""
+        t = Tuning()
+        self._check(lib().oic_get_tuning(self._ctx, ctypes.byref(t)))
+        for k, v in kw.items():
+            if not hasattr(t, k):
+                raise KeyError(k)
+            setattr(t, k, v)
+        self._check(lib().oic_set_tuning(self._ctx, ctypes.byref(t)))
+
+    def autotune(self, bgn_ref, end_ref):
+        """Pick the fastest launch shapes on a sample of ref reads (oic_autotune); returns
+        (chosen tuning, list of trials)."""
+        best = Tuning()
+        cap = 64
+        trials = (TuneTrial * cap)()
+        n = ctypes.c_uint32()
+        self._check(lib().oic_autotune(self._ctx, bgn_ref, end_ref, ctypes.byref(best), trials, cap, ctypes.byref(n)))
+        fields = [f for f, _ in Tuning._fields_]
+        out = [dict({f: getattr(t.tuning, f) for f in fields}, stage=("fwd", "lists")[t.stage], ms=t.ms)
+               for t in trials[:min(n.value, cap)]]
+        return {f: getattr(best, f) for f in fields}, out
 
     # ---- learned cost model
     def set_tracing(self, on=True):

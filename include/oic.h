@@ -114,10 +114,44 @@ typedef struct oic_counters {
   uint64_t overflow_retries;   /* DP tasks re-run with a wider band tile */
   uint64_t spec_tasks;         /* forward DPs run ahead of Process_Matches (batched schedules) */
   uint64_t spec_cells;         /* cells of those runs, used or not */
+  uint64_t spec_rows;          /* DP rows (e >= 1) of those runs */
+  uint64_t spec_lane_slots;    /* lane-cells issued by those runs */
+  uint64_t spec_wave_rows;     /* row iterations wavefronts executed for them (a wavefront waits for
+                                  its slowest packed DP) */
+  uint64_t spec_bases;         /* bases compared by those runs */
+  uint64_t list_wave_pairs;    /* pairs whose match list one wavefront built */
+  uint64_t list_handed_back;   /* of those, pairs with an offset of more hits than the wavefront holds */
   double   ms_index, ms_gather, ms_lists, ms_extend, ms_total;
   double   ms_dp_kernel;       /* device time of all DP kernels (HIP events) */
   double   ms_fwd_kernel;      /* of which the packed forward-DP kernels */
 } oic_counters;
+
+/* Launch-shape knobs.  No setting changes a result; the defaults suit MI355X.  No reference
+ * counterpart: overlapInCore has no device work to shape. */
+typedef struct oic_tuning {
+  uint32_t list_wave_min_hits;  /* pairs with at least this many seed hits build their match list
+                                   with one wavefront, smaller ones with one thread (default 2048) */
+  uint32_t fwd_lanes;           /* lanes per packed forward DP: 4, 8, 16 or 32 (64/fwd_lanes DPs
+                                   per wavefront; default 16) */
+  uint32_t fwd_tile;            /* band cells a packed forward DP holds in LDS: 128, 256 or 512;
+                                   wider bands re-run one per wavefront (default 256) */
+} oic_tuning;
+
+int oic_get_tuning(const oic_ctx *ctx, oic_tuning *out);
+int oic_set_tuning(oic_ctx *ctx, const oic_tuning *t);
+
+typedef struct oic_tune_trial {
+  oic_tuning tuning;
+  int32_t    stage;             /* 0: packed forward-DP kernel, 1: match-list stage */
+  double     ms;                /* time of that stage over the sample */
+} oic_tune_trial;
+
+/* Autotuner: searches ref reads [bgn_ref, end_ref] against the current index under each candidate
+ * launch shape (forward DP: 4/8/16/32 lanes x 128/256/512-cell tile; list builder threshold
+ * 256..16384 hits), keeps the fastest, and reports the trials (up to cap) when trials is not null.
+ * Counters are left as they were.  No reference counterpart. */
+int oic_autotune(oic_ctx *ctx, uint32_t bgn_ref, uint32_t end_ref, oic_tuning *best, oic_tune_trial *trials,
+                 uint32_t cap, uint32_t *ntrials);
 
 int         oic_create(const oic_params *p, oic_ctx **out);
 void        oic_destroy(oic_ctx *ctx);

@@ -130,6 +130,7 @@ def run_both(R, threads=4, schedule=oic.SCHED_PLAIN, **kw):
              minkmers=("minkmers", "minkmers"), hashbits=("hashbits", "hashBits"), hashload=("hashload", "hashLoad"),
              maxerate=("maxerate", "maxErate"), minlength=("minlength", "minOlap"))
     ranges = {}
+    tuning = kw.pop("tuning", None)
     for k, v in kw.items():
         if k in ("bgn_ref", "end_ref", "bgn_hash", "end_hash"):
             ranges[k] = v
@@ -138,6 +139,8 @@ def run_both(R, threads=4, schedule=oic.SCHED_PLAIN, **kw):
         ekw[ek] = v
         okw[ok] = int(v) if isinstance(v, bool) else v
     eng = oic.Engine(**ekw)
+    if tuning:
+        eng.set_tuning(**tuning)
     eng.load_reads(R)
     res = eng.run(ranges.get("bgn_hash", 1), ranges.get("end_hash", 0xFFFFFFFF), ranges.get("bgn_ref", 1),
                   ranges.get("end_ref", 0xFFFFFFFF))
@@ -220,6 +223,64 @@ def test_run_schedules_identical(c2s):
         a, b = oic.sort_records(r.records), oic.sort_records(base.records)
         assert len(a) == len(b) and all(np.array_equal(a[f], b[f]) for f in ("a_iid", "b_iid", "w0", "w1"))
         assert r.stats == base.stats
+
+
+@pytest.mark.parametrize("fixture", ["c1s", "c2s", "c3s"])
+@pytest.mark.parametrize("min_hits", [0, 1 << 31])
+def test_list_builders(request, fixture, min_hits):
+    """Match lists built by one wavefront per pair (min_hits 0) or one thread per pair (2^31)."""
+    R = request.getfixturevalue(fixture)
+    res, ref = run_both(R, schedule=oic.SCHED_BATCHED, tuning=dict(list_wave_min_hits=min_hits))
+    assert_same(res, ref)
+    assert res.counters["dp_cells"] == ref["cells_used"]
+    if min_hits == 0:
+        assert res.counters["list_wave_pairs"] == res.counters["pairs"]
+    else:
+        assert res.counters["list_wave_pairs"] == 0
+
+
+def test_forward_launch_shapes_identical(c3s):
+    """Every lanes-per-DP x band-tile shape of the packed forward kernel gives the oracle's output."""
+    ref = oracle.run(c3s, kmer=22, maxErate=0.045, minOlap=500, nThreads=4, ompThreads=16)
+    eng = oic.Engine(kmer=22, maxerate=0.045, minlength=500, threads=4, schedule=oic.SCHED_BATCHED)
+    eng.load_reads(c3s)
+    for lanes in (4, 8, 16, 32):
+        for tile in (128, 256, 512):
+            if lanes == 4 and tile == 512:
+                continue
+            eng.set_tuning(fwd_lanes=lanes, fwd_tile=tile)
+            res = eng.run()
+            assert_same(res, ref)
+            assert res.counters["dp_cells"] == ref["cells_used"], (lanes, tile)
+    with pytest.raises(oic.OicError):
+        eng.set_tuning(fwd_lanes=64)
+
+
+def test_autotune_keeps_results(c3s):
+    ref = oracle.run(c3s, kmer=22, maxErate=0.045, minOlap=500, nThreads=1, ompThreads=16)
+    eng = oic.Engine(kmer=22, maxerate=0.045, minlength=500, threads=1, schedule=oic.SCHED_BATCHED)
+    eng.load_reads(c3s)
+    eng.build_hash_index(1, c3s.n)
+    best, trials = eng.autotune(1, min(c3s.n, 60))
+    assert len(trials) == 1 + 11 + 4
+    assert best["fwd_lanes"] in (4, 8, 16, 32) and best["fwd_tile"] in (128, 256, 512)
+    assert eng.tuning() == best
+    assert_same(eng.run(), ref)
+
+
+def test_list_builder_hands_back_crowded_offsets():
+    """An S k-mer with more T occurrences than the wavefront builder holds (550 copies of a 20 bp
+    unit) is handed back to the one-thread builder; results stay exact."""
+    rng = np.random.default_rng(12)
+    unit = "".join(rng.choice(list("ACGT"), 20))
+    other = "".join(rng.choice(list("ACGT"), 4000))
+    t = other[:2000] + unit * 550 + other[2000:]
+    s = unit * 30 + other[2000:2600]
+    R = readsim.from_strings([s, t])
+    res, ref = run_both(R, threads=1, schedule=oic.SCHED_BATCHED, tuning=dict(list_wave_min_hits=0))
+    assert_same(res, ref)
+    assert res.counters["list_handed_back"] >= 1
+    assert res.counters["dp_cells"] == ref["cells_used"]
 
 
 def test_run_reads_with_n_and_short_reads():
