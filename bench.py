@@ -49,7 +49,7 @@ def traffic_from_profiles(kernel):
         return None
     for line in open(path):
         m = re.match(r"\s*(\S+)\s+hbm_bytes_per_dispatch=(\d+)", line)
-        if m and m.group(1) == kernel:
+        if m and m.group(1) == kernel.replace(" ", ""):
             return int(m.group(2))
     return None
 
@@ -111,11 +111,27 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain one-pair-per-wavefront comparison")
     ap.add_argument("--plain-refs", type=int, default=200, help="ref reads of the plain-vs-scheduled comparison sample")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--fwd-lanes", type=int, default=0, help="fix the packed forward kernel's lanes per DP (skips autotuning)")
+    ap.add_argument("--fwd-tile", type=int, default=0, help="fix its band tile (with --fwd-lanes)")
     ap.add_argument("--tune-refs", type=int, default=200, help="ref reads the autotuner times each launch shape on")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-genome", type=int, default=300_000)
     ap.add_argument("--train-genome", type=int, default=12_000_000, help="genome of the c3 instance the model learns from")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES between two device buffers (the reference kernel for HBM byte counters)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration
+        src = torch.randint(0, 255, (n,), dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "launches": 3, "ok": bool(torch.equal(src, dst))}), flush=True)
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -148,8 +164,10 @@ def main():
 
     first = 1
     timed_end = first + (args.warmup + args.steps) * args.batch - 1
+    if args.fwd_lanes:
+        eng.set_tuning(fwd_lanes=args.fwd_lanes, fwd_tile=args.fwd_tile or 128)
     tuning = eng.tuning()
-    if not args.no_autotune and args.schedule != "plain":
+    if not args.no_autotune and not args.fwd_lanes and args.schedule != "plain":
         tb = min(R.n - 1, timed_end + 1)
         t0 = time.perf_counter()
         tuning, trials = eng.autotune(tb, min(R.n, tb + args.tune_refs - 1))
@@ -163,11 +181,11 @@ def main():
     olap_s = recs / tot
     steps = args.steps
     if args.schedule == "plain":
-        kname, kms, ops, kbases = "k_process_pairs<512>", d["ms_dp_kernel"], 4 * d["dp_cells"], d["dp_bases"]
+        kname, kms, ops, kbases, nl = "k_process_pairs<512>", d["ms_dp_kernel"], 4 * d["dp_cells"], d["dp_bases"], None
     else:
-        # the packed forward-DP kernel does almost all of the DP work of the scheduled paths
+        # the packed narrow-band forward-DP kernel does almost all of the DP work of the scheduled paths
         kname = f"k_fwd_tasks<{tuning['fwd_lanes']}, {tuning['fwd_tile']}>"
-        kms, ops, kbases = d["ms_fwd_kernel"], 4 * d["spec_cells"], d["spec_bases"]
+        kms, ops, kbases, nl = d["ms_fwd_narrow"], 4 * d["fwd_cells"], d["fwd_bases"], d["fwd_launches"]
     ops += 2 * kbases / 16.0
     achieved = ops / (kms / 1e3) / 1e12 if kms > 0 else 0.0
     line = {
@@ -184,7 +202,10 @@ def main():
         "cost_model": info,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT_TOPS, "unit": "TOPS",
                      "frac": achieved / PEAK_INT_TOPS, "traffic": traffic_from_profiles(kname),
-                     "kernel": kname, "kernel_ms_per_step": kms / steps, "int_ops_per_step": ops / steps},
+                     "kernel": kname, "kernel_ms_per_step": kms / steps, "int_ops_per_step": ops / steps,
+                     "launches_per_step": (nl / steps) if nl else None,
+                     "avg_launch_ms": (kms / nl) if nl else None,
+                     "int_ops_per_launch": (ops / nl) if nl else None},
         "counters_per_step": {k: (v / steps) for k, v in d.items()},
     }
     if not args.no_plain and args.schedule != "plain":

@@ -579,6 +579,7 @@ static int run_fwd(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
   if (n == 0) return OIC_OK;
   FA.ntasks = n;
   HIPCHK(c, hipMemsetAsync(FA.queue, 0, 4, c->st));
+  HIPCHK(c, hipMemsetAsync(FA.stats, 0, 64, c->st));
   const uint32_t waves = (uint32_t)std::min<uint64_t>((uint64_t)c->ncu * 16, (n + (OIC_WAVE / G) - 1) / (OIC_WAVE / G));
   HIPCHK(c, hipEventRecord(c->ev0, c->st));
   hipLaunchKernelGGL((k_fwd_tasks<G, CAP>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
@@ -588,6 +589,19 @@ static int run_fwd(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
   float f = 0;
   hipEventElapsedTime(&f, c->ev0, c->ev1);
   *ms += f;
+  unsigned long long st[8];
+  HIPCHK(c, hipMemcpy(st, FA.stats, sizeof st, hipMemcpyDeviceToHost));
+  c->ctr.spec_cells += st[0];
+  c->ctr.spec_rows += st[1];
+  c->ctr.spec_lane_slots += st[2];
+  c->ctr.spec_wave_rows += st[3];
+  c->ctr.spec_bases += st[4];
+  if (FA.ovf) {   // the packed narrow-band kernel
+    c->ctr.fwd_launches++;
+    c->ctr.fwd_cells += st[0];
+    c->ctr.fwd_bases += st[4];
+    c->ctr.ms_fwd_narrow += f;
+  }
   return OIC_OK;
 }
 
@@ -627,7 +641,6 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
   ALLOC(c, c->b_off, np * 8);
   ALLOC(c, c->b_stats, 64);
   HIPCHK(c, hipMemsetAsync(c->b_cache.p, 0, H, c->st));
-  HIPCHK(c, hipMemsetAsync(c->b_stats.p, 0, 64, c->st));
   const unsigned g = (unsigned)((np + 255) / 256);
   hipLaunchKernelGGL(k_probe_tasks, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->p_nodes.as<int4>(), c->b_probe.as<FwdTask>());
   FwdArgs FA;
@@ -663,13 +676,6 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
     rc = run_fwd_binned(c, FA, nt, &ms);
     if (rc) return rc;
   }
-  unsigned long long st[8];
-  HIPCHK(c, hipMemcpy(st, c->b_stats.p, sizeof st, hipMemcpyDeviceToHost));
-  c->ctr.spec_cells += st[0];
-  c->ctr.spec_rows += st[1];
-  c->ctr.spec_lane_slots += st[2];
-  c->ctr.spec_wave_rows += st[3];
-  c->ctr.spec_bases += st[4];
   c->ctr.spec_tasks += np + nt;
   c->ctr.ms_fwd_kernel += ms;
   c->ctr.ms_dp_kernel += ms;

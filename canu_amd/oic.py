@@ -78,10 +78,10 @@ class Counters(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("kmer_probes", "hits", "pairs", "nodes", "extensions", "dp_calls",
                                                "dp_cells", "dp_lane_slots", "dp_bases", "overflow_retries",
                                                "spec_tasks", "spec_cells", "spec_rows", "spec_lane_slots", "spec_wave_rows",
-                                               "spec_bases", "list_wave_pairs",
+                                               "spec_bases", "fwd_launches", "fwd_cells", "fwd_bases", "list_wave_pairs",
                                                "list_handed_back")] + \
                [(n, ctypes.c_double) for n in ("ms_index", "ms_gather", "ms_lists", "ms_extend", "ms_total", "ms_dp_kernel",
-                                               "ms_fwd_kernel")]
+                                               "ms_fwd_kernel", "ms_fwd_narrow")]
 
 
 EXT_TASK = np.dtype([("s_id", "<u4"), ("t_id", "<u4"), ("s_dir", "<i4"), ("start", "<i4"), ("offset", "<i4"), ("len", "<i4")])

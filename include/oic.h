@@ -119,11 +119,15 @@ typedef struct oic_counters {
   uint64_t spec_wave_rows;     /* row iterations wavefronts executed for them (a wavefront waits for
                                   its slowest packed DP) */
   uint64_t spec_bases;         /* bases compared by those runs */
+  uint64_t fwd_launches;       /* launches of the packed narrow-band forward kernel */
+  uint64_t fwd_cells;          /* cells and bases of those launches */
+  uint64_t fwd_bases;
   uint64_t list_wave_pairs;    /* pairs whose match list one wavefront built */
   uint64_t list_handed_back;   /* of those, pairs with an offset of more hits than the wavefront holds */
   double   ms_index, ms_gather, ms_lists, ms_extend, ms_total;
   double   ms_dp_kernel;       /* device time of all DP kernels (HIP events) */
   double   ms_fwd_kernel;      /* of which the packed forward-DP kernels */
+  double   ms_fwd_narrow;      /* of which the packed narrow-band kernel (HIP events on its stream) */
 } oic_counters;
 
 /* Launch-shape knobs.  No setting changes a result; the defaults suit MI355X.  No reference
