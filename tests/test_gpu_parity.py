@@ -196,6 +196,15 @@ def test_run_c3_small_repeats(c3s, schedule):
     assert res.counters["dp_cells"] == ref["cells_used"]
 
 
+def test_run_c5_small_hifi_hpc():
+    """C5 generator: homopolymer-compressed HiFi-like reads, --maxerate 0.01 --minlength 1000."""
+    R = readsim.simulate(readsim.PRESETS["c5"].scaled(400_000, seed=46))
+    res, ref = run_both(R, schedule=oic.SCHED_BATCHED, maxerate=0.01, minlength=1000)
+    assert len(ref["records"]) > 100
+    assert_same(res, ref)
+    assert res.counters["dp_cells"] == ref["cells_used"]
+
+
 @pytest.mark.parametrize("opt", [dict(unique=False), dict(partial=True), dict(hopeless=False), dict(minkmers=True),
                                  dict(maxerate=0.01, minlength=1000), dict(maxerate=0.12)])
 def test_run_options(c1s, opt):
