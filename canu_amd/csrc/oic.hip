@@ -87,7 +87,7 @@ struct oic_ctx {
   // sort / scan temporaries
   DevBuf s_k1, s_v1, s_k2, s_v2, s_hist, s_offs, s_scan, s_flag, s_idx, s_small;
   // search
-  DevBuf g_cnt, g_off, g_pbase, g_searched, g_sdsum;
+  DevBuf g_cnt, g_off, g_pbase, g_searched, g_sdsum, g_phi, g_pval;
   DevBuf l_scr, l_done, l_big, l_cnt;
   oic_tuning tune{};
   DevBuf p_pairs, p_tmp, p_nodes, p_recs, p_pout, p_order, p_queue, p_stats, p_nout, p_noff, p_outrec, p_retry;
@@ -719,8 +719,10 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   if (GA.tbits + sbits > 64) return fail(c, OIC_EINVAL, "pair key needs %d bits", GA.tbits + sbits);
   if (c->nu == 0) return OIC_OK;
   ALLOC(c, c->g_sdsum, 2 * (size_t)ns * 8);
+  ALLOC(c, c->g_phi, P * 8);
+  ALLOC(c, c->g_pval, P * 8);
   hipLaunchKernelGGL(k_hit_count, dim3(2 * ns), dim3(256), 0, c->st, GA, c->g_cnt.as<uint32_t>(), c->g_pbase.as<uint64_t>(),
-                     c->g_sdsum.as<unsigned long long>());
+                     c->g_sdsum.as<unsigned long long>(), c->g_phi.as<uint64_t>(), c->g_pval.as<uint64_t>());
   HIPCHK(c, hipGetLastError());
   ALLOC(c, c->s_scan, scan_scratch_words(P) * 8);
   uint64_t H = 0;
@@ -752,7 +754,8 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   ALLOC(c, c->s_v1, H * 8);
   if (ensure_sort_temp(c, H) != OIC_OK) return OIC_ENOMEM;
   uint64_t *hk = c->s_k1.as<uint64_t>(), *hv = c->s_v1.as<uint64_t>();
-  hipLaunchKernelGGL(k_hit_emit, dim3(2 * ns), dim3(256), 0, c->st, GA, c->g_off.as<uint64_t>(), c->g_pbase.as<uint64_t>(), hk, hv);
+  hipLaunchKernelGGL(k_hit_emit_tiles, dim3((unsigned)((H + EMIT_TILE - 1) / EMIT_TILE)), dim3(256), 0, c->st, c->g_off.as<uint64_t>(),
+                     c->g_phi.as<uint64_t>(), c->g_pval.as<uint64_t>(), P, H, GA.X.occ, GA.tbits, hk, hv);
   HIPCHK(c, hipGetLastError());
   {
     RadixTemp tmp{c->s_k2.as<uint64_t>(), c->s_v2.as<uint64_t>(), c->s_hist.as<uint32_t>(), c->s_offs.as<uint64_t>(), c->s_scan.as<uint64_t>()};

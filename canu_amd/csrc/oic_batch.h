@@ -48,13 +48,45 @@ __device__ __forceinline__ void wave_fence() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// window of 16 bases at view position p: from the LDS copy when it covers p, else from HBM
-__device__ __forceinline__ uint32_t win16_staged(const SeqV &v, const uint32_t *__restrict__ lw, int64_t w0, int64_t p) {
-  const int64_t q = v.b0 + p;
-  const int64_t wi = (q >> 4) - w0;
-  const uint32_t s = (uint32_t)(q & 15) * 2u;
-  if (wi >= 0 && wi + 1 < OIC_SEQ_WIN) return __builtin_amdgcn_alignbit(lw[wi + 1], lw[wi], s);
-  return __builtin_amdgcn_alignbit(v.w[(q >> 4) + 1], v.w[q >> 4], s);
+// A tail staged in LDS: its first OIC_SEQ_WIN words, from the word holding base 0 of the view.
+// Positions are 32-bit offsets from that word's first base; HBM is read only past the copy.
+struct Staged {
+  const uint32_t *w;        // all packed reads
+  int64_t w0;               // first staged word
+  int32_t r0;               // base 0 of the view within word w0 (0..15)
+  const uint32_t *lds;
+};
+
+__device__ __forceinline__ Staged make_staged(const SeqV &v, const uint32_t *lds) {
+  Staged t;
+  t.w = v.w;
+  t.w0 = v.b0 >> 4;
+  t.r0 = (int32_t)(v.b0 & 15);
+  t.lds = lds;
+  return t;
+}
+
+// 16 bases from view position p (p >= 0)
+__device__ __forceinline__ uint32_t win16_staged(const Staged &t, int32_t p) {
+  const int32_t q = t.r0 + p;
+  const int32_t wi = q >> 4;
+  const uint32_t sh = (uint32_t)(q & 15) * 2u;
+  if (wi + 1 < OIC_SEQ_WIN) return __builtin_amdgcn_alignbit(t.lds[wi + 1], t.lds[wi], sh);
+  return __builtin_amdgcn_alignbit(t.w[t.w0 + wi + 1], t.w[t.w0 + wi], sh);
+}
+
+// lexicographic (max v, then min d) over the G lanes of a group, by DPP moves within rows of 16
+template <int G>
+__device__ __forceinline__ void group_argmax(int32_t &v, int32_t &d) {
+  auto take = [&](int32_t ov, int32_t od) {
+    if (ov > v || (ov == v && od < d)) { v = ov; d = od; }
+  };
+  if (G >= 2) take(__builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false), __builtin_amdgcn_mov_dpp(d, 0xB1, 0xF, 0xF, false));    // quad_perm [1,0,3,2]
+  if (G >= 4) take(__builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false), __builtin_amdgcn_mov_dpp(d, 0x4E, 0xF, 0xF, false));    // quad_perm [2,3,0,1]
+  if (G >= 8) take(__builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false), __builtin_amdgcn_mov_dpp(d, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  if (G >= 16) take(__builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false), __builtin_amdgcn_mov_dpp(d, 0x140, 0xF, 0xF, false)); // row_mirror
+  if (G >= 32) take(__shfl_xor(v, 16), __shfl_xor(d, 16));
+  if (G >= 64) take(__shfl_xor(v, 32), __shfl_xor(d, 32));
 }
 
 template <int G>
@@ -68,11 +100,11 @@ struct GroupDP {
   bool overflow;
   int32_t e;          // errors at the end (mte) or of the best branch point
   int32_t exit_d;     // diagonal of the end cell / best branch point
-  uint64_t cells;
+  uint32_t cells;
   uint32_t rows;      // rows e >= 1 entered
   uint32_t slots;     // lane-cells issued (G per band chunk)
   uint32_t wrows;     // row iterations the whole wavefront executed (counted on its first active lane)
-  uint64_t bases;     // bases compared: slide lengths plus one terminating compare per cell
+  uint32_t bases;     // this lane's bases compared: slide lengths plus one terminating compare per cell
 };
 
 // One prefix DP (forward() row loop, no row storage) on a G-lane group; sa/st are the group's LDS
@@ -84,10 +116,10 @@ __device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, 
   GroupDP r;
   r.mte = false; r.overflow = false; r.e = 0; r.exit_d = 0; r.cells = 1; r.rows = 0; r.slots = 0; r.wrows = 0; r.bases = 0;
   const bool nA = a.nm != nullptr, nB = b.nm != nullptr;
-  const int64_t wa0 = a.b0 >> 4, wb0 = b.b0 >> 4;
+  const Staged ta = make_staged(a, sa), tb_ = make_staged(b, sb);
   for (int i = gl; i < OIC_SEQ_WIN; i += G) {
-    sa[i] = a.w[wa0 + i];
-    sb[i] = b.w[wb0 + i];
+    sa[i] = a.w[ta.w0 + i];
+    sb[i] = b.w[tb_.w0 + i];
   }
   wave_fence();
   int32_t row0 = 0;
@@ -97,7 +129,7 @@ __device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, 
     int32_t k = 0;
     bool stop = true;
     if (rem > 0) {
-      uint32_t x = win16_staged(a, sa, wa0, off) ^ win16_staged(b, sb, wb0, off);
+      uint32_t x = win16_staged(ta, off) ^ win16_staged(tb_, off);
       x = (x | (x >> 1)) & 0x55555555u;
       if (nA) x &= ~nwin16(a.nm, a.b0 + off);
       if (nB) x &= ~nwin16(b.nm, b.b0 + off);
@@ -115,6 +147,35 @@ __device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, 
   if (gl == 0) pbuf[0] = row0;
   int32_t Left = 0, Right = 0, Longest = 0, Best_d = 0, Best_e = 0, MSL = 0, MSd = 0, MSe = 0;
   double MaxScore = 0.0;
+  // slide along diagonal d from Row while the bases match (prefixEditDistance-forward.C:155-167)
+  auto slide = [&](int32_t Row, int32_t d) -> int32_t {
+    const int32_t r1 = m - Row, r2 = n - Row - d;
+    int32_t rem = r1 < r2 ? r1 : r2;
+    int32_t adv = 0;
+    while (rem > 0) {
+      uint32_t x = win16_staged(ta, Row + adv) ^ win16_staged(tb_, Row + d + adv);
+      x = (x | (x >> 1)) & 0x55555555u;
+      if (nA) x &= ~nwin16(a.nm, a.b0 + Row + adv);
+      if (nB) x &= ~nwin16(b.nm, b.b0 + Row + d + adv);
+      const int32_t kk = x ? (int32_t)(__builtin_ctz(x) >> 1) : 16;
+      if (kk >= rem) { adv += rem; break; }
+      adv += kk;
+      if (kk < 16) break;
+      rem -= 16;
+    }
+    r.bases += (uint32_t)adv + 1u;
+    return Row + adv;
+  };
+  // the end of A or T was reached on diagonal rd at row rrow: branch-point test (:172-212)
+  auto reached = [&](int32_t e, int32_t rrow, int32_t rd) {
+    const double Score = (double)rrow * tb.bmv - (double)e;
+    const int32_t Tail_Len = rrow - MSL;
+    const double slope = (double)(MaxScore - Score) / (double)Tail_Len;
+    const bool abort_ = ((uint32_t)e > tb.min_branch_end_dist / 2u) && ((uint32_t)Tail_Len >= tb.min_branch_end_dist) &&
+                        (slope >= tb.slope_min);
+    if (abort_) { r.mte = false; r.e = MSe; r.exit_d = MSd; }
+    else        { r.mte = true;  r.e = e;   r.exit_d = rd; }
+  };
   for (int32_t e = 1; e <= limit; e++) {
     const int32_t eml = tb.match_limit[e];
     Left = (Left - 1 > -e) ? Left - 1 : -e;
@@ -137,41 +198,18 @@ __device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, 
         const int32_t x1 = 1 + pbuf[d & MASK], x2 = pbuf[(d - 1) & MASK], x3 = 1 + pbuf[(d + 1) & MASK];
         Row = x1 > x2 ? x1 : x2;
         Row = Row > x3 ? Row : x3;
-        const int32_t r1 = m - Row, r2 = n - Row - d;
-        int32_t rem = r1 < r2 ? r1 : r2;
-        int32_t adv = 0;
-        while (rem > 0) {
-          uint32_t x = win16_staged(a, sa, wa0, Row + adv) ^ win16_staged(b, sb, wb0, (int64_t)Row + d + adv);
-          x = (x | (x >> 1)) & 0x55555555u;
-          if (nA) x &= ~nwin16(a.nm, a.b0 + Row + adv);
-          if (nB) x &= ~nwin16(b.nm, b.b0 + Row + d + adv);
-          const int32_t kk = x ? (int32_t)(__builtin_ctz(x) >> 1) : 16;
-          if (kk >= rem) { adv += rem; break; }
-          adv += kk;
-          if (kk < 16) break;
-          rem -= 16;
-        }
-        Row += adv;
-        r.bases += (uint64_t)adv + 1;
+        Row = slide(Row, d);
         cbuf[d & MASK] = Row;
       }
       const bool reach = act && (Row == m || Row + d == n);
       const uint64_t bl = group_ballot<G>(reach, g);
       if (bl) {
         const int f = __builtin_ctzll(bl);
-        r.cells += (uint64_t)(f + 1);
-        const int32_t rd = c0 + f;
-        const int32_t rrow = __shfl(Row, g * G + f);
-        const double Score = (double)rrow * tb.bmv - (double)e;
-        const int32_t Tail_Len = rrow - MSL;
-        const double slope = (double)(MaxScore - Score) / (double)Tail_Len;
-        const bool abort_ = ((uint32_t)e > tb.min_branch_end_dist / 2u) && ((uint32_t)Tail_Len >= tb.min_branch_end_dist) &&
-                            (slope >= tb.slope_min);
-        if (abort_) { r.mte = false; r.e = MSe; r.exit_d = MSd; }
-        else        { r.mte = true;  r.e = e;   r.exit_d = rd; }
+        r.cells += (uint32_t)(f + 1);
+        reached(e, __shfl(Row, g * G + f), c0 + f);
         return r;
       }
-      r.cells += (uint64_t)((Right - c0 + 1) < G ? (Right - c0 + 1) : G);
+      r.cells += (uint32_t)((Right - c0 + 1) < G ? (Right - c0 + 1) : G);
     }
     wave_fence();
     int32_t nL = Right + 1;
@@ -199,11 +237,7 @@ __device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, 
         if (v > bv) { bv = v; bd = d; }
       }
     }
-#pragma unroll
-    for (int o = 1; o < G; o <<= 1) {
-      const int32_t ov = __shfl_xor(bv, o), od = __shfl_xor(bd, o);
-      if (ov > bv || (ov == bv && od < bd)) { bv = ov; bd = od; }
-    }
+    group_argmax<G>(bv, bd);
     if (bv > Longest) { Longest = bv; Best_d = bd; Best_e = e; }
     const double Score = (double)Longest * tb.bmv - (double)e;
     if (Score > MaxScore) { MaxScore = Score; MSL = Longest; MSd = Best_d; MSe = Best_e; }
@@ -223,7 +257,9 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
   __shared__ uint32_t fetched[NG];
   const int lane = lane_id();
   const int g = lane / G, gl = lane % G;
-  uint64_t cells_acc = 0, rows_acc = 0, slots_acc = 0, wrows_acc = 0, bases_acc = 0;
+  // per-launch totals per lane stay far below 2^32 (about 10^6 per group at the c3 workload)
+  uint64_t cells_acc = 0;
+  uint32_t rows_acc = 0, slots_acc = 0, wrows_acc = 0, bases_acc = 0;
   uint64_t ti = 0, tend = 0;
 
   while (true) {
@@ -247,7 +283,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
     const int32_t Total_Olap = (Start < Offset ? Start : Offset) + Len + (S_Right_Len < T_Right_Len ? S_Right_Len : T_Right_Len);
     const int32_t limit = A.tb.error_bound[Total_Olap];
     const SeqV S = read_view(A.R, pr.s_id, pr.dir), T = read_view(A.R, pr.t_id, 0);
-    uint64_t cells = 0, rows = 0, slots = 0, wrows = 0, bases = 0;
+    uint32_t cells = 0, rows = 0, slots = 0, wrows = 0, bases = 0;
     bool rmte = true, overflow = false;
     int32_t right_e = 0, exit_d = T_Right_Begin - S_Right_Begin;
     if (!(S_Right_Len == 0 || T_Right_Len == 0)) {
@@ -292,7 +328,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
       // 1: not a dovetail (Process_Matches only removes the node), 2: dovetail, 0: not computed
       A.cache[gnode] = overflow ? 0 : ((rmte && lmte) ? 2 : 1);
       if (overflow && A.ovf) A.ovf[atomicAdd(A.novf, 1u)] = tk;
-      A.ncells[gnode] = (uint32_t)(cells > 0xffffffffull ? 0xffffffffull : cells);
+      A.ncells[gnode] = cells;
       A.nexit[gnode] = exit_d;
       if (!overflow) cells_acc += cells;
       rows_acc += rows;
@@ -302,19 +338,20 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
     bases_acc += bases;
     ti++;
   }
+  uint64_t rows64 = rows_acc, slots64 = slots_acc, wrows64 = wrows_acc, bases64 = bases_acc;
   for (int o = 32; o > 0; o >>= 1) {
     cells_acc += __shfl_xor(cells_acc, o);
-    rows_acc += __shfl_xor(rows_acc, o);
-    slots_acc += __shfl_xor(slots_acc, o);
-    wrows_acc += __shfl_xor(wrows_acc, o);
-    bases_acc += __shfl_xor(bases_acc, o);
+    rows64 += __shfl_xor(rows64, o);
+    slots64 += __shfl_xor(slots64, o);
+    wrows64 += __shfl_xor(wrows64, o);
+    bases64 += __shfl_xor(bases64, o);
   }
   if (lane == 0 && A.stats) {
-    atomicAdd(&A.stats[3], (unsigned long long)wrows_acc);
-    atomicAdd(&A.stats[4], (unsigned long long)bases_acc);
+    atomicAdd(&A.stats[3], (unsigned long long)wrows64);
+    atomicAdd(&A.stats[4], (unsigned long long)bases64);
     atomicAdd(&A.stats[0], (unsigned long long)cells_acc);
-    atomicAdd(&A.stats[1], (unsigned long long)rows_acc);
-    atomicAdd(&A.stats[2], (unsigned long long)slots_acc);
+    atomicAdd(&A.stats[1], (unsigned long long)rows64);
+    atomicAdd(&A.stats[2], (unsigned long long)slots64);
   }
 }
 

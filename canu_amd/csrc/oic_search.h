@@ -204,7 +204,8 @@ struct GatherArgs {
 // one block per (S, orientation): count hits with S_ID < T_ID for every window; sdsum gets the
 // block's total
 __global__ void __launch_bounds__(256) k_hit_count(GatherArgs A, uint32_t *__restrict__ cnt, const uint64_t *__restrict__ pbase,
-                                                   unsigned long long *__restrict__ sdsum) {
+                                                   unsigned long long *__restrict__ sdsum, uint64_t *__restrict__ phi,
+                                                   uint64_t *__restrict__ pval) {
   __shared__ unsigned long long bsum;
   if (threadIdx.x == 0) bsum = 0;
   __syncthreads();
@@ -230,6 +231,12 @@ __global__ void __launch_bounds__(256) k_hit_count(GatherArgs A, uint32_t *__res
       if (table_find(A.X, key, &lo, &hi)) {
         const uint64_t f = (tl_max == 0xffffffffu) ? lo : occ_upper(A.X.occ, lo, hi, tl_max);
         c = (uint32_t)(hi - f);
+        if (c) {
+          const uint64_t mult = hi - lo;
+          const uint64_t mcap = mult > 0x3fffffull ? 0x3fffffull : mult;
+          phi[pb + o] = hi;
+          pval[pb + o] = (uint64_t)o | (mcap << 21) | ((uint64_t)sd << 43);
+        }
       }
     }
     cnt[pb + o] = c;
@@ -240,63 +247,67 @@ __global__ void __launch_bounds__(256) k_hit_count(GatherArgs A, uint32_t *__res
   if (threadIdx.x == 0) sdsum[sd] = bsum;
 }
 
-// same walk; write the hits in (o asc, T desc, T offset desc) order at the scanned offsets.
-// The hits of 64 consecutive offsets are one contiguous output range, written by the whole
-// wavefront: output slot s comes from the lane whose range holds s (binary search over the 64
-// range starts in LDS), so stores are coalesced and an offset with 10^4 occurrences (satellite
-// k-mers) is spread over all lanes instead of serialised on one.
-__global__ void __launch_bounds__(256) k_hit_emit(GatherArgs A, const uint64_t *__restrict__ off, const uint64_t *__restrict__ pbase,
-                                                  uint64_t *__restrict__ hkey, uint64_t *__restrict__ hval) {
-  __shared__ uint32_t s_start[4][64];
-  __shared__ uint64_t s_hi[4][64];
-  __shared__ uint64_t s_val[4][64];     // o | mult << 42
-  const uint32_t sd = blockIdx.x;
-  const uint32_t si = sd >> 1, dir = sd & 1;
-  if (si >= A.ns || !A.searched[si]) return;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const uint32_t S = A.s0 + si;
-  const int32_t L = A.R.len[S];
-  const int32_t npos = L - A.k + 1;
-  const int64_t b0 = dir ? A.R.b0r[S] : A.R.b0f[S];
-  const bool hn = A.R.hasN[S];
-  const uint64_t pb = pbase[sd];
-  const uint32_t tl_max = (S >= A.hbgn) ? (S - A.hbgn) : 0xffffffffu;
-  const uint64_t pref = (uint64_t)sd << A.tbits;
-  for (int32_t o0 = wv * 64; o0 < npos; o0 += 256) {
-    const int32_t o = o0 + lane;
-    uint64_t hi = 0, cnt = 0, mcap = 0;
-    if (o < npos && !(hn && kmer_has_n(A.R.nmask, b0 + o, A.k))) {
-      const uint64_t key = kmer_at(A.R.words, b0 + o, A.k);
-      uint64_t lo;
-      if (table_find(A.X, key, &lo, &hi)) {
-        const uint64_t f = (tl_max == 0xffffffffu) ? lo : occ_upper(A.X.occ, lo, hi, tl_max);
-        cnt = hi - f;
-        const uint64_t mult = hi - lo;
-        mcap = mult > 0x3fffffull ? 0x3fffffull : mult;
+// Write the hits in (o asc, T desc, T offset desc) order at the scanned offsets, load-balanced:
+// workgroup t writes output slots [t*EMIT_TILE, (t+1)*EMIT_TILE), so the 10^4 occurrences of a
+// satellite k-mer spread over many workgroups and stores are coalesced.  The S windows feeding a tile are found by binary search over the exclusive
+// scan of per-window counts; each slot's window is the last one whose first slot is <= the slot
+// (a window with no hits shares its first slot with the next window, so the search lands on the
+// window that holds the slot).  k_hit_count saved each window's occurrence bound and the packed
+// (o, multiplicity, S/orientation), so no table lookup is repeated.
+#define EMIT_TILE 4096
+#define EMIT_PCAP 4096
+
+__device__ __forceinline__ uint64_t last_le(const uint64_t *__restrict__ a, uint64_t lo, uint64_t hi, uint64_t s) {
+  // largest i in [lo, hi] with a[i] <= s; a[lo] <= s is given
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo + 1) / 2;
+    if (a[mid] <= s) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) k_hit_emit_tiles(const uint64_t *__restrict__ off, const uint64_t *__restrict__ phi,
+                                                        const uint64_t *__restrict__ pval, uint64_t npos, uint64_t nhits,
+                                                        const uint64_t *__restrict__ occ, int tbits, uint64_t *__restrict__ hkey,
+                                                        uint64_t *__restrict__ hval) {
+  __shared__ uint64_t s_off[EMIT_PCAP];
+  __shared__ uint64_t s_p0, s_np;
+  const uint64_t s0 = (uint64_t)blockIdx.x * EMIT_TILE;
+  if (s0 >= nhits) return;
+  const uint64_t s1 = (s0 + EMIT_TILE < nhits) ? s0 + EMIT_TILE : nhits;
+  if (threadIdx.x == 0) {
+    const uint64_t p0 = last_le(off, 0, npos - 1, s0);
+    const uint64_t p1 = last_le(off, p0, npos - 1, s1 - 1);
+    s_p0 = p0;
+    s_np = p1 - p0 + 1;
+  }
+  __syncthreads();
+  const uint64_t p0 = s_p0, np = s_np;
+  const bool in_lds = np <= EMIT_PCAP;
+  if (in_lds)
+    for (uint64_t i = threadIdx.x; i < np; i += blockDim.x) s_off[i] = off[p0 + i];
+  __syncthreads();
+  for (uint64_t s = s0 + threadIdx.x; s < s1; s += blockDim.x) {
+    uint64_t p, first;
+    if (in_lds) {
+      uint64_t lo = 0, hi = np - 1;
+      while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo + 1) / 2;
+        if (s_off[mid] <= s) lo = mid;
+        else hi = mid - 1;
       }
+      p = p0 + lo;
+      first = s_off[lo];
+    } else {
+      p = last_le(off, p0, p0 + np - 1, s);
+      first = off[p];
     }
-    const uint64_t w0 = off[pb + o0];               // first output slot of these 64 offsets
-    const uint64_t mine = (o < npos) ? off[pb + o] : 0;
-    const uint64_t last = __shfl(mine + cnt, 63 < (npos - 1 - o0) ? 63 : (npos - 1 - o0));
-    const uint32_t total = (uint32_t)(last - w0);
-    s_start[wv][lane] = (o < npos) ? (uint32_t)(mine - w0) : total;
-    s_hi[wv][lane] = hi;
-    s_val[wv][lane] = (uint64_t)(o < npos ? o : 0) | (mcap << 42);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    for (uint32_t s = lane; s < total; s += 64) {
-      int32_t l = 0;
-#pragma unroll
-      for (int32_t step = 32; step > 0; step >>= 1)
-        if (s_start[wv][l + step] <= s) l += step;
-      const uint64_t q = s_hi[wv][l] - 1 - (s - s_start[wv][l]);
-      const uint64_t oc = A.X.occ[q];
-      const uint32_t tl = (uint32_t)(oc >> 32), toff = (uint32_t)oc;
-      hkey[w0 + s] = pref | tl;
-      hval[w0 + s] = s_val[wv][l] | ((uint64_t)toff << 21);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    const uint64_t q = phi[p] - 1 - (s - first);
+    const uint64_t oc = occ[q];
+    const uint64_t v = pval[p];
+    hkey[s] = ((v >> 43) << tbits) | (oc >> 32);
+    hval[s] = (v & 0x1fffffull) | ((oc & 0xffffffffull) << 21) | (((v >> 21) & 0x3fffffull) << 42);
   }
 }
 

@@ -70,3 +70,24 @@ def test_gpu_training_fits_and_predicts():
     # oracle forward on the same weights
     yo, _ = M.forward(w.astype(np.float64), (x - mu) / sd)
     assert np.max(np.abs(yo - pred)) < 1e-3 * (1 + np.abs(yo).max())
+
+
+@pytest.mark.gpu
+def test_gpu_learned_model_beats_static_binning():
+    """What the trained network gains over the method without it: the static length-based cost
+    estimate (nodes x shorter read), on a C3-generator instance the model was not trained on.
+    Ranking quality of predicted DP cells, and the makespan of longest-first orders over 4096
+    workers evaluated on the true cells."""
+    from canu_amd import learned, oic
+    ftr, ttr = learned.collect_traces("c3", 3_000_000, seed=4, max_traces=600_000)
+    fte, tte = learned.collect_traces("c3", 2_000_000, seed=9, max_traces=300_000)
+    assert len(ftr) > 50_000 and len(fte) > 20_000
+    eng = oic.Engine()
+    eng.train_model(ftr, ttr, epochs=12)
+    pred = eng.predict(fte)[:, 0]
+    err = learned.prediction_error(tte, pred, fte)
+    ms = learned.ordering_gain(fte, tte, pred, workers=4096, chunk=80_000)
+    print(f"\nheld-out: {err}\nmakespans (cells): {ms}")
+    assert err["learned_spearman"] > err["static_spearman"] + 0.02
+    assert ms["learned"] <= 1.02 * ms["static"]
+    assert ms["learned"] < ms["plain"]
