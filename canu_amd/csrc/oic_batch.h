@@ -189,6 +189,12 @@ __device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, 
       pbuf[sd & MASK] = -2;
     }
     wave_fence();
+    // One pass over the band computes the row and, on the fly, the pruned range [nL, nR] (first and
+    // last cell with Row + max(d, 0) >= Edit_Match_Limit[e], forward.C:245-280) and the first-max
+    // cell inside it: per lane, "committed" holds the best cell at or before the latest kept cell,
+    // "pending" the best cell after it, promoted when a later kept cell appears.
+    int32_t nL = 0x7fffffff, nR = 0;
+    int32_t cbv = -1, cbd = 0x7fffffff, pbv = -1, pbd = 0x7fffffff;
     for (int32_t c0 = Left; c0 <= Right; c0 += G) {
       const int32_t d = c0 + gl;
       const bool act = d <= Right;
@@ -210,35 +216,32 @@ __device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, 
         return r;
       }
       r.cells += (uint32_t)((Right - c0 + 1) < G ? (Right - c0 + 1) : G);
-    }
-    wave_fence();
-    int32_t nL = Right + 1;
-    for (int32_t c0 = Left; c0 <= Right; c0 += G) {
-      const int32_t d = c0 + gl;
-      const bool keep = (d <= Right) && (cbuf[d & MASK] + (d > 0 ? d : 0) >= eml);
-      const uint64_t bl = group_ballot<G>(keep, g);
-      if (bl) { nL = c0 + __builtin_ctzll(bl); break; }
-    }
-    if (nL > Right) break;
-    int32_t nR = nL;
-    for (int32_t c1 = Right; c1 >= nL; c1 -= G) {
-      const int32_t d = c1 - gl;
-      const bool keep = (d >= nL) && (cbuf[d & MASK] + (d > 0 ? d : 0) >= eml);
-      const uint64_t bl = group_ballot<G>(keep, g);
-      if (bl) { nR = c1 - __builtin_ctzll(bl); break; }
-    }
-    Left = nL;
-    Right = nR;
-    int32_t bv = -1, bd = 0x7fffffff;
-    for (int32_t c0 = Left; c0 <= Right; c0 += G) {
-      const int32_t d = c0 + gl;
-      if (d <= Right) {
-        const int32_t v = cbuf[d & MASK];
-        if (v > bv) { bv = v; bd = d; }
+      const bool keep = act && (Row + (d > 0 ? d : 0) >= eml);
+      const uint64_t kb = group_ballot<G>(keep, g);
+      if (kb) {
+        const int32_t kl = c0 + (63 - __builtin_clzll(kb));
+        if (nL == 0x7fffffff) nL = c0 + __builtin_ctzll(kb);
+        nR = kl;
+        if (pbv > cbv || (pbv == cbv && pbd < cbd)) { cbv = pbv; cbd = pbd; }
+        pbv = -1;
+        pbd = 0x7fffffff;
+        if (act && d >= nL) {
+          if (d <= kl) {
+            if (Row > cbv || (Row == cbv && d < cbd)) { cbv = Row; cbd = d; }
+          } else {
+            pbv = Row;
+            pbd = d;
+          }
+        }
+      } else if (act && nL != 0x7fffffff) {
+        if (Row > pbv || (Row == pbv && d < pbd)) { pbv = Row; pbd = d; }
       }
     }
-    group_argmax<G>(bv, bd);
-    if (bv > Longest) { Longest = bv; Best_d = bd; Best_e = e; }
+    if (nL == 0x7fffffff) break;
+    Left = nL;
+    Right = nR;
+    group_argmax<G>(cbv, cbd);
+    if (cbv > Longest) { Longest = cbv; Best_d = cbd; Best_e = e; }
     const double Score = (double)Longest * tb.bmv - (double)e;
     if (Score > MaxScore) { MaxScore = Score; MSL = Longest; MSd = Best_d; MSe = Best_e; }
     int32_t *t = pbuf; pbuf = cbuf; cbuf = t;
