@@ -51,28 +51,26 @@ __device__ __forceinline__ void wave_fence() {
 // A tail staged in LDS: its first OIC_SEQ_WIN words, from the word holding base 0 of the view.
 // Positions are 32-bit offsets from that word's first base; HBM is read only past the copy.
 struct Staged {
-  const uint32_t *w;        // all packed reads
   int64_t w0;               // first staged word
   int32_t r0;               // base 0 of the view within word w0 (0..15)
   const uint32_t *lds;
 };
 
-__device__ __forceinline__ Staged make_staged(const SeqV &v, const uint32_t *lds) {
+__device__ __forceinline__ Staged make_staged(int64_t b0, const uint32_t *lds) {
   Staged t;
-  t.w = v.w;
-  t.w0 = v.b0 >> 4;
-  t.r0 = (int32_t)(v.b0 & 15);
+  t.w0 = b0 >> 4;
+  t.r0 = (int32_t)(b0 & 15);
   t.lds = lds;
   return t;
 }
 
-// 16 bases from view position p (p >= 0)
-__device__ __forceinline__ uint32_t win16_staged(const Staged &t, int32_t p) {
+// 16 bases from view position p (p >= 0); words = all packed reads
+__device__ __forceinline__ uint32_t win16_staged(const uint32_t *__restrict__ words, const Staged &t, int32_t p) {
   const int32_t q = t.r0 + p;
   const int32_t wi = q >> 4;
   const uint32_t sh = (uint32_t)(q & 15) * 2u;
   if (wi + 1 < OIC_SEQ_WIN) return __builtin_amdgcn_alignbit(t.lds[wi + 1], t.lds[wi], sh);
-  return __builtin_amdgcn_alignbit(t.w[t.w0 + wi + 1], t.w[t.w0 + wi], sh);
+  return __builtin_amdgcn_alignbit(words[t.w0 + wi + 1], words[t.w0 + wi], sh);
 }
 
 // lexicographic (max v, then min d) over the G lanes of a group, by DPP moves within rows of 16
@@ -109,17 +107,20 @@ struct GroupDP {
 
 // One prefix DP (forward() row loop, no row storage) on a G-lane group; sa/st are the group's LDS
 // copies of the first OIC_SEQ_WIN words of a and b.
+// a / b: start (in bases) of the two tails in the packed reads, and whether their reads hold N;
+// words / nmask are the packed reads and N mask of the whole read set.
 template <int G, int CAP>
-__device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, const int32_t n, const int32_t limit,
-                                   const DPTables &tb, int32_t *pbuf, int32_t *cbuf, uint32_t *sa, uint32_t *sb, int g, int gl) {
+__device__ GroupDP group_prefix_dp(const uint32_t *__restrict__ words, const uint32_t *__restrict__ nmask, const int64_t a_b0,
+                                   const bool nA, const int32_t m, const int64_t b_b0, const bool nB, const int32_t n,
+                                   const int32_t limit, const DPTables &tb, int32_t *pbuf, int32_t *cbuf, uint32_t *sa, uint32_t *sb,
+                                   int g, int gl) {
   constexpr int32_t MASK = CAP - 1;
   GroupDP r;
   r.mte = false; r.overflow = false; r.e = 0; r.exit_d = 0; r.cells = 1; r.rows = 0; r.slots = 0; r.wrows = 0; r.bases = 0;
-  const bool nA = a.nm != nullptr, nB = b.nm != nullptr;
-  const Staged ta = make_staged(a, sa), tb_ = make_staged(b, sb);
+  const Staged ta = make_staged(a_b0, sa), tb_ = make_staged(b_b0, sb);
   for (int i = gl; i < OIC_SEQ_WIN; i += G) {
-    sa[i] = a.w[ta.w0 + i];
-    sb[i] = b.w[tb_.w0 + i];
+    sa[i] = words[ta.w0 + i];
+    sb[i] = words[tb_.w0 + i];
   }
   wave_fence();
   int32_t row0 = 0;
@@ -129,10 +130,10 @@ __device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, 
     int32_t k = 0;
     bool stop = true;
     if (rem > 0) {
-      uint32_t x = win16_staged(ta, off) ^ win16_staged(tb_, off);
+      uint32_t x = win16_staged(words, ta, off) ^ win16_staged(words, tb_, off);
       x = (x | (x >> 1)) & 0x55555555u;
-      if (nA) x &= ~nwin16(a.nm, a.b0 + off);
-      if (nB) x &= ~nwin16(b.nm, b.b0 + off);
+      if (nA) x &= ~nwin16(nmask, a_b0 + off);
+      if (nB) x &= ~nwin16(nmask, b_b0 + off);
       const int32_t kk = x ? (int32_t)(__builtin_ctz(x) >> 1) : 16;
       if (kk >= rem) { k = rem; stop = true; } else { k = kk; stop = kk < 16; }
     }
@@ -153,10 +154,10 @@ __device__ GroupDP group_prefix_dp(const SeqV a, const int32_t m, const SeqV b, 
     int32_t rem = r1 < r2 ? r1 : r2;
     int32_t adv = 0;
     while (rem > 0) {
-      uint32_t x = win16_staged(ta, Row + adv) ^ win16_staged(tb_, Row + d + adv);
+      uint32_t x = win16_staged(words, ta, Row + adv) ^ win16_staged(words, tb_, Row + d + adv);
       x = (x | (x >> 1)) & 0x55555555u;
-      if (nA) x &= ~nwin16(a.nm, a.b0 + Row + adv);
-      if (nB) x &= ~nwin16(b.nm, b.b0 + Row + d + adv);
+      if (nA) x &= ~nwin16(nmask, a_b0 + Row + adv);
+      if (nB) x &= ~nwin16(nmask, b_b0 + Row + d + adv);
       const int32_t kk = x ? (int32_t)(__builtin_ctz(x) >> 1) : 16;
       if (kk >= rem) { adv += rem; break; }
       adv += kk;
@@ -274,51 +275,58 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
       tend = ti + FWD_CHUNK < A.ntasks ? ti + FWD_CHUNK : A.ntasks;
     }
     if (ti >= A.ntasks) break;               // group-uniform
+    uint32_t cells = 0, rows = 0, slots = 0, wrows = 0, bases = 0;
+    bool rmte = true, overflow = false, lmte = true;
+    int32_t right_e = 0, exit_d = 0, limit = 0;
+    {
+      const FwdTask tk = A.tasks[ti];
+      const PairRec pr = A.pairs[tk.pair];
+      const int4 nd = A.nodes[pr.hb + tk.node];
+      const int32_t S_Len = A.R.len[pr.s_id], T_Len = A.R.len[pr.t_id];
+      const int32_t Start = nd.z, Offset = nd.x, Len = nd.y;
+      const int32_t S_Right_Begin = Start + Len, S_Right_Len = S_Len - S_Right_Begin;
+      const int32_t T_Right_Begin = Offset + Len, T_Right_Len = T_Len - T_Right_Begin;
+      const int32_t Total_Olap = (Start < Offset ? Start : Offset) + Len + (S_Right_Len < T_Right_Len ? S_Right_Len : T_Right_Len);
+      limit = A.tb.error_bound[Total_Olap];
+      exit_d = T_Right_Begin - S_Right_Begin;
+      if (!(S_Right_Len == 0 || T_Right_Len == 0)) {
+        const int64_t s0 = (pr.dir ? A.R.b0r[pr.s_id] : A.R.b0f[pr.s_id]) + S_Right_Begin, t0 = A.R.b0f[pr.t_id] + T_Right_Begin;
+        const bool sN = A.R.hasN[pr.s_id], tN = A.R.hasN[pr.t_id];
+        const bool s_short = S_Right_Len <= T_Right_Len;   // the shorter tail is A (prefixEditDistance-extend.C:81-99)
+        const GroupDP f = group_prefix_dp<G, CAP>(A.R.words, A.R.nmask, s_short ? s0 : t0, s_short ? sN : tN,
+                                                  s_short ? S_Right_Len : T_Right_Len, s_short ? t0 : s0, s_short ? tN : sN,
+                                                  s_short ? T_Right_Len : S_Right_Len, limit, A.tb, tile[g][0], tile[g][1], seqA[g],
+                                                  seqT[g], g, gl);
+        cells += f.cells;
+        rows += f.rows;
+        slots += f.slots;
+        wrows += f.wrows;
+        bases += f.bases;
+        overflow = f.overflow;
+        rmte = f.mte;
+        right_e = f.e;
+        exit_d += s_short ? f.exit_d : -f.exit_d;
+      }
+    }
+    // The task's fields are read again below rather than kept in registers across the DP.
+    __asm__ volatile("" ::: "memory");
     const FwdTask tk = A.tasks[ti];
     const PairRec pr = A.pairs[tk.pair];
     const int4 nd = A.nodes[pr.hb + tk.node];
-    const int32_t S_Len = A.R.len[pr.s_id], T_Len = A.R.len[pr.t_id];
-    const int32_t Start = nd.z, Offset = nd.x, Len = nd.y;
-    const int32_t S_Left_Begin = Start - 1, T_Left_Begin = Offset - 1;
-    const int32_t S_Right_Begin = Start + Len, S_Right_Len = S_Len - S_Right_Begin;
-    const int32_t T_Right_Begin = Offset + Len, T_Right_Len = T_Len - T_Right_Begin;
     const uint64_t gnode = pr.hb + tk.node;
-    const int32_t Total_Olap = (Start < Offset ? Start : Offset) + Len + (S_Right_Len < T_Right_Len ? S_Right_Len : T_Right_Len);
-    const int32_t limit = A.tb.error_bound[Total_Olap];
-    const SeqV S = read_view(A.R, pr.s_id, pr.dir), T = read_view(A.R, pr.t_id, 0);
-    uint32_t cells = 0, rows = 0, slots = 0, wrows = 0, bases = 0;
-    bool rmte = true, overflow = false;
-    int32_t right_e = 0, exit_d = T_Right_Begin - S_Right_Begin;
-    if (!(S_Right_Len == 0 || T_Right_Len == 0)) {
-      SeqV a, b;
-      int32_t m, n;
-      const bool s_short = S_Right_Len <= T_Right_Len;
-      if (s_short) { a = S; a.b0 += S_Right_Begin; m = S_Right_Len; b = T; b.b0 += T_Right_Begin; n = T_Right_Len; }
-      else         { a = T; a.b0 += T_Right_Begin; m = T_Right_Len; b = S; b.b0 += S_Right_Begin; n = S_Right_Len; }
-      const GroupDP f = group_prefix_dp<G, CAP>(a, m, b, n, limit, A.tb, tile[g][0], tile[g][1], seqA[g], seqT[g], g, gl);
-      cells += f.cells;
-      rows += f.rows;
-      slots += f.slots;
-      wrows += f.wrows;
-      bases += f.bases;
-      overflow = f.overflow;
-      rmte = f.mte;
-      right_e = f.e;
-      exit_d += s_short ? f.exit_d : -f.exit_d;
-    }
-    bool lmte = true;
+    const int32_t S_Left_Begin = nd.z - 1, T_Left_Begin = nd.x - 1;
     if (!overflow && rmte && !(S_Left_Begin < 0 || T_Left_Begin < 0)) {
-      const SeqV Src = read_view(A.R, pr.s_id, 1 - pr.dir), Trc = read_view(A.R, pr.t_id, 1);
-      SeqV a, b;
-      int32_t m, n;
-      if (S_Right_Begin <= T_Right_Begin) {
-        a = Src; a.b0 += S_Len - 1 - S_Left_Begin; m = S_Left_Begin + 1;
-        b = Trc; b.b0 += T_Len - 1 - T_Left_Begin; n = T_Left_Begin + 1;
-      } else {
-        a = Trc; a.b0 += T_Len - 1 - T_Left_Begin; m = T_Left_Begin + 1;
-        b = Src; b.b0 += S_Len - 1 - S_Left_Begin; n = S_Left_Begin + 1;
-      }
-      const GroupDP l = group_prefix_dp<G, CAP>(a, m, b, n, limit - right_e, A.tb, tile[g][0], tile[g][1], seqA[g], seqT[g], g, gl);
+      const int32_t S_Len = A.R.len[pr.s_id], T_Len = A.R.len[pr.t_id];
+      const int32_t S_Right_Begin = nd.z + nd.y, T_Right_Begin = nd.x + nd.y;
+      // left tails, read rightwards on the reverse-complement strands
+      const int64_t s0 = (pr.dir ? A.R.b0f[pr.s_id] : A.R.b0r[pr.s_id]) + (S_Len - 1 - S_Left_Begin);
+      const int64_t t0 = A.R.b0r[pr.t_id] + (T_Len - 1 - T_Left_Begin);
+      const bool sN = A.R.hasN[pr.s_id], tN = A.R.hasN[pr.t_id];
+      const bool s_first = S_Right_Begin <= T_Right_Begin;   // prefixEditDistance-extend.C:112-132
+      const GroupDP l = group_prefix_dp<G, CAP>(A.R.words, A.R.nmask, s_first ? s0 : t0, s_first ? sN : tN,
+                                                s_first ? S_Left_Begin + 1 : T_Left_Begin + 1, s_first ? t0 : s0, s_first ? tN : sN,
+                                                s_first ? T_Left_Begin + 1 : S_Left_Begin + 1, limit - right_e, A.tb, tile[g][0],
+                                                tile[g][1], seqA[g], seqT[g], g, gl);
       cells += l.cells;
       rows += l.rows;
       slots += l.slots;
