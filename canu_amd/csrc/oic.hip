@@ -89,6 +89,8 @@ struct oic_ctx {
   // search
   DevBuf g_cnt, g_off, g_pbase, g_searched, g_sdsum, g_phi, g_pval;
   DevBuf l_scr, l_done, l_big, l_cnt;
+  std::vector<uint64_t> skip;           // -k skip k-mers, both strands, sorted, unique
+  DevBuf d_skip, x_tscreen, g_sscreen;
   oic_tuning tune{};
   DevBuf p_pairs, p_tmp, p_nodes, p_recs, p_pout, p_order, p_queue, p_stats, p_nout, p_noff, p_outrec, p_retry;
   DevBuf sc_rowL, sc_rowR, sc_rowOff, sc_rowVal, sc_dbuf, sc_ld, sc_rd;
@@ -250,6 +252,58 @@ extern "C" int oic_dp_tables(double max_erate, double align_noise, int32_t *eb, 
   if (eb) memcpy(eb, T.error_bound.data(), T.error_bound.size() * 4);
   if (ml) memcpy(ml, T.match_limit.data(), (size_t)T.max_errors * 4);
   *max_errors = T.max_errors;
+  return OIC_OK;
+}
+
+// Mark_Skip_Kmers' reading of a -k file (overlapInCore-Build_Hash_Index.C:185-257): each line -- or,
+// after a '>' line, the next one -- is cut at its first blank and must hold exactly k bases; the
+// k-mer and its reverse complement are skipped.
+extern "C" int oic_set_skip_kmers(oic_ctx *c, const char *text, uint64_t len) {
+  if (!c || (!text && len)) return OIC_EINVAL;
+  const uint32_t K = c->P.kmer;
+  std::vector<uint64_t> keys;
+  uint64_t i = 0;
+  int32_t line_no = 0;
+  auto next_line = [&](std::string &out) -> bool {
+    if (i >= len) return false;
+    const uint64_t b = i;
+    while (i < len && text[i] != '\n') i++;
+    out.assign(text + b, i - b);
+    if (i < len) i++;
+    line_no++;
+    return true;
+  };
+  std::string line;
+  while (next_line(line)) {
+    if (!line.empty() && line[0] == '>') {
+      if (!next_line(line)) line.clear();
+    }
+    const size_t cut = line.find_first_of(" \t\r\n");
+    if (cut != std::string::npos) line.resize(cut);
+    if (line.size() != K)
+      return fail(c, OIC_EINVAL, "Short kmer skip kmer '%s' at line %d, expecting length %u got length %zu.", line.c_str(), line_no, K,
+                  line.size());
+    uint64_t f = 0, r = 0;
+    bool ok = true;
+    for (uint32_t j = 0; j < K; j++) {
+      const char ch = (char)tolower((unsigned char)line[j]);
+      const int code = ch == 'a' ? 0 : ch == 'c' ? 1 : ch == 'g' ? 2 : ch == 't' ? 3 : -1;
+      if (code < 0) { ok = false; break; }
+      f |= (uint64_t)code << (2 * j);                    // base j at bits 2j, as the index keys
+      r |= (uint64_t)(3 - code) << (2 * (K - 1 - j));    // reverse complement
+    }
+    if (!ok) return fail(c, OIC_EUNSUPPORTED, "skip k-mer '%s' at line %d holds a base other than ACGT", line.c_str(), line_no);
+    keys.push_back(f);
+    keys.push_back(r);
+  }
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  c->skip = keys;
+  if (!keys.empty()) {
+    ALLOC(c, c->d_skip, keys.size() * 8);
+    HIPCHK(c, hipMemcpy(c->d_skip.p, keys.data(), keys.size() * 8, hipMemcpyHostToDevice));
+  }
+  c->have_index = false;   // takes effect at the next index build
   return OIC_OK;
 }
 
@@ -432,6 +486,21 @@ static int build_table(oic_ctx *c, uint32_t bgn, uint32_t end, std::vector<uint3
     first_t->resize(nb);
     HIPCHK(c, hipMemcpyAsync(first_t->data(), d_first.p, nb * 4, hipMemcpyDeviceToHost, c->st));
   }
+  if (!c->skip.empty()) {
+    ALLOC(c, c->x_tscreen, (uint64_t)nb * 4);
+    HIPCHK(c, hipMemsetAsync(c->x_tscreen.p, 0, (uint64_t)nb * 4, c->st));
+    KmerTable X;
+    X.ukey = c->x_ukey.as<uint64_t>();
+    X.ustart = c->x_ustart.as<uint64_t>();
+    X.occ = c->x_occ.as<uint64_t>();
+    X.dir = c->x_dir.as<uint64_t>();
+    X.nu = nu;
+    X.shift = c->dshift;
+    const uint32_t ns = (uint32_t)c->skip.size();
+    hipLaunchKernelGGL(k_mark_screens, dim3((ns + 255) / 256), dim3(256), 0, c->st, c->d_skip.as<uint64_t>(), ns, X,
+                       c->d_len.as<int32_t>(), bgn, K, c->x_tscreen.as<uint32_t>());
+    HIPCHK(c, hipGetLastError());
+  }
   HIPCHK(c, hipStreamSynchronize(c->st));
   c->nu = nu;
   c->nvalid = nvalid;
@@ -520,6 +589,10 @@ static SearchConst search_const(oic_ctx *c, bool elide) {
   C.K = (int32_t)c->P.kmer;
   C.min_kmers = c->P.minkmers ? c->d_minkmers.as<uint32_t>() : nullptr;
   C.elide = elide ? 1 : 0;
+  C.sscreen = nullptr;
+  C.tscreen = c->skip.empty() ? nullptr : c->x_tscreen.as<uint32_t>();
+  C.s0 = 0;
+  C.hbgn = c->hbgn;
   return C;
 }
 
@@ -715,6 +788,10 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   GA.min_olap = c->P.min_olap;
   GA.searched = c->g_searched.as<uint8_t>();
   GA.tbits = std::max(1, ceil_log2(c->nblock));
+  GA.skip = c->skip.empty() ? nullptr : c->d_skip.as<uint64_t>();
+  GA.nskip = (uint32_t)c->skip.size();
+  ALLOC(c, c->g_sscreen, 2 * (size_t)ns * 4);
+  GA.sscreen = c->g_sscreen.as<uint32_t>();
   const int sbits = std::max(1, ceil_log2(2ull * ns));
   if (GA.tbits + sbits > 64) return fail(c, OIC_EINVAL, "pair key needs %d bits", GA.tbits + sbits);
   if (c->nu == 0) return OIC_OK;
@@ -855,6 +932,10 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   PairKernelArgs A;
   A.R = dev_reads(c);
   A.C = search_const(c, true);
+  if (!c->skip.empty()) {
+    A.C.sscreen = c->g_sscreen.as<uint32_t>();
+    A.C.s0 = s0;
+  }
   A.cache = batched ? c->b_cache.as<uint8_t>() : nullptr;
   A.ncells = batched ? c->b_ncells.as<uint32_t>() : nullptr;
   A.nexit = batched ? c->b_nexit.as<int32_t>() : nullptr;

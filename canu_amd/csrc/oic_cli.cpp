@@ -1,6 +1,6 @@
 // oic_cli.cpp -- overlapInCore's command line on the MI355X engine.
 //
-//   overlapInCore [-partial] [-t T] -k K [--hashbits B] [--hashload L] [--hashdatalen N]
+//   overlapInCore [-partial] [-t T] -k K [-k skip.kmers] [--hashbits B] [--hashload L] [--hashdatalen N]
 //                 [--maxerate E] [--alignnoise A] [--minlength M] [--minkmers] [-m|-u] [-z]
 //                 [-l L] [-h a-b] [-r c-d] [--hpc] -o out.ovb[.WORKING] [-s out.stats] <reads.fasta|.fastq>
 //
@@ -175,7 +175,7 @@ int main(int argc, char **argv) {
   p.kmer = 0;
   p.schedule = OIC_SCHED_LEARNED;
   uint32_t bgn_hash = 1, end_hash = UINT32_MAX, bgn_ref = 1, end_ref = UINT32_MAX;
-  const char *out_name = nullptr, *stats_name = nullptr, *reads_path = nullptr;
+  const char *out_name = nullptr, *stats_name = nullptr, *reads_path = nullptr, *skip_path = nullptr;
   bool hpc = false;
   int err = 0;
   for (int arg = 1; arg < argc; arg++) {
@@ -188,11 +188,11 @@ int main(int argc, char **argv) {
     else if (!strcmp(a, "-h")) decode_range(next(), bgn_hash, end_hash);
     else if (!strcmp(a, "-r")) decode_range(next(), bgn_ref, end_ref);
     else if (!strcmp(a, "-H") || !strcmp(a, "-R")) { next(); die("library ranges need a seqStore; not supported:", a); }
-    else if (!strcmp(a, "-k")) {
+    else if (!strcmp(a, "-k")) {             // a k-mer size, or a skip-k-mer file if one exists by that name
       const char *v = next();
       FILE *f = fopen(v, "r");
-      if (f) { fclose(f); die("a skip-k-mer file is not supported:", v); }
-      p.kmer = (uint32_t)strtoul(v, nullptr, 10);
+      if (f) { fclose(f); skip_path = v; }
+      else p.kmer = (uint32_t)strtoul(v, nullptr, 10);
     } else if (!strcmp(a, "-l")) {
       const long v = strtol(next(), nullptr, 10);
       p.frag_olap_limit = (v < 1) ? UINT64_MAX : (uint64_t)v;
@@ -230,6 +230,16 @@ int main(int argc, char **argv) {
   if (rc != OIC_OK) {
     fprintf(stderr, "overlapInCore: oic_create failed (%d): %s\n", rc, ctx ? oic_last_error(ctx) : "no usable GPU or bad parameters");
     return 1;
+  }
+  if (skip_path) {
+    FILE *F = fopen(skip_path, "rb");
+    if (!F) die("cannot open", skip_path);
+    std::string text;
+    char buf[65536];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof buf, F)) > 0) text.append(buf, n);
+    fclose(F);
+    if ((rc = oic_set_skip_kmers(ctx, text.data(), text.size())) != OIC_OK) die(oic_last_error(ctx));
   }
   const oic_reads rd = {R.bases.data(), R.start.data(), R.len.data(), nreads};
   if ((rc = oic_load_reads(ctx, &rd)) != OIC_OK) die(oic_last_error(ctx));

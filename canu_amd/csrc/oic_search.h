@@ -199,7 +199,38 @@ struct GatherArgs {
   int32_t  min_olap;
   const uint8_t *searched;  // [ns] 1 if this ref read is searched
   int      tbits;           // bits for t_local in the pair key
+  const uint64_t *skip;     // sorted keys of the -k skip k-mers, both strands (nskip = 0: no file)
+  uint32_t nskip;
+  uint32_t *sscreen;        // [2 ns] per (S, orientation): 1 left end screened, 2 right end screened
 };
+
+__device__ __forceinline__ bool skip_has(const uint64_t *__restrict__ skip, uint32_t n, uint64_t key) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (skip[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo < n && skip[lo] == key;
+}
+
+// Mark_Skip_Kmers' screened ends of the hash reads (overlapInCore-Build_Hash_Index.C:98-121,
+// :132-177): every occurrence of a skip k-mer within HOPELESS_MATCH of a read's end flags that end.
+__global__ void k_mark_screens(const uint64_t *__restrict__ skip, uint32_t nskip, KmerTable X, const int32_t *__restrict__ len,
+                               uint32_t hbgn, int32_t k, uint32_t *__restrict__ tscreen) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nskip) return;
+  uint64_t lo, hi;
+  if (!table_find(X, skip[i], &lo, &hi)) return;
+  for (uint64_t q = lo; q < hi; q++) {
+    const uint64_t oc = X.occ[q];
+    const uint32_t tl = (uint32_t)(oc >> 32);
+    const int32_t off = (int32_t)(uint32_t)oc;
+    uint32_t f = 0;
+    if (off < OIC_HOPELESS) f |= 1u;
+    if (len[hbgn + tl] - off - k + 1 < OIC_HOPELESS) f |= 2u;
+    if (f) atomicOr(&tscreen[tl], f);
+  }
+}
 
 // one block per (S, orientation): count hits with S_ID < T_ID for every window; sdsum gets the
 // block's total
@@ -207,12 +238,13 @@ __global__ void __launch_bounds__(256) k_hit_count(GatherArgs A, uint32_t *__res
                                                    unsigned long long *__restrict__ sdsum, uint64_t *__restrict__ phi,
                                                    uint64_t *__restrict__ pval) {
   __shared__ unsigned long long bsum;
-  if (threadIdx.x == 0) bsum = 0;
+  __shared__ uint32_t screen;
+  if (threadIdx.x == 0) { bsum = 0; screen = 0; }
   __syncthreads();
   const uint32_t sd = blockIdx.x;
   const uint32_t si = sd >> 1, dir = sd & 1;
   if (si >= A.ns || !A.searched[si]) {
-    if (threadIdx.x == 0) sdsum[sd] = 0;
+    if (threadIdx.x == 0) { sdsum[sd] = 0; A.sscreen[sd] = 0; }
     return;
   }
   const uint32_t S = A.s0 + si;
@@ -228,7 +260,16 @@ __global__ void __launch_bounds__(256) k_hit_count(GatherArgs A, uint32_t *__res
     if (!(hn && kmer_has_n(A.R.nmask, b0 + o, A.k))) {
       const uint64_t key = kmer_at(A.R.words, b0 + o, A.k);
       uint64_t lo, hi;
-      if (table_find(A.X, key, &lo, &hi)) {
+      if (A.nskip && skip_has(A.skip, A.nskip, key)) {
+        // an Empty hash entry: no hits, and Hash_Find's hi_hits screens the S ends (Find_Overlaps.C:266-310)
+        uint32_t f = 0;
+        if (o == 0) f = 1u;
+        else {
+          if (o < OIC_HOPELESS) f |= 1u;
+          if (L - o - A.k + 1 < OIC_HOPELESS) f |= 2u;
+        }
+        if (f) atomicOr(&screen, f);
+      } else if (table_find(A.X, key, &lo, &hi)) {
         const uint64_t f = (tl_max == 0xffffffffu) ? lo : occ_upper(A.X.occ, lo, hi, tl_max);
         c = (uint32_t)(hi - f);
         if (c) {
@@ -244,7 +285,10 @@ __global__ void __launch_bounds__(256) k_hit_count(GatherArgs A, uint32_t *__res
   }
   atomicAdd(&bsum, mine);
   __syncthreads();
-  if (threadIdx.x == 0) sdsum[sd] = bsum;
+  if (threadIdx.x == 0) {
+    sdsum[sd] = bsum;
+    A.sscreen[sd] = screen;
+  }
 }
 
 // Write the hits in (o asc, T desc, T offset desc) order at the scanned offsets, load-balanced:
@@ -767,6 +811,9 @@ struct SearchConst {
   int32_t  K;
   const uint32_t *min_kmers; // --minkmers threshold by diagonal span, or nullptr
   int32_t  elide;          // skip reverse DP / tracebacks whose results Process_Matches cannot use
+  const uint32_t *sscreen;  // screened ends per (S - s0, orientation) and per hash read (T - hbgn),
+  const uint32_t *tscreen;  // bit 1 left, bit 2 right; nullptr without a skip file
+  uint32_t s0, hbgn;
 };
 
 // Extend_Alignment (prefixEditDistance-extend.C:35-183) for one wave.  Deltas land in sc.ldelta
@@ -1128,12 +1175,14 @@ __global__ void __launch_bounds__(OIC_WAVE) k_process_pairs(PairKernelArgs A) {
     if (C.hopeless && nn == 1 && !C.partial) {
       const int4 nd = nodes[0];
       const int32_t s_head = nd.z, t_head = nd.x;
+      const uint32_t ss = C.sscreen ? C.sscreen[2 * (S_ID - C.s0) + pr.dir] : 0u;
+      const uint32_t ts = C.tscreen ? C.tscreen[T_ID - C.hbgn] : 0u;
       bool hopeless = false;
-      if (s_head <= t_head) { if (s_head > OIC_HOPELESS) hopeless = true; }
-      else                  { if (t_head > OIC_HOPELESS) hopeless = true; }
+      if (s_head <= t_head) { if (s_head > OIC_HOPELESS && !(ss & 1u)) hopeless = true; }
+      else                  { if (t_head > OIC_HOPELESS && !(ts & 1u)) hopeless = true; }
       const int32_t s_tail = S_Len - s_head - nd.y + 1, t_tail = T_Len - t_head - nd.y + 1;
-      if (s_tail <= t_tail) { if (s_tail > OIC_HOPELESS) hopeless = true; }
-      else                  { if (t_tail > OIC_HOPELESS) hopeless = true; }
+      if (s_tail <= t_tail) { if (s_tail > OIC_HOPELESS && !(ss & 2u)) hopeless = true; }
+      else                  { if (t_tail > OIC_HOPELESS && !(ts & 2u)) hopeless = true; }
       if (hopeless) {
         if (lane == 0) {
           po.status = 2;

@@ -27,7 +27,7 @@ EXPORTS = [
     "oic_params_default", "oic_create", "oic_destroy", "oic_last_error", "oic_load_reads", "oic_index_build",
     "oic_search", "oic_run", "oic_get_counters", "oic_extend_batch", "oic_prefix_dp_batch", "oic_tables",
     "oic_dp_tables", "oic_set_tracing", "oic_get_traces", "oic_model_train", "oic_model_get", "oic_model_set",
-    "oic_model_predict", "oic_model_grad", "oic_get_tuning", "oic_set_tuning", "oic_autotune",
+    "oic_model_predict", "oic_model_grad", "oic_get_tuning", "oic_set_tuning", "oic_autotune", "oic_set_skip_kmers",
 ]
 
 NFEAT, NOUT = 12, 2
@@ -125,6 +125,7 @@ def lib():
         L.oic_model_set.argtypes = [vp, vp, vp, vp]
         L.oic_model_predict.argtypes = [vp, vp, ctypes.c_uint64, vp]
         L.oic_model_grad.argtypes = [vp, vp, vp, vp, ctypes.c_uint64, ctypes.c_float, ctypes.POINTER(ctypes.c_float), vp]
+        L.oic_set_skip_kmers.argtypes = [vp, ctypes.c_char_p, ctypes.c_uint64]
         L.oic_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
         L.oic_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
         L.oic_autotune.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Tuning), ctypes.POINTER(TuneTrial),
@@ -240,6 +241,11 @@ class Engine:
         delta = np.zeros((len(tasks), delta_stride), np.int32)
         self._check(lib().oic_prefix_dp_batch(self._ctx, tasks.ctypes.data, len(tasks), res.ctypes.data, delta.ctypes.data, delta_stride))
         return res, delta
+
+    def set_skip_kmers(self, text):
+        """-k <file>: the file's text; its k-mers (both strands) give no seed hits and screen read ends."""
+        b = text.encode() if isinstance(text, str) else bytes(text)
+        self._check(lib().oic_set_skip_kmers(self._ctx, b, len(b)))
 
     def tuning(self):
         t = Tuning()

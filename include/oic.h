@@ -161,6 +161,12 @@ int         oic_create(const oic_params *p, oic_ctx **out);
 void        oic_destroy(oic_ctx *ctx);
 const char *oic_last_error(const oic_ctx *ctx);
 
+/* -k <file>: the text of a skip-k-mer file, read as Mark_Skip_Kmers does
+ * (overlapInCore-Build_Hash_Index.C:185-257).  Each k-mer and its reverse complement yield no seed
+ * hits, and mark read ends within HOPELESS_MATCH of them as screened for the hopeless check.
+ * Applies from the next index build.  Malformed line: OIC_EINVAL with the reference's message. */
+int oic_set_skip_kmers(oic_ctx *ctx, const char *text, uint64_t len);
+
 /* Upload and 2-bit pack reads 1..nreads (both strands) into device memory. */
 int oic_load_reads(oic_ctx *ctx, const oic_reads *reads);
 

@@ -724,6 +724,8 @@ static void init_codes(void) {
 
 typedef struct oic_index {
   uint32_t  bgnID, endID;           /* hash reads in this block (IDs) */
+  kmap      empty;                  /* k-mers whose hash entries are Empty: Hash_Find reports hi_hits */
+  uint8_t  *tscreen;                /* per hash read: 1 lfrag_end_screened, 2 rfrag_end_screened */
   uint32_t  nstr;                   /* String_Ct */
   char     *lower;                  /* lowercased hash reads, NUL separated: basesData */
   int64_t  *strStart;               /* String_Start (-1 = not loaded) */
@@ -810,12 +812,53 @@ static uint32_t build_index(const oo_params *P, const char *bases, const uint64_
   }
   free(fill);
   (void)nkmers;
+
+  /* Mark_Skip_Kmers / Hash_Mark_Empty / Mark_Screened_Ends_Chain (Build_Hash_Index.C:98-257): each
+   * skip k-mer and then its reverse complement is marked Empty.  One present in the block marks
+   * the ends of the hash reads holding it within HOPELESS_MATCH of an end as screened (first time
+   * only); an absent one is added as an Empty entry when the hopeless check is on. */
+  X->tscreen = (uint8_t *)calloc(X->nstr ? X->nstr : 1, 1);
+  kmap_init(&X->empty, 1024);
+  for (uint64_t q = 0; q < P->nSkip; q++) {
+    char km[64], rc[64];
+    memcpy(km, P->skipKmers + q * k, k);
+    for (uint32_t i = 0; i < k; i++) {
+      const char c = (char)tolower((unsigned char)km[i]);
+      rc[k - 1 - i] = (c == 'a') ? 't' : (c == 'c') ? 'g' : (c == 'g') ? 'c' : (c == 't') ? 'a' : c;
+    }
+    for (int pass = 0; pass < 2; pass++) {
+      const char *s = pass ? rc : km;
+      uint64_t key = 0;
+      int ok = 1;
+      for (uint32_t i = 0; i < k; i++) {
+        const int c = code_of[(unsigned char)s[i]];
+        if (c < 0) { ok = 0; break; }
+        key = (key << 2) | (uint64_t)c;
+      }
+      if (!ok) continue;
+      const int64_t h = kmap_find(&X->map, key);
+      if (h >= 0) {
+        if (kmap_find(&X->empty, key) < 0) {
+          const uint64_t f = X->map.first[h], n = X->map.cnt[h];
+          for (uint64_t o = 0; o < n; o++) {
+            const uint32_t tl = (uint32_t)(X->occ[f + o] >> 32), off = (uint32_t)X->occ[f + o];
+            if (off < HOPELESS_MATCH) X->tscreen[tl] |= 1;
+            if ((int64_t)X->strLen[tl] - off - k + 1 < HOPELESS_MATCH) X->tscreen[tl] |= 2;
+          }
+        }
+        int nw; kmap_insert(&X->empty, key, &nw);
+      } else if (P->hopeless) {
+        int nw; kmap_insert(&X->empty, key, &nw);
+      }
+    }
+  }
   return X->endID;
 }
 
 static void free_index(oic_index *X) {
-  free(X->strStart); free(X->strLen); free(X->lower); free(X->occ);
+  free(X->strStart); free(X->strLen); free(X->lower); free(X->occ); free(X->tscreen);
   kmap_free(&X->map);
+  kmap_free(&X->empty);
   memset(X, 0, sizeof(*X));
 }
 
@@ -1284,14 +1327,14 @@ static void Process_Matches(const search_ctx *C, int32_t *Start, const char *S, 
     if (s_head <= t_head) {
       if (s_head > HOPELESS_MATCH && !WA->left_end_screened) is_hopeless = 1;
     } else {
-      if (t_head > HOPELESS_MATCH /* && ! t_info.lfrag_end_screened: false for loaded reads */) is_hopeless = 1;
+      if (t_head > HOPELESS_MATCH && !(C->X->tscreen[T_ID - C->X->bgnID] & 1)) is_hopeless = 1;
     }
     s_tail = S_Len - s_head - WA->mn[*Start].Len + 1;
     t_tail = t_len - t_head - WA->mn[*Start].Len + 1;
     if (s_tail <= t_tail) {
       if (s_tail > HOPELESS_MATCH && !WA->right_end_screened) is_hopeless = 1;
     } else {
-      if (t_tail > HOPELESS_MATCH) is_hopeless = 1;
+      if (t_tail > HOPELESS_MATCH && !(C->X->tscreen[T_ID - C->X->bgnID] & 2)) is_hopeless = 1;
     }
     if (is_hopeless) {
       *Start = 0;
@@ -1491,6 +1534,14 @@ static void Find_Overlaps(const search_ctx *C, const char *Frag, int Frag_Len, u
     key = ((key << 2) | (uint64_t)c) & kmask;
     if (++valid < k) continue;
     int Offset = i + 1 - (int)k;
+    if (X->empty.n && kmap_find(&X->empty, key) >= 0) {    /* Hash_Find's hi_hits (Find_Overlaps.C:274-308) */
+      if (Offset == 0) WA->left_end_screened = 1;
+      else {
+        if (Offset < HOPELESS_MATCH) WA->left_end_screened = 1;
+        if (Frag_Len - Offset - (int)k + 1 < HOPELESS_MATCH) WA->right_end_screened = 1;
+      }
+      continue;
+    }
     int64_t h = kmap_find(&X->map, key);
     if (h < 0) continue;
     uint64_t f = X->map.first[h], n = X->map.cnt[h];

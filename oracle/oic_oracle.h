@@ -91,6 +91,9 @@ typedef struct oo_params {
   uint32_t  bgnRef, endRef;    /* -r */
   uint32_t  nThreads;          /* -t: also defines the chunking quirk of OverlapDriver */
   uint32_t  ompThreads;        /* actual OpenMP threads used (0 = nThreads) */
+  const char *skipKmers;       /* -k <file>: nSkip k-mers of kmer ACGT letters each, back to back, in
+                                  file order (parsed as Mark_Skip_Kmers does); NULL = no file */
+  uint64_t  nSkip;
 } oo_params;
 
 void oo_params_default(oo_params *P);

@@ -23,6 +23,7 @@ class Params(ctypes.Structure):
         ("bgnHash", ctypes.c_uint32), ("endHash", ctypes.c_uint32),
         ("bgnRef", ctypes.c_uint32), ("endRef", ctypes.c_uint32),
         ("nThreads", ctypes.c_uint32), ("ompThreads", ctypes.c_uint32),
+        ("skipKmers", ctypes.c_char_p), ("nSkip", ctypes.c_uint64),
     ]
 
 
@@ -226,11 +227,41 @@ def default_params(**kw):
     return p
 
 
+def parse_skip_file(text, kmer):
+    """The k-mers of a -k skip file, as Mark_Skip_Kmers reads them (overlapInCore-Build_Hash_Index.C:185-257):
+    each line, or the line after a '>' header, cut at its first blank; every k-mer must be kmer long."""
+    lines = text.split("\n")
+    out, i = [], 0
+    while i < len(lines):
+        line = lines[i]
+        if i == len(lines) - 1 and line == "":
+            break
+        if line.startswith(">"):
+            i += 1
+            line = lines[i] if i < len(lines) else ""
+        cut = len(line)
+        for j, ch in enumerate(line):
+            if ch in " \t\r":
+                cut = j
+                break
+        km = line[:cut]
+        if len(km) != kmer:
+            raise ValueError(f"skip k-mer '{km}' at line {i + 1} has length {len(km)}, expecting {kmer}")
+        out.append(km.lower())
+        i += 1
+    return out
+
+
 def run(reads, collect_trace=False, **kw):
     """Run the whole overlapInCore restatement on a 1-based read set (canu_amd.readsim.Reads).
 
     Returns dict(records=structured array, stats=[8], counters...)."""
+    skip = kw.pop("skip", None)
     p = default_params(**kw)
+    if skip:
+        buf = b"".join(x.encode() if isinstance(x, str) else bytes(x) for x in skip)
+        p.skipKmers = buf
+        p.nSkip = len(skip)
     res = Result()
     bases = np.ascontiguousarray(reads.bases)
     start = np.ascontiguousarray(reads.start, np.uint64)

@@ -292,6 +292,41 @@ def test_list_builder_hands_back_crowded_offsets():
     assert res.counters["dp_cells"] == ref["cells_used"]
 
 
+def top_kmers(R, n, seed=0):
+    """The n most frequent k-mers of a read set (ties by text), half written reverse-complemented."""
+    comp = str.maketrans("ACGTacgt", "TGCAtgca")
+    counts = {}
+    for i in range(1, R.n + 1):
+        s = R.read(i).decode().upper()
+        for j in range(len(s) - 22 + 1):
+            w = s[j:j + 22]
+            counts[w] = counts.get(w, 0) + 1
+    top = sorted(counts, key=lambda w: (-counts[w], w))[:n]
+    return [w if j % 2 else w.translate(comp)[::-1] for j, w in enumerate(top)]
+
+
+@pytest.mark.parametrize("fixture", ["c1s", "c3s"])
+@pytest.mark.parametrize("hopeless", [True, False])
+def test_skip_kmer_file(request, fixture, hopeless):
+    """-k <file> (Mark_Skip_Kmers): skipped k-mers give no hits and screen read ends for the
+    hopeless check; the file is parsed by the engine, the oracle gets the same k-mers."""
+    R = request.getfixturevalue(fixture)
+    km = top_kmers(R, 400)
+    text = "".join(f">{i}\n{w} 99\n" if i % 3 == 0 else f"{w.lower()}\t{i}\n" for i, w in enumerate(km))
+    assert oracle.parse_skip_file(text, 22) == [w.lower() for w in km]
+    eng = oic.Engine(kmer=22, maxerate=0.045, minlength=500, threads=4, hopeless=hopeless, schedule=oic.SCHED_BATCHED)
+    eng.set_skip_kmers(text)
+    eng.load_reads(R)
+    res = eng.run()
+    ref = oracle.run(R, kmer=22, maxErate=0.045, minOlap=500, nThreads=4, ompThreads=16, hopeless=int(hopeless), skip=km)
+    base = oracle.run(R, kmer=22, maxErate=0.045, minOlap=500, nThreads=4, ompThreads=16, hopeless=int(hopeless))
+    assert ref["hits"] < base["hits"]
+    assert_same(res, ref)
+    assert res.counters["dp_cells"] == ref["cells_used"]
+    with pytest.raises(oic.OicError):
+        eng.set_skip_kmers("ACGTACGT\n")
+
+
 def test_run_reads_with_n_and_short_reads():
     rng = np.random.default_rng(9)
     R = readsim.simulate(readsim.PRESETS["c1"].scaled(60_000, seed=44))

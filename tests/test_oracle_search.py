@@ -200,3 +200,41 @@ def test_thread_chunking_skips_chunk_at_end():
     assert E not in set(a["trace"]["s_id"].tolist())
     sb = set(b["trace"]["s_id"].tolist())
     assert max(sb) <= E and (E in sb or not any((R.pos[E] < R.pos[t] + R.length[t]) and (R.pos[t] < R.pos[E] + R.length[E]) for t in range(E + 1, R.n + 1)))
+
+
+def test_skip_file_parsing():
+    text = "ACGTACGTACGTACGTACGTAC 17\n>header\nttttttttttttttttttttta\tx\nGGGGGGGGGGGGGGGGGGGGGG\n"
+    assert oracle.parse_skip_file(text, K) == ["acgtacgtacgtacgtacgtac", "ttttttttttttttttttttta",
+                                                "gggggggggggggggggggggg"]
+    with pytest.raises(ValueError):
+        oracle.parse_skip_file("ACGT\n", K)
+
+
+def test_skip_kmers_remove_hits_both_strands(small):
+    """-k <file>: a skipped k-mer (or its reverse complement) yields no seed hits
+    (Mark_Skip_Kmers, Build_Hash_Index.C:185-257; Hash_Find's Empty entries, Find_Overlaps.C:177-222)."""
+    R, _ = small
+    # skip the most frequent k-mers of the read set, some given as reverse complements
+    counts = {}
+    for i in range(1, R.n + 1):
+        for w in kmer_positions(R.read(i).upper()):
+            counts[w] = counts.get(w, 0) + 1
+    top = sorted(counts, key=lambda w: (-counts[w], w))[:300]
+    skip = [w.decode() if j % 2 else rc(w).decode() for j, w in enumerate(top)]
+    out = oracle.run(R, collect_trace=True, kmer=K, maxErate=0.045, minOlap=500, nThreads=1, skip=skip)
+    banned = set(top) | {rc(w) for w in top}
+    bp = brute_pairs(R)
+    want = {}
+    for key, hits in bp.items():
+        s, t, dr = key
+        S = R.read(s).upper()
+        S = rc(S) if dr else S
+        kept = [(o, toff) for o, toff in hits if S[o:o + K] not in banned]
+        if kept:
+            want[key] = kept
+    got = {(int(x["s_id"]), int(x["t_id"]), int(x["dir"])): x for x in out["trace"]}
+    assert set(got) == set(want)
+    for key, hits in want.items():
+        assert got[key]["n_hits"] == len(hits), key
+        assert got[key]["n_nodes"] == len(runs(hits)), key
+    assert out["hits"] < small[1]["hits"]
