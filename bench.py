@@ -113,6 +113,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--fwd-lanes", type=int, default=0, help="fix the packed forward kernel's lanes per DP (skips autotuning)")
     ap.add_argument("--fwd-tile", type=int, default=0, help="fix its band tile (with --fwd-lanes)")
+    ap.add_argument("--fwd-counters", action="store_true", help="count rows, lane slots and slide iterations of the packed DPs")
     ap.add_argument("--tune-refs", type=int, default=200, help="ref reads the autotuner times each launch shape on")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-genome", type=int, default=300_000)
@@ -166,6 +167,8 @@ def main():
     timed_end = first + (args.warmup + args.steps) * args.batch - 1
     if args.fwd_lanes:
         eng.set_tuning(fwd_lanes=args.fwd_lanes, fwd_tile=args.fwd_tile or 128)
+    if args.fwd_counters:
+        eng.set_tuning(fwd_counters=1)
     tuning = eng.tuning()
     if not args.no_autotune and not args.fwd_lanes and args.schedule != "plain":
         tb = min(R.n - 1, timed_end + 1)

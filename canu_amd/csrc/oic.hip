@@ -60,7 +60,7 @@ int ceil_log2(uint64_t x) {
 }
 
 constexpr int kCapSmall = 512;
-constexpr uint64_t kHitCap = 600000000ull;   // seed hits per search pass (about 50 GB of pass buffers)
+constexpr uint64_t kPassBytesPerHit = 140;   // device bytes a search pass holds per seed hit (hits, sort, lists, nodes, cache)
 constexpr int kCapLarge = 8192;
 
 }  // namespace
@@ -92,6 +92,7 @@ struct oic_ctx {
   std::vector<uint64_t> skip;           // -k skip k-mers, both strands, sorted, unique
   DevBuf d_skip, x_tscreen, g_sscreen;
   oic_tuning tune{};
+  uint64_t hit_cap = 0;                 // seed hits per search pass
   DevBuf p_pairs, p_tmp, p_nodes, p_recs, p_pout, p_order, p_queue, p_stats, p_nout, p_noff, p_outrec, p_retry;
   DevBuf sc_rowL, sc_rowR, sc_rowOff, sc_rowVal, sc_dbuf, sc_ld, sc_rd;
   DevBuf b_cache, b_ncells, b_nexit, b_probe, b_spec, b_cnt, b_off, b_stats, b_order, b_ovf, b_novf;
@@ -191,6 +192,12 @@ extern "C" int oic_create(const oic_params *p, oic_ctx **out) {
   if (hipGetDeviceProperties(&prop, p->device) != hipSuccess) { delete c; return OIC_EHIP; }
   c->ncu = prop.multiProcessorCount;
   c->tune.list_wave_min_hits = 2048;
+  {
+    // seed hits per search pass: 45 % of the device's memory for pass buffers (about 9e8 on a 288 GB MI355X)
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) tot = (size_t)64 << 30;
+    c->hit_cap = std::min<uint64_t>(1200000000ull, std::max<uint64_t>(50000000ull, (uint64_t)(0.45 * (double)tot) / kPassBytesPerHit));
+  }
   c->tune.fwd_lanes = 16;
   c->tune.fwd_tile = 256;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; return OIC_EHIP; }
@@ -647,7 +654,7 @@ static int predict_pairs(oic_ctx *c, uint64_t np, std::vector<float> &pred) {
   return OIC_OK;
 }
 
-template <int G, int CAP>
+template <int G, int CAP, bool STATS>
 static int run_fwd(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
   if (n == 0) return OIC_OK;
   FA.ntasks = n;
@@ -655,7 +662,7 @@ static int run_fwd(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
   HIPCHK(c, hipMemsetAsync(FA.stats, 0, 64, c->st));
   const uint32_t waves = (uint32_t)std::min<uint64_t>((uint64_t)c->ncu * 16, (n + (OIC_WAVE / G) - 1) / (OIC_WAVE / G));
   HIPCHK(c, hipEventRecord(c->ev0, c->st));
-  hipLaunchKernelGGL((k_fwd_tasks<G, CAP>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
+  hipLaunchKernelGGL((k_fwd_tasks<G, CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->st));
   HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -669,6 +676,8 @@ static int run_fwd(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
   c->ctr.spec_lane_slots += st[2];
   c->ctr.spec_wave_rows += st[3];
   c->ctr.spec_bases += st[4];
+  c->ctr.spec_slide_iters += st[5];
+  c->ctr.spec_wave_slide_iters += st[6];
   if (FA.ovf) {   // the packed narrow-band kernel
     c->ctr.fwd_launches++;
     c->ctr.fwd_cells += st[0];
@@ -681,7 +690,8 @@ static int run_fwd(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
 // packed narrow-band pass, then the tasks whose band outgrew its tile, one per wavefront
 static int run_fwd_narrow(oic_ctx *c, const FwdArgs &FA, uint64_t n, double *ms) {
   const uint32_t g = c->tune.fwd_lanes, t = c->tune.fwd_tile;
-#define FWD_CASE(G_, T_) if (g == G_ && t == T_) return run_fwd<G_, T_>(c, FA, n, ms);
+#define FWD_CASE(G_, T_) \
+  if (g == G_ && t == T_) return c->tune.fwd_counters ? run_fwd<G_, T_, true>(c, FA, n, ms) : run_fwd<G_, T_, false>(c, FA, n, ms);
   FWD_CASE(4, 128) FWD_CASE(4, 256)
   FWD_CASE(8, 128) FWD_CASE(8, 256) FWD_CASE(8, 512)
   FWD_CASE(16, 128) FWD_CASE(16, 256) FWD_CASE(16, 512)
@@ -700,7 +710,7 @@ static int run_fwd_binned(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
   c->ctr.overflow_retries += nov;
   FA.tasks = FA.ovf;
   FA.ovf = nullptr;
-  return run_fwd<64, 8192>(c, FA, nov, ms);
+  return c->tune.fwd_counters ? run_fwd<64, 8192, true>(c, FA, nov, ms) : run_fwd<64, 8192, false>(c, FA, nov, ms);
 }
 
 // probe pass over every pair's longest node, then every other node of the pairs whose probe
@@ -804,7 +814,8 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   ALLOC(c, c->s_scan, scan_scratch_words(P) * 8);
   uint64_t H = 0;
   HIPCHK(c, scan_excl<uint32_t>(c->g_cnt.as<uint32_t>(), c->g_off.as<uint64_t>(), P, c->s_scan.as<uint64_t>(), c->st, &H));
-  if (H > kHitCap && ns > 1) {
+  const uint64_t hit_cap = c->hit_cap;
+  if (H > hit_cap && ns > 1) {
     // too many seed hits for one pass (repeat reads): split at read boundaries
     std::vector<uint64_t> sds(2 * (size_t)ns);
     HIPCHK(c, hipMemcpy(sds.data(), c->g_sdsum.p, sds.size() * 8, hipMemcpyDeviceToHost));
@@ -812,7 +823,7 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     uint64_t acc = 0;
     for (uint32_t i = 0; i < ns; i++) {
       const uint64_t h = sds[2 * i] + sds[2 * i + 1];
-      if (i > a && acc + h > kHitCap) {
+      if (i > a && acc + h > hit_cap) {
         std::vector<uint8_t> sub(searched.begin() + a, searched.begin() + i);
         int rc = search_batch(c, s0 + a, i - a, sub, sink);
         if (rc) return rc;

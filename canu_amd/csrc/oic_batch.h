@@ -103,20 +103,22 @@ struct GroupDP {
   uint32_t slots;     // lane-cells issued (G per band chunk)
   uint32_t wrows;     // row iterations the whole wavefront executed (counted on its first active lane)
   uint32_t bases;     // this lane's bases compared: slide lengths plus one terminating compare per cell
+  uint32_t siters;    // this lane's 16-base window compares in slides
+  uint32_t wsiters;   // slide-loop iterations the whole wavefront executed (counted on its first active lane)
 };
 
 // One prefix DP (forward() row loop, no row storage) on a G-lane group; sa/st are the group's LDS
 // copies of the first OIC_SEQ_WIN words of a and b.
 // a / b: start (in bases) of the two tails in the packed reads, and whether their reads hold N;
 // words / nmask are the packed reads and N mask of the whole read set.
-template <int G, int CAP>
+template <int G, int CAP, bool STATS>
 __device__ GroupDP group_prefix_dp(const uint32_t *__restrict__ words, const uint32_t *__restrict__ nmask, const int64_t a_b0,
                                    const bool nA, const int32_t m, const int64_t b_b0, const bool nB, const int32_t n,
                                    const int32_t limit, const DPTables &tb, int32_t *pbuf, int32_t *cbuf, uint32_t *sa, uint32_t *sb,
                                    int g, int gl) {
   constexpr int32_t MASK = CAP - 1;
   GroupDP r;
-  r.mte = false; r.overflow = false; r.e = 0; r.exit_d = 0; r.cells = 1; r.rows = 0; r.slots = 0; r.wrows = 0; r.bases = 0;
+  r.mte = false; r.overflow = false; r.e = 0; r.exit_d = 0; r.cells = 1; r.rows = 0; r.slots = 0; r.wrows = 0; r.bases = 0; r.siters = 0; r.wsiters = 0;
   const Staged ta = make_staged(a_b0, sa), tb_ = make_staged(b_b0, sb);
   for (int i = gl; i < OIC_SEQ_WIN; i += G) {
     sa[i] = words[ta.w0 + i];
@@ -154,6 +156,10 @@ __device__ GroupDP group_prefix_dp(const uint32_t *__restrict__ words, const uin
     int32_t rem = r1 < r2 ? r1 : r2;
     int32_t adv = 0;
     while (rem > 0) {
+      if (STATS) {
+        r.siters++;
+        if (lane_id() == __builtin_ctzll(__ballot(1))) r.wsiters++;
+      }
       uint32_t x = win16_staged(words, ta, Row + adv) ^ win16_staged(words, tb_, Row + d + adv);
       x = (x | (x >> 1)) & 0x55555555u;
       if (nA) x &= ~nwin16(nmask, a_b0 + Row + adv);
@@ -181,8 +187,10 @@ __device__ GroupDP group_prefix_dp(const uint32_t *__restrict__ words, const uin
     const int32_t eml = tb.match_limit[e];
     Left = (Left - 1 > -e) ? Left - 1 : -e;
     Right = (Right + 1 < e) ? Right + 1 : e;
-    r.rows++;
-    if (lane_id() == __builtin_ctzll(__ballot(1))) r.wrows++;
+    if (STATS) {
+      r.rows++;
+      if (lane_id() == __builtin_ctzll(__ballot(1))) r.wrows++;
+    }
     if (Right - Left + 3 > CAP) { r.overflow = true; return r; }
     wave_fence();
     if (gl < 4) {
@@ -199,7 +207,7 @@ __device__ GroupDP group_prefix_dp(const uint32_t *__restrict__ words, const uin
     for (int32_t c0 = Left; c0 <= Right; c0 += G) {
       const int32_t d = c0 + gl;
       const bool act = d <= Right;
-      r.slots += G;
+      if (STATS) r.slots += G;
       int32_t Row = 0;
       if (act) {
         const int32_t x1 = 1 + pbuf[d & MASK], x2 = pbuf[(d - 1) & MASK], x3 = 1 + pbuf[(d + 1) & MASK];
@@ -242,9 +250,13 @@ __device__ GroupDP group_prefix_dp(const uint32_t *__restrict__ words, const uin
     Left = nL;
     Right = nR;
     group_argmax<G>(cbv, cbd);
-    if (cbv > Longest) { Longest = cbv; Best_d = cbd; Best_e = e; }
-    const double Score = (double)Longest * tb.bmv - (double)e;
-    if (Score > MaxScore) { MaxScore = Score; MSL = Longest; MSd = Best_d; MSe = Best_e; }
+    // Score = Longest * bmv - e only rises when Longest does: with Longest unchanged since row e',
+    // this row's score is at most row e''s, which MaxScore already holds (rounding is monotonic).
+    if (cbv > Longest) {
+      Longest = cbv; Best_d = cbd; Best_e = e;
+      const double Score = (double)Longest * tb.bmv - (double)e;
+      if (Score > MaxScore) { MaxScore = Score; MSL = Longest; MSd = Best_d; MSe = Best_e; }
+    }
     int32_t *t = pbuf; pbuf = cbuf; cbuf = t;
   }
   r.mte = false; r.e = MSe; r.exit_d = MSd;
@@ -253,7 +265,7 @@ __device__ GroupDP group_prefix_dp(const uint32_t *__restrict__ words, const uin
 
 // Extend_Alignment's outcome for one node without deltas: forward on the right tails, then (only
 // if it reached the end) reverse on the left tails with the remaining error budget.
-template <int G, int CAP>
+template <int G, int CAP, bool STATS>
 __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
   constexpr int NG = OIC_WAVE / G;
   __shared__ int32_t tile[NG][2][CAP];
@@ -263,7 +275,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
   const int g = lane / G, gl = lane % G;
   // per-launch totals per lane stay far below 2^32 (about 10^6 per group at the c3 workload)
   uint64_t cells_acc = 0;
-  uint32_t rows_acc = 0, slots_acc = 0, wrows_acc = 0, bases_acc = 0;
+  uint32_t rows_acc = 0, slots_acc = 0, wrows_acc = 0, bases_acc = 0, siters_acc = 0, wsiters_acc = 0;
   uint64_t ti = 0, tend = 0;
 
   while (true) {
@@ -275,7 +287,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
       tend = ti + FWD_CHUNK < A.ntasks ? ti + FWD_CHUNK : A.ntasks;
     }
     if (ti >= A.ntasks) break;               // group-uniform
-    uint32_t cells = 0, rows = 0, slots = 0, wrows = 0, bases = 0;
+    uint32_t cells = 0, rows = 0, slots = 0, wrows = 0, bases = 0, siters = 0, wsiters = 0;
     bool rmte = true, overflow = false, lmte = true;
     int32_t right_e = 0, exit_d = 0, limit = 0;
     {
@@ -293,7 +305,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
         const int64_t s0 = (pr.dir ? A.R.b0r[pr.s_id] : A.R.b0f[pr.s_id]) + S_Right_Begin, t0 = A.R.b0f[pr.t_id] + T_Right_Begin;
         const bool sN = A.R.hasN[pr.s_id], tN = A.R.hasN[pr.t_id];
         const bool s_short = S_Right_Len <= T_Right_Len;   // the shorter tail is A (prefixEditDistance-extend.C:81-99)
-        const GroupDP f = group_prefix_dp<G, CAP>(A.R.words, A.R.nmask, s_short ? s0 : t0, s_short ? sN : tN,
+        const GroupDP f = group_prefix_dp<G, CAP, STATS>(A.R.words, A.R.nmask, s_short ? s0 : t0, s_short ? sN : tN,
                                                   s_short ? S_Right_Len : T_Right_Len, s_short ? t0 : s0, s_short ? tN : sN,
                                                   s_short ? T_Right_Len : S_Right_Len, limit, A.tb, tile[g][0], tile[g][1], seqA[g],
                                                   seqT[g], g, gl);
@@ -302,6 +314,8 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
         slots += f.slots;
         wrows += f.wrows;
         bases += f.bases;
+        siters += f.siters;
+        wsiters += f.wsiters;
         overflow = f.overflow;
         rmte = f.mte;
         right_e = f.e;
@@ -323,7 +337,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
       const int64_t t0 = A.R.b0r[pr.t_id] + (T_Len - 1 - T_Left_Begin);
       const bool sN = A.R.hasN[pr.s_id], tN = A.R.hasN[pr.t_id];
       const bool s_first = S_Right_Begin <= T_Right_Begin;   // prefixEditDistance-extend.C:112-132
-      const GroupDP l = group_prefix_dp<G, CAP>(A.R.words, A.R.nmask, s_first ? s0 : t0, s_first ? sN : tN,
+      const GroupDP l = group_prefix_dp<G, CAP, STATS>(A.R.words, A.R.nmask, s_first ? s0 : t0, s_first ? sN : tN,
                                                 s_first ? S_Left_Begin + 1 : T_Left_Begin + 1, s_first ? t0 : s0, s_first ? tN : sN,
                                                 s_first ? T_Left_Begin + 1 : S_Left_Begin + 1, limit - right_e, A.tb, tile[g][0],
                                                 tile[g][1], seqA[g], seqT[g], g, gl);
@@ -332,6 +346,8 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
       slots += l.slots;
       wrows += l.wrows;
       bases += l.bases;
+      siters += l.siters;
+      wsiters += l.wsiters;
       overflow = l.overflow;
       lmte = l.mte;
     }
@@ -347,19 +363,26 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
     }
     wrows_acc += wrows;
     bases_acc += bases;
+    siters_acc += siters;
+    wsiters_acc += wsiters;
     ti++;
   }
-  uint64_t rows64 = rows_acc, slots64 = slots_acc, wrows64 = wrows_acc, bases64 = bases_acc;
+  uint64_t rows64 = rows_acc, slots64 = slots_acc, wrows64 = wrows_acc, bases64 = bases_acc, si64 = siters_acc,
+           wsi64 = wsiters_acc;
   for (int o = 32; o > 0; o >>= 1) {
     cells_acc += __shfl_xor(cells_acc, o);
     rows64 += __shfl_xor(rows64, o);
     slots64 += __shfl_xor(slots64, o);
     wrows64 += __shfl_xor(wrows64, o);
     bases64 += __shfl_xor(bases64, o);
+    si64 += __shfl_xor(si64, o);
+    wsi64 += __shfl_xor(wsi64, o);
   }
   if (lane == 0 && A.stats) {
     atomicAdd(&A.stats[3], (unsigned long long)wrows64);
     atomicAdd(&A.stats[4], (unsigned long long)bases64);
+    atomicAdd(&A.stats[5], (unsigned long long)si64);
+    atomicAdd(&A.stats[6], (unsigned long long)wsi64);
     atomicAdd(&A.stats[0], (unsigned long long)cells_acc);
     atomicAdd(&A.stats[1], (unsigned long long)rows64);
     atomicAdd(&A.stats[2], (unsigned long long)slots64);

@@ -114,11 +114,13 @@ typedef struct oic_counters {
   uint64_t overflow_retries;   /* DP tasks re-run with a wider band tile */
   uint64_t spec_tasks;         /* forward DPs run ahead of Process_Matches (batched schedules) */
   uint64_t spec_cells;         /* cells of those runs, used or not */
-  uint64_t spec_rows;          /* DP rows (e >= 1) of those runs */
-  uint64_t spec_lane_slots;    /* lane-cells issued by those runs */
+  uint64_t spec_rows;          /* DP rows (e >= 1) of those runs (with tuning.fwd_counters) */
+  uint64_t spec_lane_slots;    /* lane-cells issued by those runs (with tuning.fwd_counters) */
   uint64_t spec_wave_rows;     /* row iterations wavefronts executed for them (a wavefront waits for
-                                  its slowest packed DP) */
+                                  its slowest packed DP) (with tuning.fwd_counters) */
   uint64_t spec_bases;         /* bases compared by those runs */
+  uint64_t spec_slide_iters;   /* 16-base window compares of their slides, per lane (with tuning.fwd_counters) */
+  uint64_t spec_wave_slide_iters; /* slide iterations wavefronts executed (with tuning.fwd_counters) */
   uint64_t fwd_launches;       /* launches of the packed narrow-band forward kernel */
   uint64_t fwd_cells;          /* cells and bases of those launches */
   uint64_t fwd_bases;
@@ -139,6 +141,9 @@ typedef struct oic_tuning {
                                    per wavefront; default 16) */
   uint32_t fwd_tile;            /* band cells a packed forward DP holds in LDS: 128, 256 or 512;
                                    wider bands re-run one per wavefront (default 256) */
+  uint32_t fwd_counters;        /* 1: count rows, lane slots and wavefront rows of the packed
+                                   forward DPs (spec_rows, spec_lane_slots, spec_wave_rows); costs
+                                   time, off by default */
 } oic_tuning;
 
 int oic_get_tuning(const oic_ctx *ctx, oic_tuning *out);
