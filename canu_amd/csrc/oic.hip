@@ -107,6 +107,7 @@ struct oic_ctx {
   oic_counters ctr{};
   bool have_model = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evf[8] = {};
 };
 
 static int fail(oic_ctx *c, int code, const char *fmt, ...) {
@@ -203,6 +204,8 @@ extern "C" int oic_create(const oic_params *p, oic_ctx **out) {
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; return OIC_EHIP; }
   hipEventCreate(&c->ev0);
   hipEventCreate(&c->ev1);
+  for (auto &e : c->evf) hipEventCreate(&e);
+  c->tune.fwd_graph = 1;
   c->T.build(p->max_erate, p->align_noise);
   // overlapInCore.C:400-401
   if (c->T.maxErate > 0.06) c->P.hopeless = 0;
@@ -240,6 +243,8 @@ extern "C" void oic_destroy(oic_ctx *c) {
   if (c->st) hipStreamSynchronize(c->st);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  for (auto &e : c->evf)
+    if (e) hipEventDestroy(e);
   if (c->st) hipStreamDestroy(c->st);
   delete c;
 }
@@ -324,6 +329,7 @@ extern "C" int oic_set_tuning(oic_ctx *c, const oic_tuning *t) {
   if (!c || !t) return OIC_EINVAL;
   const bool lanes_ok = t->fwd_lanes == 4 || t->fwd_lanes == 8 || t->fwd_lanes == 16 || t->fwd_lanes == 32;
   const bool tile_ok = t->fwd_tile == 128 || t->fwd_tile == 256 || (t->fwd_tile == 512 && t->fwd_lanes != 4);
+  if (t->fwd_order > 1) return fail(c, OIC_EINVAL, "fwd_order must be 0 or 1");
   if (!lanes_ok || !tile_ok)
     return fail(c, OIC_EINVAL, "fwd_lanes must be 4, 8, 16 or 32 and fwd_tile 128, 256 or 512 (not 512 with 4 lanes)");
   c->tune = *t;
@@ -736,6 +742,7 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
   FA.ncells = c->b_ncells.as<uint32_t>();
   FA.nexit = c->b_nexit.as<int32_t>();
   FA.stats = c->b_stats.as<unsigned long long>();
+  FA.ntasks_dev = nullptr;
   double ms = 0;
   ALLOC(c, c->b_ovf, np * sizeof(FwdTask));
   ALLOC(c, c->b_novf, 16);
@@ -754,12 +761,171 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
     hipLaunchKernelGGL(k_spec_write, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
                        c->p_order.as<uint32_t>(), c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), c->b_spec.as<FwdTask>());
     FA.tasks = c->b_spec.as<FwdTask>();
+    if (c->tune.fwd_order == 1) {
+      // the hit buffers are free once the match lists exist
+      if (ensure_sort_temp(c, nt) != OIC_OK) return OIC_ENOMEM;
+      ALLOC(c, c->s_k1, nt * 8);
+      ALLOC(c, c->s_v1, nt * 8);
+      uint64_t *tk = c->s_k1.as<uint64_t>(), *tv = c->s_v1.as<uint64_t>();
+      hipLaunchKernelGGL(k_task_keys, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->st, c->b_spec.as<FwdTask>(), nt,
+                         c->p_pairs.as<PairRec>(), c->p_nodes.as<int4>(), tk, tv);
+      RadixTemp tmp{c->s_k2.as<uint64_t>(), c->s_v2.as<uint64_t>(), c->s_hist.as<uint32_t>(), c->s_offs.as<uint64_t>(), c->s_scan.as<uint64_t>()};
+      HIPCHK(c, radix_sort(&tk, &tv, nt, 16, tmp, c->st));
+      FA.tasks = (const FwdTask *)tv;
+    }
     ALLOC(c, c->b_ovf, nt * sizeof(FwdTask));
     FA.ovf = c->b_ovf.as<FwdTask>();
     rc = run_fwd_binned(c, FA, nt, &ms);
     if (rc) return rc;
   }
   c->ctr.spec_tasks += np + nt;
+  c->ctr.ms_fwd_kernel += ms;
+  c->ctr.ms_dp_kernel += ms;
+  return OIC_OK;
+}
+
+// ------------------------------------------------------------ forward passes as one hipGraph
+
+// e0 / e1 (recorded only outside graph capture) time the launch
+template <int G, int CAP, bool STATS>
+static void enqueue_fwd(oic_ctx *c, const FwdArgs &FA, uint64_t max_tasks, hipEvent_t e0, hipEvent_t e1) {
+  const uint32_t waves = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->ncu * 16, (max_tasks + (OIC_WAVE / G) - 1) / (OIC_WAVE / G)));
+  if (e0) hipEventRecord(e0, c->st);
+  hipLaunchKernelGGL((k_fwd_tasks<G, CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
+  if (e1) hipEventRecord(e1, c->st);
+}
+
+static int enqueue_fwd_narrow(oic_ctx *c, const FwdArgs &FA, uint64_t max_tasks, hipEvent_t e0, hipEvent_t e1) {
+  const uint32_t g = c->tune.fwd_lanes, t = c->tune.fwd_tile;
+#define FWD_CASE(G_, T_)                                                                                   \
+  if (g == G_ && t == T_) {                                                                                \
+    if (c->tune.fwd_counters) enqueue_fwd<G_, T_, true>(c, FA, max_tasks, e0, e1);                         \
+    else enqueue_fwd<G_, T_, false>(c, FA, max_tasks, e0, e1);                                             \
+    return OIC_OK;                                                                                         \
+  }
+  FWD_CASE(4, 128) FWD_CASE(4, 256)
+  FWD_CASE(8, 128) FWD_CASE(8, 256) FWD_CASE(8, 512)
+  FWD_CASE(16, 128) FWD_CASE(16, 256) FWD_CASE(16, 512)
+  FWD_CASE(32, 128) FWD_CASE(32, 256) FWD_CASE(32, 512)
+#undef FWD_CASE
+  return fail(c, OIC_EINVAL, "no forward-DP kernel for %u lanes x %u-cell tile", g, t);
+}
+
+// The probe and speculation passes of one search batch, captured into a hipGraph and launched once:
+// task counts stay on the device (the spec count from the scan, overflow counts from the narrow
+// kernels), buffers are sized by the batch's node count, and counters are cleared by a fill
+// kernel rather than memset nodes.  Results are the same as run_forward_passes'.
+static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_t nnodes) {
+  ALLOC(c, c->b_cache, H + 4);
+  ALLOC(c, c->b_ncells, H * 4);
+  ALLOC(c, c->b_nexit, H * 4);
+  ALLOC(c, c->b_probe, np * sizeof(FwdTask));
+  ALLOC(c, c->b_cnt, np * 4);
+  ALLOC(c, c->b_off, np * 8);
+  ALLOC(c, c->b_spec, std::max<uint64_t>(nnodes, 1) * sizeof(FwdTask));
+  ALLOC(c, c->b_ovf, (np + nnodes) * sizeof(FwdTask));
+  ALLOC(c, c->b_stats, 4 * 64);
+  ALLOC(c, c->b_novf, 64);
+  ALLOC(c, c->s_scan, scan_scratch_words(np) * 8);
+  uint32_t *cnt = c->b_novf.as<uint32_t>();   // [0..3] queues, [4] probe overflows, [5] spec tasks, [6] spec overflows
+  unsigned long long *stats = c->b_stats.as<unsigned long long>();
+  const unsigned g = (unsigned)((np + 255) / 256);
+  const bool graph = c->tune.fwd_graph != 0;
+  hipGraph_t gr = nullptr;
+  if (graph) HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  const uint64_t cw = (H + 4) / 4;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((cw + 255) / 256)), dim3(256), 0, c->st, (uint32_t *)c->b_cache.p, cw, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, cnt, (uint64_t)16, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, (uint32_t *)stats, (uint64_t)(4 * 16), 0u);
+  hipLaunchKernelGGL(k_probe_tasks, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->p_nodes.as<int4>(), c->b_probe.as<FwdTask>());
+  FwdArgs FA;
+  FA.R = dev_reads(c);
+  FA.tb = dev_tables(c);
+  FA.pairs = c->p_pairs.as<PairRec>();
+  FA.nodes = c->p_nodes.as<int4>();
+  FA.cache = c->b_cache.as<uint8_t>();
+  FA.ncells = c->b_ncells.as<uint32_t>();
+  FA.nexit = c->b_nexit.as<int32_t>();
+  FwdTask *ovf = c->b_ovf.as<FwdTask>();
+  // probe pass and its wide retry
+  FA.tasks = c->b_probe.as<FwdTask>(); FA.ntasks = np; FA.ntasks_dev = nullptr;
+  FA.queue = cnt + 0; FA.stats = stats; FA.ovf = ovf; FA.novf = cnt + 4;
+  hipEvent_t *ev = graph ? nullptr : c->evf;
+  int rc = enqueue_fwd_narrow(c, FA, np, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
+  if (rc) return rc;
+  FA.tasks = ovf; FA.ntasks = 0; FA.ntasks_dev = cnt + 4;
+  FA.queue = cnt + 1; FA.stats = stats + 8; FA.ovf = nullptr; FA.novf = nullptr;
+  if (c->tune.fwd_counters) enqueue_fwd<64, 8192, true>(c, FA, np, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
+  else enqueue_fwd<64, 8192, false>(c, FA, np, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
+  // speculation: count, scan (total stays on the device), write, run, wide retry
+  hipLaunchKernelGGL(k_spec_count, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
+                     c->b_cache.as<uint8_t>(), c->p_order.as<uint32_t>(), c->b_cnt.as<uint32_t>());
+  HIPCHK(c, scan_excl<uint32_t>(c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), np, c->s_scan.as<uint64_t>(), c->st, nullptr));
+  hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, c->st, c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), np, cnt + 5);
+  hipLaunchKernelGGL(k_spec_write, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
+                     c->p_order.as<uint32_t>(), c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), c->b_spec.as<FwdTask>());
+  FA.tasks = c->b_spec.as<FwdTask>(); FA.ntasks = 0; FA.ntasks_dev = cnt + 5;
+  FA.queue = cnt + 2; FA.stats = stats + 16; FA.ovf = ovf + np; FA.novf = cnt + 6;
+  rc = enqueue_fwd_narrow(c, FA, nnodes, ev ? ev[4] : nullptr, ev ? ev[5] : nullptr);
+  if (rc) return rc;
+  FA.tasks = ovf + np; FA.ntasks = 0; FA.ntasks_dev = cnt + 6;
+  FA.queue = cnt + 3; FA.stats = stats + 24; FA.ovf = nullptr; FA.novf = nullptr;
+  if (c->tune.fwd_counters) enqueue_fwd<64, 8192, true>(c, FA, nnodes, ev ? ev[6] : nullptr, ev ? ev[7] : nullptr);
+  else enqueue_fwd<64, 8192, false>(c, FA, nnodes, ev ? ev[6] : nullptr, ev ? ev[7] : nullptr);
+  HIPCHK(c, hipGetLastError());
+  if (graph) {
+    HIPCHK(c, hipStreamEndCapture(c->st, &gr));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+    if (ie != hipSuccess) { hipGraphDestroy(gr); return fail(c, OIC_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ie)); }
+    hipEventRecord(c->ev0, c->st);
+    const hipError_t le = hipGraphLaunch(ge, c->st);
+    hipEventRecord(c->ev1, c->st);
+    const hipError_t se = hipStreamSynchronize(c->st);
+    hipGraphExecDestroy(ge);
+    hipGraphDestroy(gr);
+    if (le != hipSuccess || se != hipSuccess) return fail(c, OIC_EHIP, "forward-pass graph: %s", hipGetErrorString(le != hipSuccess ? le : se));
+    c->ctr.graphs_launched++;
+  } else {
+    HIPCHK(c, hipStreamSynchronize(c->st));
+  }
+  unsigned long long st[32];
+  uint32_t cn[8];
+  HIPCHK(c, hipMemcpy(st, stats, sizeof st, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(cn, cnt, sizeof cn, hipMemcpyDeviceToHost));
+  // launch times: per kernel when issued directly; in a graph, the whole graph counts as the
+  // narrow launches' time (its fill, count and scan kernels take well under 1 %)
+  float f[4] = {0, 0, 0, 0};
+  const uint64_t nt = cn[5];
+  if (graph) {
+    float gms = 0;
+    HIPCHK(c, hipEventElapsedTime(&gms, c->ev0, c->ev1));
+    const double w0 = (double)st[0], w2 = (double)st[16];
+    f[0] = (w0 + w2 > 0) ? (float)(gms * w0 / (w0 + w2)) : gms;
+    f[2] = gms - f[0];
+  } else {
+    for (int i = 0; i < 4; i++) HIPCHK(c, hipEventElapsedTime(&f[i], c->evf[2 * i], c->evf[2 * i + 1]));
+  }
+  for (int i = 0; i < 4; i++) {
+    const unsigned long long *x = st + 8 * i;
+    c->ctr.spec_cells += x[0];
+    c->ctr.spec_rows += x[1];
+    c->ctr.spec_lane_slots += x[2];
+    c->ctr.spec_wave_rows += x[3];
+    c->ctr.spec_bases += x[4];
+    c->ctr.spec_slide_iters += x[5];
+    c->ctr.spec_wave_slide_iters += x[6];
+  }
+  for (int i : {0, 2}) {   // the packed narrow-band launches
+    if (i == 2 && nt == 0) continue;
+    c->ctr.fwd_launches++;
+    c->ctr.fwd_cells += st[8 * i];
+    c->ctr.fwd_bases += st[8 * i + 4];
+    c->ctr.ms_fwd_narrow += f[i];
+  }
+  c->ctr.overflow_retries += cn[4] + cn[6];
+  c->ctr.spec_tasks += np + nt;
+  const double ms = (double)f[0] + f[1] + f[2] + f[3];
   c->ctr.ms_fwd_kernel += ms;
   c->ctr.ms_dp_kernel += ms;
   return OIC_OK;
@@ -867,6 +1033,7 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   const double t1 = now_ms();
   ALLOC(c, c->p_tmp, H * 16);
   ALLOC(c, c->p_nodes, H * 16);
+  uint64_t batch_nodes = 0;
   ALLOC(c, c->l_scr, H * (8 * 4 + 1));
   {
     ListScratch LS;
@@ -906,9 +1073,8 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->ctr.list_wave_pairs += lc[0];
     c->ctr.list_handed_back += lc[2];
-    uint64_t nn;
-    memcpy(&nn, lc + 4, 8);
-    c->ctr.nodes += nn;
+    memcpy(&batch_nodes, lc + 4, 8);
+    c->ctr.nodes += batch_nodes;
   }
   c->ctr.ms_lists += now_ms() - t1;
 
@@ -936,7 +1102,7 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   // batched schedules: forward DPs of probe and speculative nodes, packed 4 per wavefront
   const bool batched = c->P.schedule != OIC_SCHED_PLAIN && !c->P.partial;
   if (batched) {
-    int rc = run_forward_passes(c, np, H);
+    int rc = (c->tune.fwd_order == 0) ? run_forward_passes_graph(c, np, H, batch_nodes) : run_forward_passes(c, np, H);
     if (rc) return rc;
   }
 
@@ -1147,6 +1313,18 @@ extern "C" int oic_autotune(oic_ctx *c, uint32_t bgn_ref, uint32_t end_ref, oic_
         if (rc) { c->ctr = saved; return rc; }
         if (ms < bms) { bms = ms; b = x; }
       }
+    cur = b;
+  }
+  if (c->P.schedule != OIC_SCHED_PLAIN && !c->P.partial) {
+    double bms = 1e300;
+    oic_tuning b = cur;
+    for (uint32_t o : {0u, 1u}) {
+      oic_tuning x = cur;
+      x.fwd_order = o;
+      rc = trial(x, 0, &ms);
+      if (rc) { c->ctr = saved; return rc; }
+      if (ms < bms) { bms = ms; b = x; }
+    }
     cur = b;
   }
   {

@@ -41,7 +41,18 @@ struct FwdArgs {
   unsigned long long *stats;
   FwdTask *ovf;             // tasks whose band outgrew the tile are appended here (if not null)
   uint32_t *novf;
+  const uint32_t *ntasks_dev;  // if not null, the task count is read here when the kernel starts
 };
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t n, uint32_t v) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// the total of an exclusive scan, for launches that read their size from device memory
+__global__ void k_scan_total(const uint32_t *__restrict__ in, const uint64_t *__restrict__ out, uint64_t n, uint32_t *__restrict__ total) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *total = n ? (uint32_t)(out[n - 1] + in[n - 1]) : 0u;
+}
 
 __device__ __forceinline__ void wave_fence() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -277,6 +288,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
   uint64_t cells_acc = 0;
   uint32_t rows_acc = 0, slots_acc = 0, wrows_acc = 0, bases_acc = 0, siters_acc = 0, wsiters_acc = 0;
   uint64_t ti = 0, tend = 0;
+  const uint64_t ntasks = A.ntasks_dev ? (uint64_t)*A.ntasks_dev : A.ntasks;
 
   while (true) {
     if (ti == tend) {                        // group-uniform: take the next FWD_CHUNK tasks
@@ -284,9 +296,9 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
       wave_fence();
       ti = fetched[g];
       wave_fence();
-      tend = ti + FWD_CHUNK < A.ntasks ? ti + FWD_CHUNK : A.ntasks;
+      tend = ti + FWD_CHUNK < ntasks ? ti + FWD_CHUNK : ntasks;
     }
-    if (ti >= A.ntasks) break;               // group-uniform
+    if (ti >= ntasks) break;                 // group-uniform
     uint32_t cells = 0, rows = 0, slots = 0, wrows = 0, bases = 0, siters = 0, wsiters = 0;
     bool rmte = true, overflow = false, lmte = true;
     int32_t right_e = 0, exit_d = 0, limit = 0;
@@ -435,4 +447,16 @@ __global__ void k_spec_write(const PairRec *__restrict__ pairs, uint64_t np, con
     tasks[w].node = (uint32_t)i;
     w++;
   }
+}
+
+// sort keys of packed DP tasks by the seed length of their node, longest first, so that the
+// DPs sharing a wavefront (which waits for its slowest) tend to be alike; values are the tasks
+__global__ void k_task_keys(const FwdTask *__restrict__ t, uint64_t n, const PairRec *__restrict__ pairs, const int4 *__restrict__ nodes,
+                            uint64_t *__restrict__ keys, uint64_t *__restrict__ vals) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const FwdTask x = t[i];
+  const int32_t len = nodes[pairs[x.pair].hb + x.node].y;
+  keys[i] = 0xFFFFull - (uint64_t)(len < 0xFFFF ? len : 0xFFFF);
+  vals[i] = (uint64_t)x.pair | ((uint64_t)x.node << 32);
 }

@@ -36,7 +36,7 @@ NPARAM = 64 * NFEAT + 64 + 64 * 64 + 64 + NOUT * 64 + NOUT
 
 class Tuning(ctypes.Structure):
     _fields_ = [("list_wave_min_hits", ctypes.c_uint32), ("fwd_lanes", ctypes.c_uint32), ("fwd_tile", ctypes.c_uint32),
-                ("fwd_counters", ctypes.c_uint32)]
+                ("fwd_order", ctypes.c_uint32), ("fwd_graph", ctypes.c_uint32), ("fwd_counters", ctypes.c_uint32)]
 
 
 class TuneTrial(ctypes.Structure):
@@ -79,7 +79,7 @@ class Counters(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("kmer_probes", "hits", "pairs", "nodes", "extensions", "dp_calls",
                                                "dp_cells", "dp_lane_slots", "dp_bases", "overflow_retries",
                                                "spec_tasks", "spec_cells", "spec_rows", "spec_lane_slots", "spec_wave_rows",
-                                               "spec_bases", "spec_slide_iters", "spec_wave_slide_iters", "fwd_launches", "fwd_cells", "fwd_bases", "list_wave_pairs",
+                                               "spec_bases", "spec_slide_iters", "spec_wave_slide_iters", "fwd_launches", "fwd_cells", "fwd_bases", "graphs_launched", "list_wave_pairs",
                                                "list_handed_back")] + \
                [(n, ctypes.c_double) for n in ("ms_index", "ms_gather", "ms_lists", "ms_extend", "ms_total", "ms_dp_kernel",
                                                "ms_fwd_kernel", "ms_fwd_narrow")]

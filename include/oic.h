@@ -124,6 +124,7 @@ typedef struct oic_counters {
   uint64_t fwd_launches;       /* launches of the packed narrow-band forward kernel */
   uint64_t fwd_cells;          /* cells and bases of those launches */
   uint64_t fwd_bases;
+  uint64_t graphs_launched;    /* forward-pass hipGraphs launched */
   uint64_t list_wave_pairs;    /* pairs whose match list one wavefront built */
   uint64_t list_handed_back;   /* of those, pairs with an offset of more hits than the wavefront holds */
   double   ms_index, ms_gather, ms_lists, ms_extend, ms_total;
@@ -141,6 +142,10 @@ typedef struct oic_tuning {
                                    per wavefront; default 16) */
   uint32_t fwd_tile;            /* band cells a packed forward DP holds in LDS: 128, 256 or 512;
                                    wider bands re-run one per wavefront (default 256) */
+  uint32_t fwd_order;          /* speculative DP tasks in pair order (0) or by seed length,
+                                   longest first (1); the autotuner picks */
+  uint32_t fwd_graph;           /* 1 (default): each search batch's probe and speculation passes run
+                                   as one captured hipGraph; 0: the same launches, issued directly */
   uint32_t fwd_counters;        /* 1: count rows, lane slots and wavefront rows of the packed
                                    forward DPs (spec_rows, spec_lane_slots, spec_wave_rows); costs
                                    time, off by default */
@@ -156,8 +161,8 @@ typedef struct oic_tune_trial {
 } oic_tune_trial;
 
 /* Autotuner: searches ref reads [bgn_ref, end_ref] against the current index under each candidate
- * launch shape (forward DP: 4/8/16/32 lanes x 128/256/512-cell tile; list builder threshold
- * 256..16384 hits), keeps the fastest, and reports the trials (up to cap) when trials is not null.
+ * launch shape (forward DP: 4/8/16/32 lanes x 128/256/512-cell tile, then task order; list builder
+ * threshold 256..16384 hits), keeps the fastest, and reports the trials (up to cap) when trials is not null.
  * Counters are left as they were.  No reference counterpart. */
 int oic_autotune(oic_ctx *ctx, uint32_t bgn_ref, uint32_t end_ref, oic_tuning *best, oic_tune_trial *trials,
                  uint32_t cap, uint32_t *ntrials);

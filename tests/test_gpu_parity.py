@@ -261,6 +261,17 @@ def test_forward_launch_shapes_identical(c3s):
             res = eng.run()
             assert_same(res, ref)
             assert res.counters["dp_cells"] == ref["cells_used"], (lanes, tile)
+    eng.set_tuning(fwd_lanes=16, fwd_tile=128, fwd_order=1)
+    res = eng.run()
+    assert_same(res, ref)
+    assert res.counters["dp_cells"] == ref["cells_used"]
+    # the forward passes as one hipGraph per batch (default) and as direct launches
+    for graph in (1, 0):
+        eng.set_tuning(fwd_order=0, fwd_graph=graph)
+        res = eng.run()
+        assert_same(res, ref)
+        assert res.counters["dp_cells"] == ref["cells_used"]
+        assert (res.counters["graphs_launched"] > 0) == bool(graph)
     with pytest.raises(oic.OicError):
         eng.set_tuning(fwd_lanes=64)
 
@@ -271,7 +282,7 @@ def test_autotune_keeps_results(c3s):
     eng.load_reads(c3s)
     eng.build_hash_index(1, c3s.n)
     best, trials = eng.autotune(1, min(c3s.n, 60))
-    assert len(trials) == 1 + 11 + 4
+    assert len(trials) == 1 + 11 + 2 + 4
     assert best["fwd_lanes"] in (4, 8, 16, 32) and best["fwd_tile"] in (128, 256, 512)
     assert eng.tuning() == best
     assert_same(eng.run(), ref)
