@@ -107,7 +107,7 @@ struct oic_ctx {
   oic_counters ctr{};
   bool have_model = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t evf[8] = {};
+  hipEvent_t evf[12] = {};
 };
 
 static int fail(oic_ctx *c, int code, const char *fmt, ...) {
@@ -328,10 +328,11 @@ extern "C" int oic_get_tuning(const oic_ctx *c, oic_tuning *out) {
 extern "C" int oic_set_tuning(oic_ctx *c, const oic_tuning *t) {
   if (!c || !t) return OIC_EINVAL;
   const bool lanes_ok = t->fwd_lanes == 4 || t->fwd_lanes == 8 || t->fwd_lanes == 16 || t->fwd_lanes == 32;
-  const bool tile_ok = t->fwd_tile == 128 || t->fwd_tile == 256 || (t->fwd_tile == 512 && t->fwd_lanes != 4);
+  const bool tile_ok = t->fwd_tile == 128 || t->fwd_tile == 256 || (t->fwd_tile == 512 && t->fwd_lanes != 4) ||
+                       ((t->fwd_tile == 64 || t->fwd_tile == 32) && (t->fwd_lanes == 8 || t->fwd_lanes == 16));
   if (t->fwd_order > 1) return fail(c, OIC_EINVAL, "fwd_order must be 0 or 1");
   if (!lanes_ok || !tile_ok)
-    return fail(c, OIC_EINVAL, "fwd_lanes must be 4, 8, 16 or 32 and fwd_tile 128, 256 or 512 (not 512 with 4 lanes)");
+    return fail(c, OIC_EINVAL, "fwd_lanes must be 4, 8, 16 or 32 and fwd_tile 32 or 64 (8 or 16 lanes), 128, 256 or 512 (not with 4 lanes)");
   c->tune = *t;
   return OIC_OK;
 }
@@ -684,7 +685,7 @@ static int run_fwd(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
   c->ctr.spec_bases += st[4];
   c->ctr.spec_slide_iters += st[5];
   c->ctr.spec_wave_slide_iters += st[6];
-  if (FA.ovf) {   // the packed narrow-band kernel
+  if (G < 64) {   // the packed narrow-band kernel
     c->ctr.fwd_launches++;
     c->ctr.fwd_cells += st[0];
     c->ctr.fwd_bases += st[4];
@@ -699,8 +700,8 @@ static int run_fwd_narrow(oic_ctx *c, const FwdArgs &FA, uint64_t n, double *ms)
 #define FWD_CASE(G_, T_) \
   if (g == G_ && t == T_) return c->tune.fwd_counters ? run_fwd<G_, T_, true>(c, FA, n, ms) : run_fwd<G_, T_, false>(c, FA, n, ms);
   FWD_CASE(4, 128) FWD_CASE(4, 256)
-  FWD_CASE(8, 128) FWD_CASE(8, 256) FWD_CASE(8, 512)
-  FWD_CASE(16, 128) FWD_CASE(16, 256) FWD_CASE(16, 512)
+  FWD_CASE(8, 32) FWD_CASE(8, 64) FWD_CASE(8, 128) FWD_CASE(8, 256) FWD_CASE(8, 512)
+  FWD_CASE(16, 32) FWD_CASE(16, 64) FWD_CASE(16, 128) FWD_CASE(16, 256) FWD_CASE(16, 512)
   FWD_CASE(32, 128) FWD_CASE(32, 256) FWD_CASE(32, 512)
 #undef FWD_CASE
   return fail(c, OIC_EINVAL, "no forward-DP kernel for %u lanes x %u-cell tile", g, t);
@@ -714,6 +715,14 @@ static int run_fwd_binned(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
   HIPCHK(c, hipMemcpy(&nov, FA.novf, 4, hipMemcpyDeviceToHost));
   if (nov == 0) return OIC_OK;
   c->ctr.overflow_retries += nov;
+  // one per wavefront with a 512-cell tile, then what is wider still with an 8192-cell tile
+  FA.tasks = FA.ovf;
+  FA.ovf = FA.ovf + n;
+  HIPCHK(c, hipMemsetAsync(FA.novf, 0, 4, c->st));
+  rc = c->tune.fwd_counters ? run_fwd<64, 512, true>(c, FA, nov, ms) : run_fwd<64, 512, false>(c, FA, nov, ms);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpy(&nov, FA.novf, 4, hipMemcpyDeviceToHost));
+  if (nov == 0) return OIC_OK;
   FA.tasks = FA.ovf;
   FA.ovf = nullptr;
   return c->tune.fwd_counters ? run_fwd<64, 8192, true>(c, FA, nov, ms) : run_fwd<64, 8192, false>(c, FA, nov, ms);
@@ -744,7 +753,7 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
   FA.stats = c->b_stats.as<unsigned long long>();
   FA.ntasks_dev = nullptr;
   double ms = 0;
-  ALLOC(c, c->b_ovf, np * sizeof(FwdTask));
+  ALLOC(c, c->b_ovf, 2 * np * sizeof(FwdTask));
   ALLOC(c, c->b_novf, 16);
   FA.ovf = c->b_ovf.as<FwdTask>();
   FA.novf = c->b_novf.as<uint32_t>();
@@ -773,7 +782,7 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
       HIPCHK(c, radix_sort(&tk, &tv, nt, 16, tmp, c->st));
       FA.tasks = (const FwdTask *)tv;
     }
-    ALLOC(c, c->b_ovf, nt * sizeof(FwdTask));
+    ALLOC(c, c->b_ovf, 2 * nt * sizeof(FwdTask));
     FA.ovf = c->b_ovf.as<FwdTask>();
     rc = run_fwd_binned(c, FA, nt, &ms);
     if (rc) return rc;
@@ -804,17 +813,19 @@ static int enqueue_fwd_narrow(oic_ctx *c, const FwdArgs &FA, uint64_t max_tasks,
     return OIC_OK;                                                                                         \
   }
   FWD_CASE(4, 128) FWD_CASE(4, 256)
-  FWD_CASE(8, 128) FWD_CASE(8, 256) FWD_CASE(8, 512)
-  FWD_CASE(16, 128) FWD_CASE(16, 256) FWD_CASE(16, 512)
+  FWD_CASE(8, 32) FWD_CASE(8, 64) FWD_CASE(8, 128) FWD_CASE(8, 256) FWD_CASE(8, 512)
+  FWD_CASE(16, 32) FWD_CASE(16, 64) FWD_CASE(16, 128) FWD_CASE(16, 256) FWD_CASE(16, 512)
   FWD_CASE(32, 128) FWD_CASE(32, 256) FWD_CASE(32, 512)
 #undef FWD_CASE
   return fail(c, OIC_EINVAL, "no forward-DP kernel for %u lanes x %u-cell tile", g, t);
 }
 
 // The probe and speculation passes of one search batch, captured into a hipGraph and launched once:
-// task counts stay on the device (the spec count from the scan, overflow counts from the narrow
-// kernels), buffers are sized by the batch's node count, and counters are cleared by a fill
-// kernel rather than memset nodes.  Results are the same as run_forward_passes'.
+// task counts stay on the device (the spec count from the scan, overflow counts from each level),
+// buffers are sized by the batch's node count, and counters are cleared by a fill kernel rather
+// than memset nodes.  Each pass runs in three levels: the packed narrow kernel; the tasks whose
+// band outgrew its tile, one per wavefront with a 512-cell tile; those wider still with an
+// 8192-cell tile.  Results are the same as run_forward_passes'.
 static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_t nnodes) {
   ALLOC(c, c->b_cache, H + 4);
   ALLOC(c, c->b_ncells, H * 4);
@@ -823,20 +834,22 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
   ALLOC(c, c->b_cnt, np * 4);
   ALLOC(c, c->b_off, np * 8);
   ALLOC(c, c->b_spec, std::max<uint64_t>(nnodes, 1) * sizeof(FwdTask));
-  ALLOC(c, c->b_ovf, (np + nnodes) * sizeof(FwdTask));
-  ALLOC(c, c->b_stats, 4 * 64);
+  ALLOC(c, c->b_ovf, 2 * (np + nnodes) * sizeof(FwdTask));
+  ALLOC(c, c->b_stats, 6 * 64);
   ALLOC(c, c->b_novf, 64);
   ALLOC(c, c->s_scan, scan_scratch_words(np) * 8);
-  uint32_t *cnt = c->b_novf.as<uint32_t>();   // [0..3] queues, [4] probe overflows, [5] spec tasks, [6] spec overflows
+  // cnt: [0..5] queues of the six launches, [6] spec tasks, [8..11] overflow counts
+  uint32_t *cnt = c->b_novf.as<uint32_t>();
   unsigned long long *stats = c->b_stats.as<unsigned long long>();
   const unsigned g = (unsigned)((np + 255) / 256);
   const bool graph = c->tune.fwd_graph != 0;
+  hipEvent_t *ev = graph ? nullptr : c->evf;
   hipGraph_t gr = nullptr;
   if (graph) HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
   const uint64_t cw = (H + 4) / 4;
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((cw + 255) / 256)), dim3(256), 0, c->st, (uint32_t *)c->b_cache.p, cw, 0u);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, cnt, (uint64_t)16, 0u);
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, (uint32_t *)stats, (uint64_t)(4 * 16), 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, (uint32_t *)stats, (uint64_t)(6 * 16), 0u);
   hipLaunchKernelGGL(k_probe_tasks, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->p_nodes.as<int4>(), c->b_probe.as<FwdTask>());
   FwdArgs FA;
   FA.R = dev_reads(c);
@@ -847,31 +860,32 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
   FA.ncells = c->b_ncells.as<uint32_t>();
   FA.nexit = c->b_nexit.as<int32_t>();
   FwdTask *ovf = c->b_ovf.as<FwdTask>();
-  // probe pass and its wide retry
-  FA.tasks = c->b_probe.as<FwdTask>(); FA.ntasks = np; FA.ntasks_dev = nullptr;
-  FA.queue = cnt + 0; FA.stats = stats; FA.ovf = ovf; FA.novf = cnt + 4;
-  hipEvent_t *ev = graph ? nullptr : c->evf;
-  int rc = enqueue_fwd_narrow(c, FA, np, ev ? ev[0] : nullptr, ev ? ev[1] : nullptr);
+  // one pass: launch slot k (0 probe, 1 spec) uses queues/stats 3k..3k+2 and overflow counts 8+2k, 9+2k
+  auto pass = [&](int k, const FwdTask *tasks, uint64_t n, const uint32_t *n_dev, uint64_t cap, FwdTask *ov1, FwdTask *ov2) -> int {
+    FA.tasks = tasks; FA.ntasks = n; FA.ntasks_dev = n_dev;
+    FA.queue = cnt + 3 * k; FA.stats = stats + 8 * (3 * k); FA.ovf = ov1; FA.novf = cnt + 8 + 2 * k;
+    int rc = enqueue_fwd_narrow(c, FA, cap, ev ? ev[6 * k] : nullptr, ev ? ev[6 * k + 1] : nullptr);
+    if (rc) return rc;
+    FA.tasks = ov1; FA.ntasks = 0; FA.ntasks_dev = cnt + 8 + 2 * k;
+    FA.queue = cnt + 3 * k + 1; FA.stats = stats + 8 * (3 * k + 1); FA.ovf = ov2; FA.novf = cnt + 9 + 2 * k;
+    if (c->tune.fwd_counters) enqueue_fwd<64, 512, true>(c, FA, cap, ev ? ev[6 * k + 2] : nullptr, ev ? ev[6 * k + 3] : nullptr);
+    else enqueue_fwd<64, 512, false>(c, FA, cap, ev ? ev[6 * k + 2] : nullptr, ev ? ev[6 * k + 3] : nullptr);
+    FA.tasks = ov2; FA.ntasks = 0; FA.ntasks_dev = cnt + 9 + 2 * k;
+    FA.queue = cnt + 3 * k + 2; FA.stats = stats + 8 * (3 * k + 2); FA.ovf = nullptr; FA.novf = nullptr;
+    if (c->tune.fwd_counters) enqueue_fwd<64, 8192, true>(c, FA, cap, ev ? ev[6 * k + 4] : nullptr, ev ? ev[6 * k + 5] : nullptr);
+    else enqueue_fwd<64, 8192, false>(c, FA, cap, ev ? ev[6 * k + 4] : nullptr, ev ? ev[6 * k + 5] : nullptr);
+    return OIC_OK;
+  };
+  int rc = pass(0, c->b_probe.as<FwdTask>(), np, nullptr, np, ovf, ovf + np);
   if (rc) return rc;
-  FA.tasks = ovf; FA.ntasks = 0; FA.ntasks_dev = cnt + 4;
-  FA.queue = cnt + 1; FA.stats = stats + 8; FA.ovf = nullptr; FA.novf = nullptr;
-  if (c->tune.fwd_counters) enqueue_fwd<64, 8192, true>(c, FA, np, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
-  else enqueue_fwd<64, 8192, false>(c, FA, np, ev ? ev[2] : nullptr, ev ? ev[3] : nullptr);
-  // speculation: count, scan (total stays on the device), write, run, wide retry
   hipLaunchKernelGGL(k_spec_count, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
                      c->b_cache.as<uint8_t>(), c->p_order.as<uint32_t>(), c->b_cnt.as<uint32_t>());
   HIPCHK(c, scan_excl<uint32_t>(c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), np, c->s_scan.as<uint64_t>(), c->st, nullptr));
-  hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, c->st, c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), np, cnt + 5);
+  hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, c->st, c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), np, cnt + 6);
   hipLaunchKernelGGL(k_spec_write, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
                      c->p_order.as<uint32_t>(), c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), c->b_spec.as<FwdTask>());
-  FA.tasks = c->b_spec.as<FwdTask>(); FA.ntasks = 0; FA.ntasks_dev = cnt + 5;
-  FA.queue = cnt + 2; FA.stats = stats + 16; FA.ovf = ovf + np; FA.novf = cnt + 6;
-  rc = enqueue_fwd_narrow(c, FA, nnodes, ev ? ev[4] : nullptr, ev ? ev[5] : nullptr);
+  rc = pass(1, c->b_spec.as<FwdTask>(), 0, cnt + 6, nnodes, ovf + 2 * np, ovf + 2 * np + nnodes);
   if (rc) return rc;
-  FA.tasks = ovf + np; FA.ntasks = 0; FA.ntasks_dev = cnt + 6;
-  FA.queue = cnt + 3; FA.stats = stats + 24; FA.ovf = nullptr; FA.novf = nullptr;
-  if (c->tune.fwd_counters) enqueue_fwd<64, 8192, true>(c, FA, nnodes, ev ? ev[6] : nullptr, ev ? ev[7] : nullptr);
-  else enqueue_fwd<64, 8192, false>(c, FA, nnodes, ev ? ev[6] : nullptr, ev ? ev[7] : nullptr);
   HIPCHK(c, hipGetLastError());
   if (graph) {
     HIPCHK(c, hipStreamEndCapture(c->st, &gr));
@@ -889,24 +903,25 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
   } else {
     HIPCHK(c, hipStreamSynchronize(c->st));
   }
-  unsigned long long st[32];
-  uint32_t cn[8];
+  unsigned long long st[48];
+  uint32_t cn[16];
   HIPCHK(c, hipMemcpy(st, stats, sizeof st, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(cn, cnt, sizeof cn, hipMemcpyDeviceToHost));
-  // launch times: per kernel when issued directly; in a graph, the whole graph counts as the
-  // narrow launches' time (its fill, count and scan kernels take well under 1 %)
-  float f[4] = {0, 0, 0, 0};
-  const uint64_t nt = cn[5];
+  const uint64_t nt = cn[6];
+  // launch times: per kernel when issued directly; in a graph, the whole graph counts, shared by
+  // the two narrow launches in proportion to their cells (its fill, count and scan kernels and the
+  // retries of the few wide bands take a few percent at most)
+  float f[6] = {0, 0, 0, 0, 0, 0};
   if (graph) {
     float gms = 0;
     HIPCHK(c, hipEventElapsedTime(&gms, c->ev0, c->ev1));
-    const double w0 = (double)st[0], w2 = (double)st[16];
-    f[0] = (w0 + w2 > 0) ? (float)(gms * w0 / (w0 + w2)) : gms;
-    f[2] = gms - f[0];
+    const double w0 = (double)st[0], w1 = (double)st[24];
+    f[0] = (w0 + w1 > 0) ? (float)(gms * w0 / (w0 + w1)) : gms;
+    f[3] = gms - f[0];
   } else {
-    for (int i = 0; i < 4; i++) HIPCHK(c, hipEventElapsedTime(&f[i], c->evf[2 * i], c->evf[2 * i + 1]));
+    for (int i = 0; i < 6; i++) HIPCHK(c, hipEventElapsedTime(&f[i], c->evf[2 * i], c->evf[2 * i + 1]));
   }
-  for (int i = 0; i < 4; i++) {
+  for (int i = 0; i < 6; i++) {
     const unsigned long long *x = st + 8 * i;
     c->ctr.spec_cells += x[0];
     c->ctr.spec_rows += x[1];
@@ -916,16 +931,17 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
     c->ctr.spec_slide_iters += x[5];
     c->ctr.spec_wave_slide_iters += x[6];
   }
-  for (int i : {0, 2}) {   // the packed narrow-band launches
-    if (i == 2 && nt == 0) continue;
+  for (int i : {0, 3}) {   // the packed narrow-band launches
+    if (i == 3 && nt == 0) continue;
     c->ctr.fwd_launches++;
     c->ctr.fwd_cells += st[8 * i];
     c->ctr.fwd_bases += st[8 * i + 4];
     c->ctr.ms_fwd_narrow += f[i];
   }
-  c->ctr.overflow_retries += cn[4] + cn[6];
+  c->ctr.overflow_retries += cn[8] + cn[10];
   c->ctr.spec_tasks += np + nt;
-  const double ms = (double)f[0] + f[1] + f[2] + f[3];
+  double ms = 0;
+  for (float x : f) ms += x;
   c->ctr.ms_fwd_kernel += ms;
   c->ctr.ms_dp_kernel += ms;
   return OIC_OK;
@@ -1304,8 +1320,8 @@ extern "C" int oic_autotune(oic_ctx *c, uint32_t bgn_ref, uint32_t end_ref, oic_
     double bms = 1e300;
     oic_tuning b = cur;
     for (uint32_t g : {4u, 8u, 16u, 32u})
-      for (uint32_t t : {128u, 256u, 512u}) {
-        if (g == 4 && t == 512) continue;
+      for (uint32_t t : {32u, 64u, 128u, 256u, 512u}) {
+        if ((g == 4 && t == 512) || (t <= 64 && g != 8 && g != 16)) continue;
         oic_tuning x = cur;
         x.fwd_lanes = g;
         x.fwd_tile = t;

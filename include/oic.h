@@ -140,8 +140,9 @@ typedef struct oic_tuning {
                                    with one wavefront, smaller ones with one thread (default 2048) */
   uint32_t fwd_lanes;           /* lanes per packed forward DP: 4, 8, 16 or 32 (64/fwd_lanes DPs
                                    per wavefront; default 16) */
-  uint32_t fwd_tile;            /* band cells a packed forward DP holds in LDS: 128, 256 or 512;
-                                   wider bands re-run one per wavefront (default 256) */
+  uint32_t fwd_tile;            /* band cells a packed forward DP holds in LDS: 32 or 64 (8 or 16 lanes),
+                                   128, 256 or 512; wider bands re-run one per wavefront, with a
+                                   512-cell tile and then an 8192-cell one (default 256) */
   uint32_t fwd_order;          /* speculative DP tasks in pair order (0) or by seed length,
                                    longest first (1); the autotuner picks */
   uint32_t fwd_graph;           /* 1 (default): each search batch's probe and speculation passes run
@@ -161,7 +162,7 @@ typedef struct oic_tune_trial {
 } oic_tune_trial;
 
 /* Autotuner: searches ref reads [bgn_ref, end_ref] against the current index under each candidate
- * launch shape (forward DP: 4/8/16/32 lanes x 128/256/512-cell tile, then task order; list builder
+ * launch shape (forward DP: 4/8/16/32 lanes x 32..512-cell tile, then task order; list builder
  * threshold 256..16384 hits), keeps the fastest, and reports the trials (up to cap) when trials is not null.
  * Counters are left as they were.  No reference counterpart. */
 int oic_autotune(oic_ctx *ctx, uint32_t bgn_ref, uint32_t end_ref, oic_tuning *best, oic_tune_trial *trials,

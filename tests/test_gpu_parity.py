@@ -254,8 +254,8 @@ def test_forward_launch_shapes_identical(c3s):
     eng = oic.Engine(kmer=22, maxerate=0.045, minlength=500, threads=4, schedule=oic.SCHED_BATCHED)
     eng.load_reads(c3s)
     for lanes in (4, 8, 16, 32):
-        for tile in (128, 256, 512):
-            if lanes == 4 and tile == 512:
+        for tile in (32, 64, 128, 256, 512):
+            if (lanes == 4 and tile == 512) or (tile <= 64 and lanes not in (8, 16)):
                 continue
             eng.set_tuning(fwd_lanes=lanes, fwd_tile=tile)
             res = eng.run()
@@ -282,7 +282,7 @@ def test_autotune_keeps_results(c3s):
     eng.load_reads(c3s)
     eng.build_hash_index(1, c3s.n)
     best, trials = eng.autotune(1, min(c3s.n, 60))
-    assert len(trials) == 1 + 11 + 2 + 4
+    assert len(trials) == 1 + 15 + 2 + 4
     assert best["fwd_lanes"] in (4, 8, 16, 32) and best["fwd_tile"] in (128, 256, 512)
     assert eng.tuning() == best
     assert_same(eng.run(), ref)
