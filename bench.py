@@ -187,7 +187,7 @@ def main():
         kname, kms, ops, kbases, nl = "k_process_pairs<512>", d["ms_dp_kernel"], 4 * d["dp_cells"], d["dp_bases"], None
     else:
         # the packed narrow-band forward-DP kernel does almost all of the DP work of the scheduled paths
-        kname = f"k_fwd_tasks<{tuning['fwd_lanes']}, {tuning['fwd_tile']}>"
+        kname = f"k_fwd_tasks<{tuning['fwd_lanes']}, {tuning['fwd_tile']}, {'true' if tuning.get('fwd_counters') else 'false'}>"
         kms, ops, kbases, nl = d["ms_fwd_narrow"], 4 * d["fwd_cells"], d["fwd_bases"], d["fwd_launches"]
     ops += 2 * kbases / 16.0
     achieved = ops / (kms / 1e3) / 1e12 if kms > 0 else 0.0
@@ -207,6 +207,10 @@ def main():
                      "frac": achieved / PEAK_INT_TOPS, "traffic": traffic_from_profiles(kname),
                      "kernel": kname, "kernel_ms_per_step": kms / steps, "int_ops_per_step": ops / steps,
                      "launches_per_step": (nl / steps) if nl else None,
+                     "timing": ("HIP events around each forward-pass hipGraph on its stream; inside it, each packed "
+                                "launch's span from its first wavefront's start to its last one's end (device wall clock)")
+                               if args.schedule != "plain" else "HIP events around each launch on its stream",
+                     "graph_ms_per_step": d.get("ms_fwd_graph", 0.0) / steps,
                      "avg_launch_ms": (kms / nl) if nl else None,
                      "int_ops_per_launch": (ops / nl) if nl else None},
         "counters_per_step": {k: (v / steps) for k, v in d.items()},

@@ -93,6 +93,7 @@ struct oic_ctx {
   DevBuf d_skip, x_tscreen, g_sscreen;
   oic_tuning tune{};
   uint64_t hit_cap = 0;                 // seed hits per search pass
+  int wallclock_khz = 0;                // rate of wall_clock64()
   DevBuf p_pairs, p_tmp, p_nodes, p_recs, p_pout, p_order, p_queue, p_stats, p_nout, p_noff, p_outrec, p_retry;
   DevBuf sc_rowL, sc_rowR, sc_rowOff, sc_rowVal, sc_dbuf, sc_ld, sc_rd;
   DevBuf b_cache, b_ncells, b_nexit, b_probe, b_spec, b_cnt, b_off, b_stats, b_order, b_ovf, b_novf;
@@ -192,6 +193,10 @@ extern "C" int oic_create(const oic_params *p, oic_ctx **out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, p->device) != hipSuccess) { delete c; return OIC_EHIP; }
   c->ncu = prop.multiProcessorCount;
+  {
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, p->device) == hipSuccess) c->wallclock_khz = khz;
+  }
   c->tune.list_wave_min_hits = 2048;
   {
     // seed hits per search pass: 45 % of the device's memory for pass buffers (about 9e8 on a 288 GB MI355X)
@@ -752,6 +757,7 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
   FA.nexit = c->b_nexit.as<int32_t>();
   FA.stats = c->b_stats.as<unsigned long long>();
   FA.ntasks_dev = nullptr;
+  FA.tstamp = nullptr;
   double ms = 0;
   ALLOC(c, c->b_ovf, 2 * np * sizeof(FwdTask));
   ALLOC(c, c->b_novf, 16);
@@ -835,7 +841,7 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
   ALLOC(c, c->b_off, np * 8);
   ALLOC(c, c->b_spec, std::max<uint64_t>(nnodes, 1) * sizeof(FwdTask));
   ALLOC(c, c->b_ovf, 2 * (np + nnodes) * sizeof(FwdTask));
-  ALLOC(c, c->b_stats, 6 * 64);
+  ALLOC(c, c->b_stats, 6 * 64 + 32);
   ALLOC(c, c->b_novf, 64);
   ALLOC(c, c->s_scan, scan_scratch_words(np) * 8);
   // cnt: [0..5] queues of the six launches, [6] spec tasks, [8..11] overflow counts
@@ -849,7 +855,11 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
   const uint64_t cw = (H + 4) / 4;
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((cw + 255) / 256)), dim3(256), 0, c->st, (uint32_t *)c->b_cache.p, cw, 0u);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, cnt, (uint64_t)16, 0u);
+  unsigned long long *tst = stats + 48;      // wall-clock [start, end] of the two packed launches
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, (uint32_t *)stats, (uint64_t)(6 * 16), 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, (uint32_t *)tst, (uint64_t)8, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->st, (uint32_t *)tst, (uint64_t)2, 0xffffffffu);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->st, (uint32_t *)(tst + 2), (uint64_t)2, 0xffffffffu);
   hipLaunchKernelGGL(k_probe_tasks, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->p_nodes.as<int4>(), c->b_probe.as<FwdTask>());
   FwdArgs FA;
   FA.R = dev_reads(c);
@@ -864,8 +874,10 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
   auto pass = [&](int k, const FwdTask *tasks, uint64_t n, const uint32_t *n_dev, uint64_t cap, FwdTask *ov1, FwdTask *ov2) -> int {
     FA.tasks = tasks; FA.ntasks = n; FA.ntasks_dev = n_dev;
     FA.queue = cnt + 3 * k; FA.stats = stats + 8 * (3 * k); FA.ovf = ov1; FA.novf = cnt + 8 + 2 * k;
+    FA.tstamp = tst + 2 * k;
     int rc = enqueue_fwd_narrow(c, FA, cap, ev ? ev[6 * k] : nullptr, ev ? ev[6 * k + 1] : nullptr);
     if (rc) return rc;
+    FA.tstamp = nullptr;
     FA.tasks = ov1; FA.ntasks = 0; FA.ntasks_dev = cnt + 8 + 2 * k;
     FA.queue = cnt + 3 * k + 1; FA.stats = stats + 8 * (3 * k + 1); FA.ovf = ov2; FA.novf = cnt + 9 + 2 * k;
     if (c->tune.fwd_counters) enqueue_fwd<64, 512, true>(c, FA, cap, ev ? ev[6 * k + 2] : nullptr, ev ? ev[6 * k + 3] : nullptr);
@@ -903,7 +915,7 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
   } else {
     HIPCHK(c, hipStreamSynchronize(c->st));
   }
-  unsigned long long st[48];
+  unsigned long long st[52];
   uint32_t cn[16];
   HIPCHK(c, hipMemcpy(st, stats, sizeof st, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(cn, cnt, sizeof cn, hipMemcpyDeviceToHost));
@@ -913,11 +925,19 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
   // retries of the few wide bands take a few percent at most)
   float f[6] = {0, 0, 0, 0, 0, 0};
   if (graph) {
+    // the graph's time (HIP events around its launch); the packed launches' own spans from the
+    // wall-clock stamps of their first and last wavefronts; the rest of the graph is the retries
+    // and the small fill / count / scan kernels
     float gms = 0;
     HIPCHK(c, hipEventElapsedTime(&gms, c->ev0, c->ev1));
-    const double w0 = (double)st[0], w1 = (double)st[24];
-    f[0] = (w0 + w1 > 0) ? (float)(gms * w0 / (w0 + w1)) : gms;
-    f[3] = gms - f[0];
+    double narrow = 0;
+    for (int k = 0; k < 2; k++) {
+      const unsigned long long a = st[48 + 2 * k], b = st[49 + 2 * k];
+      f[3 * k] = (b > a && c->wallclock_khz > 0) ? (float)((double)(b - a) / (double)c->wallclock_khz) : 0.0f;
+      narrow += f[3 * k];
+    }
+    f[1] = (float)std::max(0.0, (double)gms - narrow);
+    c->ctr.ms_fwd_graph += gms;
   } else {
     for (int i = 0; i < 6; i++) HIPCHK(c, hipEventElapsedTime(&f[i], c->evf[2 * i], c->evf[2 * i + 1]));
   }

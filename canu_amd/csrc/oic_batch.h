@@ -42,6 +42,7 @@ struct FwdArgs {
   FwdTask *ovf;             // tasks whose band outgrew the tile are appended here (if not null)
   uint32_t *novf;
   const uint32_t *ntasks_dev;  // if not null, the task count is read here when the kernel starts
+  unsigned long long *tstamp;  // if not null: [0] earliest wavefront start, [1] latest end (wall clock)
 };
 
 __global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t n, uint32_t v) {
@@ -289,6 +290,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
   uint32_t rows_acc = 0, slots_acc = 0, wrows_acc = 0, bases_acc = 0, siters_acc = 0, wsiters_acc = 0;
   uint64_t ti = 0, tend = 0;
   const uint64_t ntasks = A.ntasks_dev ? (uint64_t)*A.ntasks_dev : A.ntasks;
+  if (A.tstamp && lane == 0) atomicMin(&A.tstamp[0], (unsigned long long)wall_clock64());
 
   while (true) {
     if (ti == tend) {                        // group-uniform: take the next FWD_CHUNK tasks
@@ -390,6 +392,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
     si64 += __shfl_xor(si64, o);
     wsi64 += __shfl_xor(wsi64, o);
   }
+  if (A.tstamp && lane == 0) atomicMax(&A.tstamp[1], (unsigned long long)wall_clock64());
   if (lane == 0 && A.stats) {
     atomicAdd(&A.stats[3], (unsigned long long)wrows64);
     atomicAdd(&A.stats[4], (unsigned long long)bases64);

@@ -130,7 +130,10 @@ typedef struct oic_counters {
   double   ms_index, ms_gather, ms_lists, ms_extend, ms_total;
   double   ms_dp_kernel;       /* device time of all DP kernels (HIP events) */
   double   ms_fwd_kernel;      /* of which the packed forward-DP kernels */
-  double   ms_fwd_narrow;      /* of which the packed narrow-band kernel (HIP events on its stream) */
+  double   ms_fwd_narrow;      /* of which the packed narrow-band kernel: HIP events around each launch,
+                                  or, inside a forward-pass graph, the span from its first wavefront's
+                                  start to its last one's end (device wall clock) */
+  double   ms_fwd_graph;       /* forward-pass graphs, HIP events around their launches */
 } oic_counters;
 
 /* Launch-shape knobs.  No setting changes a result; the defaults suit MI355X.  No reference
