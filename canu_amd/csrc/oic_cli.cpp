@@ -2,7 +2,7 @@
 //
 //   overlapInCore [-partial] [-t T] -k K [-k skip.kmers] [--hashbits B] [--hashload L] [--hashdatalen N]
 //                 [--maxerate E] [--alignnoise A] [--minlength M] [--minkmers] [-m|-u] [-z]
-//                 [-l L] [-h a-b] [-r c-d] [--hpc] -o out.ovb[.WORKING] [-s out.stats] <reads.fasta|.fastq>
+//                 [-l L] [-h a-b] [-r c-d] [--hpc] -o out.ovb[.WORKING] [-s out.stats] <seqStore|reads.fasta|.fastq>
 //
 // Options and their defaults follow overlapInCore.C:293-411 (main's argv loop) and
 // overlapInCore.H:367-473 (oicParameters::initialize).  Output, as ovFile(ovFileFullWrite) and
@@ -15,9 +15,12 @@
 //     ".WORKING" and ".ovb", which is how canu's pipeline names it (OverlapInCore.pm:188,197);
 //   - the -s file (stderr if absent): the 8 stats lines (overlapInCore.C:539-560).
 //
-// Reads come from a FASTA or FASTQ file, numbered 1.. in file order, where the reference reads a
-// seqStore.  --hpc homopolymer-compresses them first, as a store with the homopolymerCompression
-// marker hands them over (stores/sqRead.H:211, :454).
+// Reads come from a seqStore directory, in the version and form the store hands over (read by
+// oic_seqstore.h), or from a FASTA or FASTQ file, numbered 1.. in file order.  --hpc
+// homopolymer-compresses FASTA/FASTQ reads, as a store with the homopolymerCompression marker hands
+// them over (stores/sqRead.H:211, :454).
+//
+//   overlapInCore --dump-store <seqStore>   prints the reads a run would see, as FASTA (">id lib len").
 #include <cctype>
 #include <cerrno>
 #include <cinttypes>
@@ -28,6 +31,7 @@
 #include <vector>
 
 #include "../../include/oic.h"
+#include "oic_seqstore.h"
 #include "oic_snappy.h"
 
 static void die(const char *msg, const char *arg = nullptr) {
@@ -106,6 +110,34 @@ static ReadSet load_reads(const char *path, bool hpc) {
   return R;
 }
 
+// every read of a store, 1..last, with length 0 where the store hands over nothing
+static ReadSet load_store(const char *path) {
+  oic_store::Store S;
+  std::string err, seq;
+  if (!S.open(path, err)) die(err.c_str());
+  ReadSet R;
+  for (uint32_t id = 1; id <= S.nreads; id++) {
+    seq.clear();
+    if (S.length(id) > 0) {
+      if (!S.sequence(id, seq, err)) die(err.c_str());
+      if (seq.size() != S.length(id)) die("read length differs from the store's metadata in", path);
+    }
+    add_read(R, seq, false);
+  }
+  if (R.len.size() < 2) die("no reads in", path);
+  return R;
+}
+
+static int dump_store(const char *path) {
+  oic_store::Store S;
+  std::string err;
+  if (!S.open(path, err)) die(err.c_str());
+  ReadSet R = load_store(path);
+  for (uint32_t id = 1; id < R.len.size(); id++)
+    printf(">%u %u %u\n%s\n", id, S.library(id), R.len[id], R.bases.c_str() + R.start[id]);
+  return 0;
+}
+
 // ovFile(ovFileFullWrite): buffers of (bufferSize / (lcm * 4)) * lcm uint32 with lcm = 20 * 24 and a
 // 1 MiB bufferSize (ovStoreFile.C:107-121), each snappy-compressed (:190-207)
 class OvbWriter {
@@ -170,6 +202,7 @@ static int snappy_block(const char *in, const char *out) {
 
 int main(int argc, char **argv) {
   if (argc == 4 && !strcmp(argv[1], "--snappy-block")) return snappy_block(argv[2], argv[3]);
+  if (argc == 3 && !strcmp(argv[1], "--dump-store")) return dump_store(argv[2]);
   oic_params p;
   oic_params_default(&p);
   p.kmer = 0;
@@ -187,7 +220,7 @@ int main(int argc, char **argv) {
     if (!strcmp(a, "-partial")) p.partial = 1;
     else if (!strcmp(a, "-h")) decode_range(next(), bgn_hash, end_hash);
     else if (!strcmp(a, "-r")) decode_range(next(), bgn_ref, end_ref);
-    else if (!strcmp(a, "-H") || !strcmp(a, "-R")) { next(); die("library ranges need a seqStore; not supported:", a); }
+    else if (!strcmp(a, "-H") || !strcmp(a, "-R")) { next(); die("library ranges are not supported:", a); }
     else if (!strcmp(a, "-k")) {             // a k-mer size, or a skip-k-mer file if one exists by that name
       const char *v = next();
       FILE *f = fopen(v, "r");
@@ -219,11 +252,13 @@ int main(int argc, char **argv) {
   if (p.kmer == 0) fprintf(stderr, "* No kmer length supplied; -k needed!\n"), err++;
   if (!out_name) fprintf(stderr, "ERROR:  No output file name specified\n"), err++;
   if (err || !reads_path) {
-    fprintf(stderr, "USAGE:  %s [options] <reads.fasta|reads.fastq>\n", argv[0]);
+    fprintf(stderr, "USAGE:  %s [options] <seqStore|reads.fasta|reads.fastq>\n", argv[0]);
     return 1;
   }
 
-  ReadSet R = load_reads(reads_path, hpc);
+  const bool store = oic_store::is_dir(reads_path);
+  if (store && hpc) die("--hpc applies to FASTA/FASTQ input; a seqStore's homopolymerCompression marker decides:", reads_path);
+  ReadSet R = store ? load_store(reads_path) : load_reads(reads_path, hpc);
   const uint32_t nreads = (uint32_t)(R.len.size() - 1);
   oic_ctx *ctx = nullptr;
   int rc = oic_create(&p, &ctx);

@@ -2,7 +2,8 @@
 
 CPU: the engine's snappy block encoder (overlapInCore --snappy-block) against Google's snappy as
 pyarrow ships it, both ways, and the pure-Python decoder of canu_amd/ovfile.py.
-GPU: the command line end to end -- FASTA in; .ovb, .oc and .stats out -- against the oracle.
+GPU: the command line end to end -- FASTA or seqStore in; .ovb, .oc and .stats out -- against the
+oracle, and store input against FASTA input.
 """
 import os
 import subprocess
@@ -121,3 +122,42 @@ def test_cli_hpc_ranges_and_options(tmp_path):
                      bgnHash=2, endHash=n - 1, bgnRef=3, endRef=n)
     assert len(ref["records"]) > 100
     check_outputs(ovb, oc, stats, Rh, ref)
+
+
+def write_fasta_strings(path, reads):
+    with open(path, "w") as f:
+        for i, s in enumerate(reads):
+            f.write(f">r{i + 1}\n{s}\n")
+
+
+@pytest.mark.gpu
+def test_cli_reads_seqstore(tmp_path):
+    """A seqStore and a FASTA of the same reads give identical .ovb records, .oc and .stats; ignored
+    reads match empty FASTA records, and the store's homopolymerCompression marker matches --hpc."""
+    from canu_amd import seqstore
+    from oracle import oracle
+    R = readsim.simulate(readsim.PRESETS["c1"].scaled(150_000, seed=53))
+    reads = [R.read(i).decode() for i in range(1, R.n + 1)]
+    ignore = {5, 17, R.n}
+    fasta = tmp_path / "reads.fasta"
+    write_fasta_strings(fasta, ["" if i + 1 in ignore else s for i, s in enumerate(reads)])
+    for marker in (False, True):
+        outs = []
+        for kind in ("store", "fasta"):
+            d = tmp_path / f"{kind}{int(marker)}"
+            d.mkdir()
+            src = fasta
+            extra = ["--hpc"] if (marker and kind == "fasta") else []
+            if kind == "store":
+                src = d / "asm.seqStore"
+                seqstore.write_store(src, raw=reads, hpc_marker=marker, ignore=ignore)
+            print(f"cli {kind} marker={marker}", flush=True)
+            outs.append(run_cli(d, src, "--maxerate", "0.045", "--minlength", "500", *extra))
+        (ovb_s, oc_s, st_s), (ovb_f, oc_f, st_f) = outs
+        a, b = oracle.sort_records(ovfile.read_ovb(ovb_s)), oracle.sort_records(ovfile.read_ovb(ovb_f))
+        assert len(a) > 300 and len(a) == len(b)
+        for f in ("a_iid", "b_iid", "w0", "w1"):
+            assert np.array_equal(a[f], b[f]), f
+        assert not np.isin(a["a_iid"], list(ignore)).any() and not np.isin(a["b_iid"], list(ignore)).any()
+        assert oc_s.read_bytes() == oc_f.read_bytes()
+        assert st_s.read_bytes() == st_f.read_bytes()
