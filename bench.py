@@ -113,6 +113,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--fwd-lanes", type=int, default=0, help="fix the packed forward kernel's lanes per DP (skips autotuning)")
     ap.add_argument("--fwd-tile", type=int, default=0, help="fix its band tile (with --fwd-lanes)")
+    ap.add_argument("--fwd-loop", type=int, default=-1, help="fix its loop form (0 row-stepped, 1 per task; with --fwd-lanes)")
     ap.add_argument("--fwd-counters", action="store_true", help="count rows, lane slots and slide iterations of the packed DPs")
     ap.add_argument("--tune-refs", type=int, default=200, help="ref reads the autotuner times each launch shape on")
     ap.add_argument("--no-cpu", action="store_true")
@@ -167,6 +168,8 @@ def main():
     timed_end = first + (args.warmup + args.steps) * args.batch - 1
     if args.fwd_lanes:
         eng.set_tuning(fwd_lanes=args.fwd_lanes, fwd_tile=args.fwd_tile or 128)
+        if args.fwd_loop >= 0:
+            eng.set_tuning(fwd_loop=args.fwd_loop)
     if args.fwd_counters:
         eng.set_tuning(fwd_counters=1)
     tuning = eng.tuning()
@@ -187,7 +190,8 @@ def main():
         kname, kms, ops, kbases, nl = "k_process_pairs<512>", d["ms_dp_kernel"], 4 * d["dp_cells"], d["dp_bases"], None
     else:
         # the packed narrow-band forward-DP kernel does almost all of the DP work of the scheduled paths
-        kname = f"k_fwd_tasks<{tuning['fwd_lanes']}, {tuning['fwd_tile']}, {'true' if tuning.get('fwd_counters') else 'false'}>"
+        kform = "k_fwd_tasks" if tuning.get("fwd_loop") else "k_fwd_sm"
+        kname = f"{kform}<{tuning['fwd_lanes']}, {tuning['fwd_tile']}, {'true' if tuning.get('fwd_counters') else 'false'}>"
         kms, ops, kbases, nl = d["ms_fwd_narrow"], 4 * d["fwd_cells"], d["fwd_bases"], d["fwd_launches"]
     ops += 2 * kbases / 16.0
     achieved = ops / (kms / 1e3) / 1e12 if kms > 0 else 0.0

@@ -336,6 +336,7 @@ extern "C" int oic_set_tuning(oic_ctx *c, const oic_tuning *t) {
   const bool tile_ok = t->fwd_tile == 128 || t->fwd_tile == 256 || (t->fwd_tile == 512 && t->fwd_lanes != 4) ||
                        ((t->fwd_tile == 64 || t->fwd_tile == 32) && (t->fwd_lanes == 8 || t->fwd_lanes == 16));
   if (t->fwd_order > 1) return fail(c, OIC_EINVAL, "fwd_order must be 0 or 1");
+  if (t->fwd_loop > 1) return fail(c, OIC_EINVAL, "fwd_loop must be 0 or 1");
   if (!lanes_ok || !tile_ok)
     return fail(c, OIC_EINVAL, "fwd_lanes must be 4, 8, 16 or 32 and fwd_tile 32 or 64 (8 or 16 lanes), 128, 256 or 512 (not with 4 lanes)");
   c->tune = *t;
@@ -666,6 +667,13 @@ static int predict_pairs(oic_ctx *c, uint64_t np, std::vector<float> &pred) {
   return OIC_OK;
 }
 
+// the packed kernel in the loop form the tuning selects (one DP per wavefront: the per-task loop)
+template <int G, int CAP, bool STATS>
+static void launch_fwd(oic_ctx *c, const FwdArgs &FA, uint32_t waves) {
+  if (G < 64 && c->tune.fwd_loop == 0) hipLaunchKernelGGL((k_fwd_sm<G, CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
+  else hipLaunchKernelGGL((k_fwd_tasks<G, CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
+}
+
 template <int G, int CAP, bool STATS>
 static int run_fwd(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
   if (n == 0) return OIC_OK;
@@ -674,7 +682,7 @@ static int run_fwd(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
   HIPCHK(c, hipMemsetAsync(FA.stats, 0, 64, c->st));
   const uint32_t waves = (uint32_t)std::min<uint64_t>((uint64_t)c->ncu * 16, (n + (OIC_WAVE / G) - 1) / (OIC_WAVE / G));
   HIPCHK(c, hipEventRecord(c->ev0, c->st));
-  hipLaunchKernelGGL((k_fwd_tasks<G, CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
+  launch_fwd<G, CAP, STATS>(c, FA, waves);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->st));
   HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -806,7 +814,7 @@ template <int G, int CAP, bool STATS>
 static void enqueue_fwd(oic_ctx *c, const FwdArgs &FA, uint64_t max_tasks, hipEvent_t e0, hipEvent_t e1) {
   const uint32_t waves = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->ncu * 16, (max_tasks + (OIC_WAVE / G) - 1) / (OIC_WAVE / G)));
   if (e0) hipEventRecord(e0, c->st);
-  hipLaunchKernelGGL((k_fwd_tasks<G, CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
+  launch_fwd<G, CAP, STATS>(c, FA, waves);
   if (e1) hipEventRecord(e1, c->st);
 }
 
@@ -1305,9 +1313,9 @@ extern "C" int oic_run(oic_ctx *c, uint32_t bgn_hash, uint32_t end_hash, uint32_
 // ------------------------------------------------------------------------------------ autotuner
 
 // Times the search of a sample of ref reads under each launch shape and keeps the fastest: the
-// packed forward-DP kernel over lanes per DP x band tile (device time of that kernel), then the
-// list-builder threshold (device + host time of the list stage).  Results never depend on the
-// shape, so the sample's records are discarded.
+// packed forward-DP kernel over lanes per DP x band tile (device time of that kernel), then its loop
+// form and task order, then the list-builder threshold (device + host time of the list stage).
+// Results never depend on the shape, so the sample's records are discarded.
 extern "C" int oic_autotune(oic_ctx *c, uint32_t bgn_ref, uint32_t end_ref, oic_tuning *best, oic_tune_trial *trials,
                             uint32_t cap, uint32_t *ntrials) {
   if (!c) return OIC_EINVAL;
@@ -1349,6 +1357,18 @@ extern "C" int oic_autotune(oic_ctx *c, uint32_t bgn_ref, uint32_t end_ref, oic_
         if (rc) { c->ctr = saved; return rc; }
         if (ms < bms) { bms = ms; b = x; }
       }
+    cur = b;
+  }
+  if (c->P.schedule != OIC_SCHED_PLAIN && !c->P.partial) {
+    double bms = 1e300;
+    oic_tuning b = cur;
+    for (uint32_t l : {0u, 1u}) {
+      oic_tuning x = cur;
+      x.fwd_loop = l;
+      rc = trial(x, 0, &ms);
+      if (rc) { c->ctr = saved; return rc; }
+      if (ms < bms) { bms = ms; b = x; }
+    }
     cur = b;
   }
   if (c->P.schedule != OIC_SCHED_PLAIN && !c->P.partial) {

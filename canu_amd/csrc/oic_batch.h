@@ -404,6 +404,338 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
   }
 }
 
+// ---------------------------------------------------------------- state-machine form
+//
+// The same tasks and results as k_fwd_tasks, with the wavefront's loop turned inside out: each
+// iteration advances every group by one DP row, and a group whose DP ends takes its next DP (the
+// reverse one, or the next task) before the following row.  In k_fwd_tasks a group that finishes
+// waits at the reconvergence point for the slowest DP of its wavefront.
+
+// one group's DP in progress (every lane of the group holds the same values)
+struct SMDP {
+  int64_t a_b0, b_b0;
+  Staged ta, tb;
+  int32_t m, n, lim, e, Left, Right, Longest, MSL, MSd, MSe;
+  double MaxScore;
+  bool nA, nB;
+};
+
+// a DP's outcome, and the work it took (accumulated over a task's forward and reverse DPs)
+struct SMRes {
+  bool mte, overflow;
+  int32_t e, d;
+  uint32_t cells, rows, slots, wrows, bases, siters, wsiters;
+};
+
+// 16 bases of both tails from positions pa (in a) and pb (in b), compared: the number of leading
+// equal bases (16 if all are)
+__device__ __forceinline__ int32_t sm_match16(const SMDP &s, const uint32_t *__restrict__ words,
+                                              const uint32_t *__restrict__ nmask, int32_t pa, int32_t pb) {
+  uint32_t x = win16_staged(words, s.ta, pa) ^ win16_staged(words, s.tb, pb);
+  x = (x | (x >> 1)) & 0x55555555u;
+  if (s.nA) x &= ~nwin16(nmask, s.a_b0 + pa);
+  if (s.nB) x &= ~nwin16(nmask, s.b_b0 + pb);
+  return x ? (int32_t)(__builtin_ctz(x) >> 1) : 16;
+}
+
+// Stage the tails and run row 0 (the exact-prefix slide, prefixEditDistance-forward.C:113-132).
+// Returns true when the DP is already over (res holds its outcome).
+template <int G>
+__device__ __forceinline__ bool sm_start(SMDP &s, SMRes &res, const uint32_t *__restrict__ words,
+                                         const uint32_t *__restrict__ nmask, int32_t *pbuf, uint32_t *sa, uint32_t *sb, int g,
+                                         int gl) {
+  res.mte = false; res.overflow = false; res.e = 0; res.d = 0;
+  s.ta = make_staged(s.a_b0, sa);
+  s.tb = make_staged(s.b_b0, sb);
+  for (int i = gl; i < OIC_SEQ_WIN; i += G) {
+    sa[i] = words[s.ta.w0 + i];
+    sb[i] = words[s.tb.w0 + i];
+  }
+  wave_fence();
+  int32_t row0 = 0;
+  for (int32_t base = 0;; base += 16 * G) {
+    const int32_t off = base + 16 * gl;
+    const int32_t rem = s.m - off;
+    int32_t k = 0;
+    bool stop = true;
+    if (rem > 0) {
+      const int32_t kk = sm_match16(s, words, nmask, off, off);
+      if (kk >= rem) { k = rem; stop = true; } else { k = kk; stop = kk < 16; }
+    }
+    const uint64_t bl = group_ballot<G>(stop, g);
+    if (bl) {
+      const int f = __builtin_ctzll(bl);
+      row0 = base + 16 * f + __shfl(k, g * G + f);
+      break;
+    }
+  }
+  res.cells += 1;
+  if (row0 == s.m) { res.mte = true; return true; }
+  if (gl == 0) pbuf[0] = row0;
+  s.e = 0; s.Left = 0; s.Right = 0; s.Longest = 0; s.MSL = 0; s.MSd = 0; s.MSe = 0;
+  s.MaxScore = 0.0;
+  return s.lim < 1;
+}
+
+// Row s.e + 1 of the DP: the body of group_prefix_dp's row loop.  Returns true when the DP is over.
+template <int G, int CAP, bool STATS>
+__device__ __forceinline__ bool sm_row(SMDP &s, SMRes &res, const uint32_t *__restrict__ words,
+                                       const uint32_t *__restrict__ nmask, const DPTables &tb, int32_t *&pbuf, int32_t *&cbuf,
+                                       int g, int gl) {
+  constexpr int32_t MASK = CAP - 1;
+  const int32_t e = ++s.e;
+  const int32_t eml = tb.match_limit[e];
+  s.Left = (s.Left - 1 > -e) ? s.Left - 1 : -e;
+  s.Right = (s.Right + 1 < e) ? s.Right + 1 : e;
+  if (STATS) {
+    res.rows++;
+    if (lane_id() == __builtin_ctzll(__ballot(1))) res.wrows++;
+  }
+  if (s.Right - s.Left + 3 > CAP) { res.overflow = true; return true; }
+  wave_fence();
+  if (gl < 4) {
+    const int32_t sd = (gl == 0) ? s.Left : (gl == 1) ? s.Left - 1 : (gl == 2) ? s.Right : s.Right + 1;
+    pbuf[sd & MASK] = -2;
+  }
+  wave_fence();
+  const int32_t Left = s.Left, Right = s.Right;
+  int32_t nL = 0x7fffffff, nR = 0;
+  int32_t cbv = -1, cbd = 0x7fffffff, pbv = -1, pbd = 0x7fffffff;
+  for (int32_t c0 = Left; c0 <= Right; c0 += G) {
+    const int32_t d = c0 + gl;
+    const bool act = d <= Right;
+    if (STATS) res.slots += G;
+    int32_t Row = 0;
+    if (act) {
+      const int32_t x1 = 1 + pbuf[d & MASK], x2 = pbuf[(d - 1) & MASK], x3 = 1 + pbuf[(d + 1) & MASK];
+      Row = x1 > x2 ? x1 : x2;
+      Row = Row > x3 ? Row : x3;
+      // slide along diagonal d while the bases match (prefixEditDistance-forward.C:155-167)
+      const int32_t r1 = s.m - Row, r2 = s.n - Row - d;
+      int32_t rem = r1 < r2 ? r1 : r2;
+      int32_t adv = 0;
+      while (rem > 0) {
+        if (STATS) {
+          res.siters++;
+          if (lane_id() == __builtin_ctzll(__ballot(1))) res.wsiters++;
+        }
+        const int32_t kk = sm_match16(s, words, nmask, Row + adv, Row + d + adv);
+        if (kk >= rem) { adv += rem; break; }
+        adv += kk;
+        if (kk < 16) break;
+        rem -= 16;
+      }
+      res.bases += (uint32_t)adv + 1u;
+      Row += adv;
+      cbuf[d & MASK] = Row;
+    }
+    const bool reach = act && (Row == s.m || Row + d == s.n);
+    const uint64_t bl = group_ballot<G>(reach, g);
+    if (bl) {
+      // the end of A or T was reached: branch-point test (prefixEditDistance-forward.C:172-212)
+      const int f = __builtin_ctzll(bl);
+      res.cells += (uint32_t)(f + 1);
+      const int32_t rrow = __shfl(Row, g * G + f);
+      const double Score = (double)rrow * tb.bmv - (double)e;
+      const int32_t Tail_Len = rrow - s.MSL;
+      const double slope = (double)(s.MaxScore - Score) / (double)Tail_Len;
+      const bool abort_ = ((uint32_t)e > tb.min_branch_end_dist / 2u) && ((uint32_t)Tail_Len >= tb.min_branch_end_dist) &&
+                          (slope >= tb.slope_min);
+      if (abort_) { res.mte = false; res.e = s.MSe; res.d = s.MSd; }
+      else        { res.mte = true;  res.e = e;     res.d = c0 + f; }
+      return true;
+    }
+    res.cells += (uint32_t)((Right - c0 + 1) < G ? (Right - c0 + 1) : G);
+    // pruned range and first-max cell in the same pass (prefixEditDistance-forward.C:245-299)
+    const bool keep = act && (Row + (d > 0 ? d : 0) >= eml);
+    const uint64_t kb = group_ballot<G>(keep, g);
+    if (kb) {
+      const int32_t kl = c0 + (63 - __builtin_clzll(kb));
+      if (nL == 0x7fffffff) nL = c0 + __builtin_ctzll(kb);
+      nR = kl;
+      if (pbv > cbv || (pbv == cbv && pbd < cbd)) { cbv = pbv; cbd = pbd; }
+      pbv = -1;
+      pbd = 0x7fffffff;
+      if (act && d >= nL) {
+        if (d <= kl) {
+          if (Row > cbv || (Row == cbv && d < cbd)) { cbv = Row; cbd = d; }
+        } else {
+          pbv = Row;
+          pbd = d;
+        }
+      }
+    } else if (act && nL != 0x7fffffff) {
+      if (Row > pbv || (Row == pbv && d < pbd)) { pbv = Row; pbd = d; }
+    }
+  }
+  if (nL == 0x7fffffff) { res.mte = false; res.e = s.MSe; res.d = s.MSd; return true; }
+  s.Left = nL;
+  s.Right = nR;
+  group_argmax<G>(cbv, cbd);
+  if (cbv > s.Longest) {
+    s.Longest = cbv;
+    const double Score = (double)cbv * tb.bmv - (double)e;
+    if (Score > s.MaxScore) { s.MaxScore = Score; s.MSL = cbv; s.MSd = cbd; s.MSe = e; }
+  }
+  int32_t *t = pbuf; pbuf = cbuf; cbuf = t;
+  if (e >= s.lim) { res.mte = false; res.e = s.MSe; res.d = s.MSd; return true; }
+  return false;
+}
+
+template <int G, int CAP, bool STATS>
+__global__ void __launch_bounds__(OIC_WAVE) k_fwd_sm(FwdArgs A) {
+  constexpr int NG = OIC_WAVE / G;
+  __shared__ int32_t tile[NG][2][CAP];
+  __shared__ uint32_t seqA[NG][OIC_SEQ_WIN], seqT[NG][OIC_SEQ_WIN];
+  __shared__ uint32_t fetched[NG];
+  const int lane = lane_id();
+  const int g = lane / G, gl = lane % G;
+  int32_t *pbuf = tile[g][0], *cbuf = tile[g][1];
+  uint64_t cells_acc = 0;
+  uint32_t rows_acc = 0, slots_acc = 0, wrows_acc = 0, bases_acc = 0, siters_acc = 0, wsiters_acc = 0;
+  uint64_t ti = 0, tend = 0;
+  const uint64_t ntasks = A.ntasks_dev ? (uint64_t)*A.ntasks_dev : A.ntasks;
+  if (A.tstamp && lane == 0) atomicMin(&A.tstamp[0], (unsigned long long)wall_clock64());
+
+  SMDP s;
+  SMRes res;
+  int state = 0;   // 0: take the next task, 1: forward rows, 2: reverse rows, 3: no tasks left
+  int32_t limit = 0, right_e = 0, exit_d = 0;
+  bool s_short = false, rmte = true, lmte = true;
+
+  // the reverse DP on the left tails, when the forward one reached the end (prefixEditDistance-extend.C:112-132);
+  // returns true when it has rows to run
+  auto start_reverse = [&]() -> bool {
+    const FwdTask tk = A.tasks[ti];
+    const PairRec pr = A.pairs[tk.pair];
+    const int4 nd = A.nodes[pr.hb + tk.node];
+    const int32_t S_Left_Begin = nd.z - 1, T_Left_Begin = nd.x - 1;
+    if (res.overflow || !rmte || S_Left_Begin < 0 || T_Left_Begin < 0) return false;
+    const int32_t S_Len = A.R.len[pr.s_id], T_Len = A.R.len[pr.t_id];
+    const int32_t S_Right_Begin = nd.z + nd.y, T_Right_Begin = nd.x + nd.y;
+    const int64_t s0 = (pr.dir ? A.R.b0f[pr.s_id] : A.R.b0r[pr.s_id]) + (S_Len - 1 - S_Left_Begin);
+    const int64_t t0 = A.R.b0r[pr.t_id] + (T_Len - 1 - T_Left_Begin);
+    const bool sN = A.R.hasN[pr.s_id], tN = A.R.hasN[pr.t_id];
+    const bool s_first = S_Right_Begin <= T_Right_Begin;
+    s.a_b0 = s_first ? s0 : t0;
+    s.nA = s_first ? sN : tN;
+    s.m = s_first ? S_Left_Begin + 1 : T_Left_Begin + 1;
+    s.b_b0 = s_first ? t0 : s0;
+    s.nB = s_first ? tN : sN;
+    s.n = s_first ? T_Left_Begin + 1 : S_Left_Begin + 1;
+    s.lim = limit - right_e;
+    if (!sm_start<G>(s, res, A.R.words, A.R.nmask, pbuf, seqA[g], seqT[g], g, gl)) return true;
+    lmte = res.mte;                          // over at row 0
+    return false;
+  };
+  // record the task's outcome and move on
+  auto finish = [&]() {
+    const FwdTask tk = A.tasks[ti];
+    const PairRec pr = A.pairs[tk.pair];
+    const uint64_t gnode = pr.hb + tk.node;
+    const bool overflow = res.overflow;
+    if (gl == 0) {
+      // 1: not a dovetail (Process_Matches only removes the node), 2: dovetail, 0: not computed
+      A.cache[gnode] = overflow ? 0 : ((rmte && lmte) ? 2 : 1);
+      if (overflow && A.ovf) A.ovf[atomicAdd(A.novf, 1u)] = tk;
+      A.ncells[gnode] = res.cells;
+      A.nexit[gnode] = exit_d;
+      if (!overflow) cells_acc += res.cells;
+      rows_acc += res.rows;
+      slots_acc += res.slots;
+    }
+    wrows_acc += res.wrows;
+    bases_acc += res.bases;
+    siters_acc += res.siters;
+    wsiters_acc += res.wsiters;
+    ti++;
+    state = 0;
+  };
+  // the forward DP is over: take its outcome, then run the reverse DP or finish the task
+  auto forward_done = [&]() {
+    rmte = res.mte;
+    right_e = res.e;
+    exit_d += s_short ? res.d : -res.d;
+    if (start_reverse()) state = 2;
+    else finish();
+  };
+
+  while (true) {
+    while (state == 0) {
+      if (ti == tend) {                      // group-uniform: take the next FWD_CHUNK tasks
+        if (gl == 0) fetched[g] = atomicAdd(A.queue, (uint32_t)FWD_CHUNK);
+        wave_fence();
+        ti = fetched[g];
+        wave_fence();
+        tend = ti + FWD_CHUNK < ntasks ? ti + FWD_CHUNK : ntasks;
+      }
+      if (ti >= ntasks) { state = 3; break; }
+      res.cells = 0; res.rows = 0; res.slots = 0; res.wrows = 0; res.bases = 0; res.siters = 0; res.wsiters = 0;
+      res.mte = true; res.overflow = false; res.e = 0; res.d = 0;
+      lmte = true;
+      const FwdTask tk = A.tasks[ti];
+      const PairRec pr = A.pairs[tk.pair];
+      const int4 nd = A.nodes[pr.hb + tk.node];
+      const int32_t S_Len = A.R.len[pr.s_id], T_Len = A.R.len[pr.t_id];
+      const int32_t Start = nd.z, Offset = nd.x, Len = nd.y;
+      const int32_t S_Right_Begin = Start + Len, S_Right_Len = S_Len - S_Right_Begin;
+      const int32_t T_Right_Begin = Offset + Len, T_Right_Len = T_Len - T_Right_Begin;
+      const int32_t Total_Olap = (Start < Offset ? Start : Offset) + Len + (S_Right_Len < T_Right_Len ? S_Right_Len : T_Right_Len);
+      limit = A.tb.error_bound[Total_Olap];
+      exit_d = T_Right_Begin - S_Right_Begin;
+      s_short = S_Right_Len <= T_Right_Len;   // the shorter tail is A (prefixEditDistance-extend.C:81-99)
+      if (S_Right_Len == 0 || T_Right_Len == 0) {
+        res.d = 0;
+        forward_done();                      // no forward DP: the right end is reached with 0 errors
+        continue;
+      }
+      const int64_t s0 = (pr.dir ? A.R.b0r[pr.s_id] : A.R.b0f[pr.s_id]) + S_Right_Begin, t0 = A.R.b0f[pr.t_id] + T_Right_Begin;
+      const bool sN = A.R.hasN[pr.s_id], tN = A.R.hasN[pr.t_id];
+      s.a_b0 = s_short ? s0 : t0;
+      s.nA = s_short ? sN : tN;
+      s.m = s_short ? S_Right_Len : T_Right_Len;
+      s.b_b0 = s_short ? t0 : s0;
+      s.nB = s_short ? tN : sN;
+      s.n = s_short ? T_Right_Len : S_Right_Len;
+      s.lim = limit;
+      if (!sm_start<G>(s, res, A.R.words, A.R.nmask, pbuf, seqA[g], seqT[g], g, gl)) { state = 1; break; }
+      forward_done();
+    }
+    if (__ballot(state != 3) == 0) break;    // every group has run out of tasks
+    if (state == 1 || state == 2) {
+      if (sm_row<G, CAP, STATS>(s, res, A.R.words, A.R.nmask, A.tb, pbuf, cbuf, g, gl)) {
+        if (state == 1) {
+          forward_done();
+        } else {
+          lmte = res.mte;
+          finish();
+        }
+      }
+    }
+  }
+  uint64_t rows64 = rows_acc, slots64 = slots_acc, wrows64 = wrows_acc, bases64 = bases_acc, si64 = siters_acc,
+           wsi64 = wsiters_acc;
+  for (int o = 32; o > 0; o >>= 1) {
+    cells_acc += __shfl_xor(cells_acc, o);
+    rows64 += __shfl_xor(rows64, o);
+    slots64 += __shfl_xor(slots64, o);
+    wrows64 += __shfl_xor(wrows64, o);
+    bases64 += __shfl_xor(bases64, o);
+    si64 += __shfl_xor(si64, o);
+    wsi64 += __shfl_xor(wsi64, o);
+  }
+  if (A.tstamp && lane == 0) atomicMax(&A.tstamp[1], (unsigned long long)wall_clock64());
+  if (lane == 0 && A.stats) {
+    atomicAdd(&A.stats[3], (unsigned long long)wrows64);
+    atomicAdd(&A.stats[4], (unsigned long long)bases64);
+    atomicAdd(&A.stats[5], (unsigned long long)si64);
+    atomicAdd(&A.stats[6], (unsigned long long)wsi64);
+    atomicAdd(&A.stats[0], (unsigned long long)cells_acc);
+    atomicAdd(&A.stats[1], (unsigned long long)rows64);
+    atomicAdd(&A.stats[2], (unsigned long long)slots64);
+  }
+}
+
 // probe tasks: each pair's first-longest node (Process_String_Overlaps.C:424-431)
 __global__ void k_probe_tasks(const PairRec *__restrict__ pairs, uint64_t np, const int4 *__restrict__ nodes, FwdTask *__restrict__ tasks) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -36,7 +36,8 @@ NPARAM = 64 * NFEAT + 64 + 64 * 64 + 64 + NOUT * 64 + NOUT
 
 class Tuning(ctypes.Structure):
     _fields_ = [("list_wave_min_hits", ctypes.c_uint32), ("fwd_lanes", ctypes.c_uint32), ("fwd_tile", ctypes.c_uint32),
-                ("fwd_order", ctypes.c_uint32), ("fwd_graph", ctypes.c_uint32), ("fwd_counters", ctypes.c_uint32)]
+                ("fwd_order", ctypes.c_uint32), ("fwd_graph", ctypes.c_uint32), ("fwd_counters", ctypes.c_uint32),
+                ("fwd_loop", ctypes.c_uint32)]
 
 
 class TuneTrial(ctypes.Structure):

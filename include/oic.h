@@ -153,6 +153,10 @@ typedef struct oic_tuning {
   uint32_t fwd_counters;        /* 1: count rows, lane slots and wavefront rows of the packed
                                    forward DPs (spec_rows, spec_lane_slots, spec_wave_rows); costs
                                    time, off by default */
+  uint32_t fwd_loop;            /* packed forward-DP loop: 0 (default) advances every DP of a
+                                   wavefront one row per iteration and starts a finished DP's next
+                                   task at once; 1 runs each DP to its end, so finished DPs wait for
+                                   the slowest of their wavefront */
 } oic_tuning;
 
 int oic_get_tuning(const oic_ctx *ctx, oic_tuning *out);
@@ -165,7 +169,7 @@ typedef struct oic_tune_trial {
 } oic_tune_trial;
 
 /* Autotuner: searches ref reads [bgn_ref, end_ref] against the current index under each candidate
- * launch shape (forward DP: 4/8/16/32 lanes x 32..512-cell tile, then task order; list builder
+ * launch shape (forward DP: 4/8/16/32 lanes x 32..512-cell tile, then loop form, then task order; list builder
  * threshold 256..16384 hits), keeps the fastest, and reports the trials (up to cap) when trials is not null.
  * Counters are left as they were.  No reference counterpart. */
 int oic_autotune(oic_ctx *ctx, uint32_t bgn_ref, uint32_t end_ref, oic_tuning *best, oic_tune_trial *trials,

@@ -249,19 +249,21 @@ def test_list_builders(request, fixture, min_hits):
 
 
 def test_forward_launch_shapes_identical(c3s):
-    """Every lanes-per-DP x band-tile shape of the packed forward kernel gives the oracle's output."""
+    """Every lanes-per-DP x band-tile shape of the packed forward kernel, in both loop forms, gives
+    the oracle's output."""
     ref = oracle.run(c3s, kmer=22, maxErate=0.045, minOlap=500, nThreads=4, ompThreads=16)
     eng = oic.Engine(kmer=22, maxerate=0.045, minlength=500, threads=4, schedule=oic.SCHED_BATCHED)
     eng.load_reads(c3s)
-    for lanes in (4, 8, 16, 32):
-        for tile in (32, 64, 128, 256, 512):
-            if (lanes == 4 and tile == 512) or (tile <= 64 and lanes not in (8, 16)):
-                continue
-            eng.set_tuning(fwd_lanes=lanes, fwd_tile=tile)
-            res = eng.run()
-            assert_same(res, ref)
-            assert res.counters["dp_cells"] == ref["cells_used"], (lanes, tile)
-    eng.set_tuning(fwd_lanes=16, fwd_tile=128, fwd_order=1)
+    for loop in (0, 1):
+        for lanes in (4, 8, 16, 32):
+            for tile in (32, 64, 128, 256, 512):
+                if (lanes == 4 and tile == 512) or (tile <= 64 and lanes not in (8, 16)):
+                    continue
+                eng.set_tuning(fwd_lanes=lanes, fwd_tile=tile, fwd_loop=loop)
+                res = eng.run()
+                assert_same(res, ref)
+                assert res.counters["dp_cells"] == ref["cells_used"], (loop, lanes, tile)
+    eng.set_tuning(fwd_lanes=16, fwd_tile=128, fwd_order=1, fwd_loop=0)
     res = eng.run()
     assert_same(res, ref)
     assert res.counters["dp_cells"] == ref["cells_used"]
@@ -274,6 +276,23 @@ def test_forward_launch_shapes_identical(c3s):
         assert (res.counters["graphs_launched"] > 0) == bool(graph)
     with pytest.raises(oic.OicError):
         eng.set_tuning(fwd_lanes=64)
+    with pytest.raises(oic.OicError):
+        eng.set_tuning(fwd_loop=2)
+
+
+def test_forward_loop_forms_do_the_same_work(c3s):
+    """The row-stepped loop runs exactly the per-task loop's DPs: the same cells, rows, lane slots
+    and bases per speculative pass, in fewer wavefront rows."""
+    eng = oic.Engine(kmer=22, maxerate=0.045, minlength=500, threads=4, schedule=oic.SCHED_BATCHED)
+    eng.load_reads(c3s)
+    got = {}
+    for loop in (1, 0):
+        eng.set_tuning(fwd_lanes=8, fwd_tile=64, fwd_loop=loop, fwd_counters=1)
+        got[loop] = eng.run().counters
+    for k in ("spec_cells", "spec_rows", "spec_lane_slots", "spec_bases", "spec_slide_iters", "dp_cells"):
+        assert got[0][k] == got[1][k], k
+    assert got[0]["spec_rows"] > 0
+    assert got[0]["spec_wave_rows"] < got[1]["spec_wave_rows"]
 
 
 def test_autotune_keeps_results(c3s):
@@ -282,7 +301,7 @@ def test_autotune_keeps_results(c3s):
     eng.load_reads(c3s)
     eng.build_hash_index(1, c3s.n)
     best, trials = eng.autotune(1, min(c3s.n, 60))
-    assert len(trials) == 1 + 15 + 2 + 4
+    assert len(trials) == 1 + 15 + 2 + 2 + 4
     assert best["fwd_lanes"] in (4, 8, 16, 32) and best["fwd_tile"] in (128, 256, 512)
     assert eng.tuning() == best
     assert_same(eng.run(), ref)
