@@ -1087,9 +1087,9 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     LS.fl = (uint8_t *)(b + 8 * H);
     // pairs with many hits: one wavefront each; the rest (and any handed back): one thread each
     ALLOC(c, c->l_done, np);
-    ALLOC(c, c->l_big, np * 4);
-    ALLOC(c, c->l_cnt, 32);
-    HIPCHK(c, hipMemsetAsync(c->l_cnt.p, 0, 32, c->st));
+    ALLOC(c, c->l_big, np * 8);
+    ALLOC(c, c->l_cnt, 64);
+    HIPCHK(c, hipMemsetAsync(c->l_cnt.p, 0, 64, c->st));
     uint32_t *cnt = c->l_cnt.as<uint32_t>();
     hipLaunchKernelGGL(k_big_select, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np,
                        (uint64_t)c->tune.list_wave_min_hits, c->l_done.as<uint8_t>(), c->l_big.as<uint32_t>(), cnt);
@@ -1105,18 +1105,29 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     BA.tmp = c->p_tmp.as<int4>();
     BA.nodes = c->p_nodes.as<int4>();
     BA.K = K;
-    const unsigned gw = (unsigned)std::min<uint64_t>(np, 8ull * (uint64_t)std::max(c->ncu, 1));
-    hipLaunchKernelGGL(k_build_lists_wave, dim3(gw), dim3(64), 0, c->st, BA);
+    // first level: up to 128 hits per offset; pairs with a more crowded offset go to the 512-hit level
+    BA.up = c->l_big.as<uint32_t>() + np;
+    BA.nup = cnt + 8;
+    const uint64_t ncu = (uint64_t)std::max(c->ncu, 1);
+    hipLaunchKernelGGL(k_build_lists_wave<LW_CAP_FIRST>, dim3((unsigned)std::min<uint64_t>(np, 16 * ncu)), dim3(64), 0, c->st, BA);
+    BigListArgs B2 = BA;
+    B2.big = BA.up;
+    B2.nbig = BA.nup;
+    B2.next = cnt + 9;
+    B2.up = nullptr;
+    B2.nup = nullptr;
+    hipLaunchKernelGGL(k_build_lists_wave<LW_CAP>, dim3((unsigned)std::min<uint64_t>(np, 4 * ncu)), dim3(64), 0, c->st, B2);
     hipLaunchKernelGGL(k_build_lists, dim3((unsigned)((np + 127) / 128)), dim3(128), 0, c->st, c->p_pairs.as<PairRec>(), np, hv,
                        c->p_tmp.as<int4>(), c->p_nodes.as<int4>(), K, LS, c->l_done.as<uint8_t>(), BA.nnodes_total);
   }
   HIPCHK(c, hipGetLastError());
   {
-    uint32_t lc[8];
-    HIPCHK(c, hipMemcpyAsync(lc, c->l_cnt.p, 32, hipMemcpyDeviceToHost, c->st));
+    uint32_t lc[16];
+    HIPCHK(c, hipMemcpyAsync(lc, c->l_cnt.p, 64, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->ctr.list_wave_pairs += lc[0];
     c->ctr.list_handed_back += lc[2];
+    c->ctr.list_second_level += lc[8];
     memcpy(&batch_nodes, lc + 4, 8);
     c->ctr.nodes += batch_nodes;
   }

@@ -546,27 +546,27 @@ __device__ __forceinline__ bool sm_row(SMDP &s, SMRes &res, const uint32_t *__re
       return true;
     }
     res.cells += (uint32_t)((Right - c0 + 1) < G ? (Right - c0 + 1) : G);
-    // pruned range and first-max cell in the same pass (prefixEditDistance-forward.C:245-299)
+    // pruned range and first-max cell in the same pass (prefixEditDistance-forward.C:245-299), as
+    // selects: per lane, "committed" is the best cell at or before the group's latest kept cell,
+    // "pending" the best after it, promoted when a later kept cell appears
     const bool keep = act && (Row + (d > 0 ? d : 0) >= eml);
     const uint64_t kb = group_ballot<G>(keep, g);
-    if (kb) {
-      const int32_t kl = c0 + (63 - __builtin_clzll(kb));
-      if (nL == 0x7fffffff) nL = c0 + __builtin_ctzll(kb);
-      nR = kl;
-      if (pbv > cbv || (pbv == cbv && pbd < cbd)) { cbv = pbv; cbd = pbd; }
-      pbv = -1;
-      pbd = 0x7fffffff;
-      if (act && d >= nL) {
-        if (d <= kl) {
-          if (Row > cbv || (Row == cbv && d < cbd)) { cbv = Row; cbd = d; }
-        } else {
-          pbv = Row;
-          pbd = d;
-        }
-      }
-    } else if (act && nL != 0x7fffffff) {
-      if (Row > pbv || (Row == pbv && d < pbd)) { pbv = Row; pbd = d; }
-    }
+    const bool any = kb != 0;
+    const int32_t kl = c0 + (63 - __builtin_clzll(kb | 1ull));
+    nL = (any & (nL == 0x7fffffff)) ? c0 + __builtin_ctzll(kb | (1ull << 63)) : nL;
+    nR = any ? kl : nR;
+    const bool prom = any & ((pbv > cbv) | ((pbv == cbv) & (pbd < cbd)));
+    cbv = prom ? pbv : cbv;
+    cbd = prom ? pbd : cbd;
+    const int32_t pv0 = any ? -1 : pbv, pd0 = any ? 0x7fffffff : pbd;
+    const bool in = act & (nL != 0x7fffffff) & (d >= nL);
+    const bool le = any & (d <= kl);
+    const bool bc = in & le & ((Row > cbv) | ((Row == cbv) & (d < cbd)));
+    cbv = bc ? Row : cbv;
+    cbd = bc ? d : cbd;
+    const bool bp = in & !le & (any | (Row > pv0) | ((Row == pv0) & (d < pd0)));
+    pbv = bp ? Row : pv0;
+    pbd = bp ? d : pd0;
   }
   if (nL == 0x7fffffff) { res.mte = false; res.e = s.MSe; res.d = s.MSd; return true; }
   s.Left = nL;

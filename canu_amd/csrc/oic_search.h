@@ -530,9 +530,11 @@ __global__ void k_build_lists(PairRec *__restrict__ pairs, uint64_t np, const ui
 //     live node, in which case the head-of-list test of Add_Match applies to the first node of the
 //     latest dead block.
 // Dead nodes go to tmp in blocks as they die, each record tagged so that its final position
-// (live nodes, then blocks newest first, each in list order) is nlive + ndead - tag.  A pair whose
-// offset has more than LW_CAP hits is handed back (done[p] = 0) to the one-thread builder.
-#define LW_CAP 512
+// (live nodes, then blocks newest first, each in list order) is nlive + ndead - tag.  LDS holds up
+// to CAP hits of one offset: a pair with a more crowded offset goes on to the next level's list
+// (the 512-hit kernel), or, from the last level, back to the one-thread builder (done[p] = 0).
+#define LW_CAP 512        // hits per offset of the last wavefront level
+#define LW_CAP_FIRST 128  // of the first: a quarter of the LDS, four times the wavefronts per CU
 
 __device__ __forceinline__ int32_t wave_incl_max_i32(int32_t v) {
   const int lane = threadIdx.x & 63;
@@ -562,6 +564,8 @@ struct BigListArgs {
   const uint32_t *nbig;
   uint32_t *next;           // work counter, starts at 0
   uint32_t *handed_back;    // pairs left to the one-thread builder
+  uint32_t *up;             // if not null: pairs too crowded for this level go here, for the next one
+  uint32_t *nup;
   uint8_t *done;            // per pair: 1 = list built by k_build_lists_wave
   unsigned long long *nnodes_total;
   const uint64_t *hval;
@@ -579,15 +583,16 @@ __global__ void k_big_select(const PairRec *__restrict__ pairs, uint64_t np, uin
   if (b) big[atomicAdd(nbig, 1u)] = (uint32_t)p;
 }
 
+template <int CAP>
 __global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
-  __shared__ int32_t sLd[2][LW_CAP];   // live diagonals in hit order of the last offset (descending)
-  __shared__ int4 sR[2][LW_CAP];       // live nodes: Offset, Len, Start, list position
-  __shared__ int32_t sLq[2][LW_CAP];   // list position -> live index
-  __shared__ int32_t sQ[LW_CAP];       // per hit: live index of its node, or -1
-  __shared__ int32_t sT[LW_CAP];       // per list position: the hit touching it, or INF
-  __shared__ int32_t sM[LW_CAP];       // exclusive prefix max of sT
-  __shared__ int32_t sRk[LW_CAP];      // per hit: 2 x rank among created / moved hits, +1 if it is one
-  __shared__ int32_t sIp[LW_CAP];      // per list position: rank among nodes extended in place
+  __shared__ int32_t sLd[2][CAP];   // live diagonals in hit order of the last offset (descending)
+  __shared__ int4 sR[2][CAP];       // live nodes: Offset, Len, Start, list position
+  __shared__ int32_t sLq[2][CAP];   // list position -> live index
+  __shared__ int32_t sQ[CAP];       // per hit: live index of its node, or -1
+  __shared__ int32_t sT[CAP];       // per list position: the hit touching it, or INF
+  __shared__ int32_t sM[CAP];       // exclusive prefix max of sT
+  __shared__ int32_t sRk[CAP];      // per hit: 2 x rank among created / moved hits, +1 if it is one
+  __shared__ int32_t sIp[CAP];      // per list position: rank among nodes extended in place
   __shared__ uint32_t sWork;
   const int lane = threadIdx.x;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -618,9 +623,9 @@ __global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
         const uint64_t b = __ballot(same);
         const int32_t run = (b == ~0ull) ? 64 : __builtin_ctzll(~b);
         c += run;
-        if (run < 64 || c > LW_CAP) break;
+        if (run < 64 || c > CAP) break;
       }
-      if (c > LW_CAP) { over = true; break; }
+      if (c > CAP) { over = true; break; }
       int4 *R = sR[cur], *R2 = sR[cur ^ 1];
       int32_t *Ld = sLd[cur], *Ld2 = sLd[cur ^ 1], *Lq = sLq[cur], *Lq2 = sLq[cur ^ 1];
       if (o != prev_o + 1 && nA > 0) {               // a gap: every live node dies, in list order
@@ -740,8 +745,12 @@ __global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
     }
     if (over) {
       if (lane == 0) {
-        A.done[p] = 0;
-        atomicAdd(A.handed_back, 1u);
+        if (A.up) {
+          A.up[atomicAdd(A.nup, 1u)] = p;
+        } else {
+          A.done[p] = 0;
+          atomicAdd(A.handed_back, 1u);
+        }
       }
       continue;
     }

@@ -81,7 +81,7 @@ class Counters(ctypes.Structure):
                                                "dp_cells", "dp_lane_slots", "dp_bases", "overflow_retries",
                                                "spec_tasks", "spec_cells", "spec_rows", "spec_lane_slots", "spec_wave_rows",
                                                "spec_bases", "spec_slide_iters", "spec_wave_slide_iters", "fwd_launches", "fwd_cells", "fwd_bases", "graphs_launched", "list_wave_pairs",
-                                               "list_handed_back")] + \
+                                               "list_handed_back", "list_second_level")] + \
                [(n, ctypes.c_double) for n in ("ms_index", "ms_gather", "ms_lists", "ms_extend", "ms_total", "ms_dp_kernel",
                                                "ms_fwd_kernel", "ms_fwd_narrow", "ms_fwd_graph")]
 

@@ -127,6 +127,8 @@ typedef struct oic_counters {
   uint64_t graphs_launched;    /* forward-pass hipGraphs launched */
   uint64_t list_wave_pairs;    /* pairs whose match list one wavefront built */
   uint64_t list_handed_back;   /* of those, pairs with an offset of more hits than the wavefront holds */
+  uint64_t list_second_level;  /* of those, pairs taken on by the 512-hit wavefront level (an offset
+                                  with more than 128 hits) */
   double   ms_index, ms_gather, ms_lists, ms_extend, ms_total;
   double   ms_dp_kernel;       /* device time of all DP kernels (HIP events) */
   double   ms_fwd_kernel;      /* of which the packed forward-DP kernels */

@@ -322,6 +322,21 @@ def test_list_builder_hands_back_crowded_offsets():
     assert res.counters["dp_cells"] == ref["cells_used"]
 
 
+def test_list_builder_second_level():
+    """An offset with 300 T occurrences (between the first wavefront level's 128 and the second's
+    512) is built by the second level; results stay exact and nothing is handed back."""
+    rng = np.random.default_rng(13)
+    unit = "".join(rng.choice(list("ACGT"), 20))
+    other = "".join(rng.choice(list("ACGT"), 4000))
+    t = other[:2000] + unit * 300 + other[2000:]
+    s = unit * 30 + other[2000:2600]
+    R = readsim.from_strings([s, t])
+    res, ref = run_both(R, threads=1, schedule=oic.SCHED_BATCHED, tuning=dict(list_wave_min_hits=0))
+    assert_same(res, ref)
+    assert res.counters["list_second_level"] >= 1 and res.counters["list_handed_back"] == 0
+    assert res.counters["dp_cells"] == ref["cells_used"]
+
+
 def top_kmers(R, n, seed=0):
     """The n most frequent k-mers of a read set (ties by text), half written reverse-complemented."""
     comp = str.maketrans("ACGTacgt", "TGCAtgca")
