@@ -99,6 +99,25 @@ __device__ __forceinline__ void group_argmax(int32_t &v, int32_t &d) {
   if (G >= 64) take(__shfl_xor(v, 32), __shfl_xor(d, 32));
 }
 
+// maximum over the G lanes of a group, by DPP moves within rows of 16
+template <int G>
+__device__ __forceinline__ uint64_t group_max_u64(uint64_t v) {
+  auto step = [&](uint32_t lo, uint32_t hi) {
+    const uint64_t o = ((uint64_t)hi << 32) | lo;
+    v = o > v ? o : v;
+  };
+#define OIC_DPP64(ctl) step(__builtin_amdgcn_mov_dpp((uint32_t)v, ctl, 0xF, 0xF, false), \
+                            __builtin_amdgcn_mov_dpp((uint32_t)(v >> 32), ctl, 0xF, 0xF, false))
+  if (G >= 2) OIC_DPP64(0xB1);    // quad_perm [1,0,3,2]
+  if (G >= 4) OIC_DPP64(0x4E);    // quad_perm [2,3,0,1]
+  if (G >= 8) OIC_DPP64(0x141);   // row_half_mirror
+  if (G >= 16) OIC_DPP64(0x140);  // row_mirror
+#undef OIC_DPP64
+  if (G >= 32) step(__shfl_xor((uint32_t)v, 16), __shfl_xor((uint32_t)(v >> 32), 16));
+  if (G >= 64) step(__shfl_xor((uint32_t)v, 32), __shfl_xor((uint32_t)(v >> 32), 32));
+  return v;
+}
+
 template <int G>
 __device__ __forceinline__ uint64_t group_ballot(bool p, int g) {
   const uint64_t b = __ballot(p);
@@ -493,14 +512,17 @@ __device__ __forceinline__ bool sm_row(SMDP &s, SMRes &res, const uint32_t *__re
   }
   if (s.Right - s.Left + 3 > CAP) { res.overflow = true; return true; }
   wave_fence();
-  if (gl < 4) {
-    const int32_t sd = (gl == 0) ? s.Left : (gl == 1) ? s.Left - 1 : (gl == 2) ? s.Right : s.Right + 1;
-    pbuf[sd & MASK] = -2;
+  {
+    // sentinels at Left, Left - 1, Right, Right + 1 (prefixEditDistance-forward.C:147-153)
+    const int32_t sd = ((gl & 2) ? s.Right : s.Left) + ((gl & 1) ? ((gl & 2) ? 1 : -1) : 0);
+    if (gl < 4) pbuf[sd & MASK] = -2;
   }
   wave_fence();
   const int32_t Left = s.Left, Right = s.Right;
   int32_t nL = 0x7fffffff, nR = 0;
-  int32_t cbv = -1, cbd = 0x7fffffff, pbv = -1, pbd = 0x7fffffff;
+  // candidate cells as keys Row << 32 | (2^30 - d): the larger key has the larger Row, then the
+  // smaller d (Row >= 0, |d| < 2^30); 0 = none
+  uint64_t ck = 0, pk = 0;
   for (int32_t c0 = Left; c0 <= Right; c0 += G) {
     const int32_t d = c0 + gl;
     const bool act = d <= Right;
@@ -555,23 +577,19 @@ __device__ __forceinline__ bool sm_row(SMDP &s, SMRes &res, const uint32_t *__re
     const int32_t kl = c0 + (63 - __builtin_clzll(kb | 1ull));
     nL = (any & (nL == 0x7fffffff)) ? c0 + __builtin_ctzll(kb | (1ull << 63)) : nL;
     nR = any ? kl : nR;
-    const bool prom = any & ((pbv > cbv) | ((pbv == cbv) & (pbd < cbd)));
-    cbv = prom ? pbv : cbv;
-    cbd = prom ? pbd : cbd;
-    const int32_t pv0 = any ? -1 : pbv, pd0 = any ? 0x7fffffff : pbd;
+    const uint64_t key = ((uint64_t)(uint32_t)Row << 32) | (uint32_t)(0x40000000 - d);
+    ck = (any & (pk > ck)) ? pk : ck;
+    const uint64_t p0 = any ? 0ull : pk;
     const bool in = act & (nL != 0x7fffffff) & (d >= nL);
     const bool le = any & (d <= kl);
-    const bool bc = in & le & ((Row > cbv) | ((Row == cbv) & (d < cbd)));
-    cbv = bc ? Row : cbv;
-    cbd = bc ? d : cbd;
-    const bool bp = in & !le & (any | (Row > pv0) | ((Row == pv0) & (d < pd0)));
-    pbv = bp ? Row : pv0;
-    pbd = bp ? d : pd0;
+    ck = (in & le & (key > ck)) ? key : ck;
+    pk = (in & !le & (key > p0)) ? key : p0;
   }
   if (nL == 0x7fffffff) { res.mte = false; res.e = s.MSe; res.d = s.MSd; return true; }
   s.Left = nL;
   s.Right = nR;
-  group_argmax<G>(cbv, cbd);
+  ck = group_max_u64<G>(ck);
+  const int32_t cbv = (int32_t)(ck >> 32), cbd = 0x40000000 - (int32_t)(uint32_t)ck;
   if (cbv > s.Longest) {
     s.Longest = cbv;
     const double Score = (double)cbv * tb.bmv - (double)e;
