@@ -1057,7 +1057,10 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   HIPCHK(c, hipGetLastError());
   {
     RadixTemp tmp{c->s_k2.as<uint64_t>(), c->s_v2.as<uint64_t>(), c->s_hist.as<uint32_t>(), c->s_offs.as<uint64_t>(), c->s_scan.as<uint64_t>()};
-    const int nbits = (GA.tbits + sbits + 7) / 8 * 8;
+    // Hits arrive in (S read, orientation) order (windows laid out by pbase).  A stable sort on the
+    // digits holding the T bits keeps equal sorted digits in that order, so each (S, orientation, T)
+    // pair ends up contiguous, its hits in generation order, without sorting the S bits above them.
+    const int nbits = (GA.tbits + 7) / 8 * 8;
     HIPCHK(c, radix_sort(&hk, &hv, H, nbits, tmp, c->st));
   }
   ALLOC(c, c->s_flag, H * 4);
@@ -1324,8 +1327,9 @@ extern "C" int oic_run(oic_ctx *c, uint32_t bgn_hash, uint32_t end_hash, uint32_
 // ------------------------------------------------------------------------------------ autotuner
 
 // Times the search of a sample of ref reads under each launch shape and keeps the fastest: the
-// packed forward-DP kernel over lanes per DP x band tile (device time of that kernel), then its loop
-// form and task order, then the list-builder threshold (device + host time of the list stage).
+// packed forward-DP kernel over lanes per DP x band tile, then its loop form (device time of that
+// kernel), the task order (time of the whole extension stage), then the list-builder threshold
+// (device + host time of the list stage).
 // Results never depend on the shape, so the sample's records are discarded.
 extern "C" int oic_autotune(oic_ctx *c, uint32_t bgn_ref, uint32_t end_ref, oic_tuning *best, oic_tune_trial *trials,
                             uint32_t cap, uint32_t *ntrials) {
@@ -1343,7 +1347,7 @@ extern "C" int oic_autotune(oic_ctx *c, uint32_t bgn_ref, uint32_t end_ref, oic_
     oic_sink sk{};
     const int rc = search_range(c, bgn_ref, end_ref, &sk);
     if (rc) return rc;
-    *ms = (what == 0) ? c->ctr.ms_fwd_kernel : c->ctr.ms_lists;
+    *ms = (what == 0) ? c->ctr.ms_fwd_kernel : (what == 1) ? c->ctr.ms_lists : c->ctr.ms_extend;
     if (trials && nt < cap) {
       trials[nt].tuning = t;
       trials[nt].stage = what;
@@ -1385,10 +1389,10 @@ extern "C" int oic_autotune(oic_ctx *c, uint32_t bgn_ref, uint32_t end_ref, oic_
   if (c->P.schedule != OIC_SCHED_PLAIN && !c->P.partial) {
     double bms = 1e300;
     oic_tuning b = cur;
-    for (uint32_t o : {0u, 1u}) {
+    for (uint32_t o : {0u, 1u}) {   // seed-length order runs without the per-batch graph: whole stage timed
       oic_tuning x = cur;
       x.fwd_order = o;
-      rc = trial(x, 0, &ms);
+      rc = trial(x, 2, &ms);
       if (rc) { c->ctr = saved; return rc; }
       if (ms < bms) { bms = ms; b = x; }
     }

@@ -273,7 +273,7 @@ class Engine:
         n = ctypes.c_uint32()
         self._check(lib().oic_autotune(self._ctx, bgn_ref, end_ref, ctypes.byref(best), trials, cap, ctypes.byref(n)))
         fields = [f for f, _ in Tuning._fields_]
-        out = [dict({f: getattr(t.tuning, f) for f in fields}, stage=("fwd", "lists")[t.stage], ms=t.ms)
+        out = [dict({f: getattr(t.tuning, f) for f in fields}, stage=("fwd", "lists", "extend")[t.stage], ms=t.ms)
                for t in trials[:min(n.value, cap)]]
         return {f: getattr(best, f) for f in fields}, out
 

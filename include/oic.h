@@ -166,7 +166,8 @@ int oic_set_tuning(oic_ctx *ctx, const oic_tuning *t);
 
 typedef struct oic_tune_trial {
   oic_tuning tuning;
-  int32_t    stage;             /* 0: packed forward-DP kernel, 1: match-list stage */
+  int32_t    stage;             /* what was timed: 0 the packed forward-DP kernel, 1 the match-list
+                                   stage, 2 the extension stage (forward passes through output) */
   double     ms;                /* time of that stage over the sample */
 } oic_tune_trial;
 
