@@ -758,14 +758,8 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_sm(FwdArgs A) {
 __global__ void k_probe_tasks(const PairRec *__restrict__ pairs, uint64_t np, const int4 *__restrict__ nodes, FwdTask *__restrict__ tasks) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const PairRec pr = pairs[p];
-  int32_t best = 0, bl = -1;
-  for (int32_t i = 0; i < pr.nnodes; i++) {
-    const int32_t l = nodes[pr.hb + i].y;
-    if (l > bl) { bl = l; best = i; }
-  }
   tasks[p].pair = (uint32_t)p;
-  tasks[p].node = (uint32_t)best;
+  tasks[p].node = (uint32_t)pairs[p].first_longest;   // found by the list builders
 }
 
 // Pairs whose other nodes are all worth a forward DP ahead of Process_Matches: those whose probe

@@ -371,7 +371,7 @@ struct PairRec {
   int32_t  max_mult;
   int32_t  longest;              // longest node Len
   int32_t  dmin, dmax;           // min / max node diagonal
-  int32_t  pad;
+  int32_t  first_longest;        // list position of the first node of Len == longest
 };
 
 __global__ void k_pair_write(const uint64_t *__restrict__ hkey, uint64_t n, const uint32_t *__restrict__ flag,
@@ -490,12 +490,12 @@ __global__ void k_build_lists(PairRec *__restrict__ pairs, uint64_t np, const ui
     prev_o = o;
   }
   // list: live nodes, then dead blocks newest first
-  int32_t n = 0, longest = 0, dmin = 0x7fffffff, dmax = -0x7fffffff;
+  int32_t n = 0, longest = 0, first_longest = 0, dmin = 0x7fffffff, dmax = -0x7fffffff;
   auto emit = [&](int32_t x) {
     const int4 nd = N[x];
     nodes[hb + n] = make_int4(nd.x, nd.y, nd.z, 0);
+    if (nd.y > longest) { longest = nd.y; first_longest = n; }
     n++;
-    if (nd.y > longest) longest = nd.y;
     const int32_t dg = nd.x - nd.z;
     if (dg < dmin) dmin = dg;
     if (dg > dmax) dmax = dg;
@@ -512,6 +512,7 @@ __global__ void k_build_lists(PairRec *__restrict__ pairs, uint64_t np, const ui
   pr.diag_end = (int32_t)(hval[he - 1] & 0x1fffff);
   pr.max_mult = maxm;
   pr.longest = longest;
+  pr.first_longest = first_longest;
   pr.dmin = dmin;
   pr.dmax = dmax;
   pairs[p] = pr;
@@ -593,6 +594,7 @@ __global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
   __shared__ int32_t sM[CAP];       // exclusive prefix max of sT
   __shared__ int32_t sRk[CAP];      // per hit: 2 x rank among created / moved hits, +1 if it is one
   __shared__ int32_t sIp[CAP];      // per list position: rank among nodes extended in place
+  __shared__ uint64_t sW[128];      // hit values [wb, wb + 128) of the pair, refilled every 64 hits
   __shared__ uint32_t sWork;
   const int lane = threadIdx.x;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -613,13 +615,25 @@ __global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
     int32_t nA = 0, nnode = 0, ndq = 0, maxm = 0, prev_o = -2;
     bool consistent = true, over = false;
     int4 head = make_int4(0, 0, 0, 0);
-    uint64_t h = hb;
+    uint64_t h = hb, wb = hb;
+    auto fill = [&]() {   // the window from wb (hits past the pair's end read as 0, never used)
+      for (int i = lane; i < 128; i += 64) sW[i] = (wb + (uint64_t)i < he) ? A.hval[wb + i] : 0ull;
+      __syncthreads();
+    };
+    // hit value hh of the pair: from the window when it holds it
+    auto hit = [&](uint64_t hh) -> uint64_t { return (hh - wb < 128) ? sW[hh - wb] : A.hval[hh]; };
+    fill();
     while (h < he) {
-      const int32_t o = (int32_t)(A.hval[h] & 0x1fffff);
+      if (h - wb >= 64) {   // keep [h, h + 64) in the window
+        __syncthreads();
+        wb = h;
+        fill();
+      }
+      const int32_t o = (int32_t)(sW[h - wb] & 0x1fffff);
       int32_t c = 0;
       for (;;) {
         const uint64_t hh = h + (uint64_t)c + (uint64_t)lane;
-        const bool same = hh < he && (int32_t)(A.hval[hh] & 0x1fffff) == o;
+        const bool same = hh < he && (int32_t)(hit(hh) & 0x1fffff) == o;
         const uint64_t b = __ballot(same);
         const int32_t run = (b == ~0ull) ? 64 : __builtin_ctzll(~b);
         c += run;
@@ -640,7 +654,7 @@ __global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
       }
       // hits -> live nodes
       for (int32_t j = lane; j < c; j += 64) {
-        const uint64_t v = A.hval[h + j];
+        const uint64_t v = hit(h + j);
         const int32_t g = (int32_t)((v >> 21) & 0x1fffff) - o;
         maxm = max(maxm, (int32_t)(v >> 42));
         int32_t lo = 0, hi = nA;
@@ -760,10 +774,12 @@ __global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
     const int4 *R = sR[cur];
     const int32_t *Lq = sLq[cur];
     int32_t longest = 0, dmin = 0x7fffffff, dmax = -0x7fffffff;
+    uint64_t lkey = 0;   // Len << 32 | ~position: the maximum is the first node of the longest Len
     for (int32_t k = lane; k < nA; k += 64) {
       int4 r = R[Lq[k]];
       r.w = 0;
       out[k] = r;
+      lkey = max(lkey, ((uint64_t)(uint32_t)r.y << 32) | (uint32_t)~(uint32_t)k);
       longest = max(longest, r.y);
       dmin = min(dmin, r.x - r.z);
       dmax = max(dmax, r.x - r.z);
@@ -773,11 +789,13 @@ __global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
       const int32_t d = nA + ndq - r.w;
       r.w = 0;
       out[d] = r;
+      lkey = max(lkey, ((uint64_t)(uint32_t)r.y << 32) | (uint32_t)~(uint32_t)d);
       longest = max(longest, r.y);
       dmin = min(dmin, r.x - r.z);
       dmax = max(dmax, r.x - r.z);
     }
     longest = wave_max_i32(longest);
+    for (int o = 32; o > 0; o >>= 1) lkey = max(lkey, (uint64_t)__shfl_xor(lkey, o));
     dmin = wave_min_i32(dmin);
     dmax = wave_max_i32(dmax);
     maxm = wave_max_i32(maxm);
@@ -789,6 +807,7 @@ __global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
       pr.diag_end = (int32_t)(A.hval[he - 1] & 0x1fffff);
       pr.max_mult = maxm;
       pr.longest = longest;
+      pr.first_longest = (int32_t)~(uint32_t)lkey;
       pr.dmin = dmin;
       pr.dmax = dmax;
       A.pairs[p] = pr;

@@ -12,7 +12,7 @@
 #include <stdint.h>
 
 #define SORT_THREADS 256
-#define SORT_ITEMS   8
+#define SORT_ITEMS   32
 #define SORT_TILE    (SORT_THREADS * SORT_ITEMS)
 #define SCAN_THREADS 256
 #define SCAN_ITEMS   8
