@@ -5,6 +5,7 @@
 // Ref reads are processed in batches sized to bound device memory; every stage of a batch runs
 // on the context's stream.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -27,9 +28,18 @@
 
 namespace {
 
+// Owns one device allocation.  Not copyable: a copy's destructor would free the buffer under the
+// original.  Exchange ownership with swap().
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  void swap(DevBuf &o) {
+    std::swap(p, o.p);
+    std::swap(bytes, o.bytes);
+  }
   hipError_t ensure(size_t b) {
     if (b <= bytes && p) return hipSuccess;
     release();
@@ -85,7 +95,7 @@ struct oic_ctx {
   int dshift = 0;
   DevBuf x_ukey, x_ustart, x_dir, x_occ;
   // sort / scan temporaries
-  DevBuf s_k1, s_v1, s_k2, s_v2, s_hist, s_offs, s_scan, s_flag, s_idx, s_small;
+  DevBuf s_k1, s_v1, s_k2, s_v2, s_scan, s_flag, s_idx, s_small, s_rp;
   // search
   DevBuf g_cnt, g_off, g_pbase, g_searched, g_sdsum, g_phi, g_pval;
   DevBuf l_scr, l_done, l_big, l_cnt;
@@ -407,13 +417,24 @@ extern "C" int oic_load_reads(oic_ctx *c, const oic_reads *rd) {
 
 // ------------------------------------------------------------------------------------ index
 
+// alternate key / value buffers for sort_pairs_rocprim, and scan scratch, for n elements
 static int ensure_sort_temp(oic_ctx *c, uint64_t n) {
-  const uint64_t nb = radix_blocks(n) + 1;
   ALLOC(c, c->s_k2, n * 8);
   ALLOC(c, c->s_v2, n * 8);
-  ALLOC(c, c->s_hist, 256 * nb * 4);
-  ALLOC(c, c->s_offs, 256 * nb * 8);
-  ALLOC(c, c->s_scan, scan_scratch_words(std::max<uint64_t>(n, 256 * nb)) * 8);
+  ALLOC(c, c->s_scan, scan_scratch_words(n) * 8);
+  return OIC_OK;
+}
+
+// Stable sort of n (key, value) pairs on key bits [0, nbits) with rocPRIM's device radix sort; the
+// result may land in the alternate buffers (s_k2 / s_v2), and *k / *v then point there.
+static int sort_pairs_rocprim(oic_ctx *c, uint64_t **k, uint64_t **v, uint64_t n, int nbits) {
+  rocprim::double_buffer<uint64_t> K(*k, c->s_k2.as<uint64_t>()), V(*v, c->s_v2.as<uint64_t>());
+  size_t bytes = 0;
+  HIPCHK(c, rocprim::radix_sort_pairs(nullptr, bytes, K, V, (size_t)n, 0u, (unsigned)nbits, c->st));
+  ALLOC(c, c->s_rp, bytes);
+  HIPCHK(c, rocprim::radix_sort_pairs(c->s_rp.p, bytes, K, V, (size_t)n, 0u, (unsigned)nbits, c->st));
+  *k = K.current();
+  *v = V.current();
   return OIC_OK;
 }
 
@@ -448,12 +469,11 @@ static int build_table(oic_ctx *c, uint32_t bgn, uint32_t end, std::vector<uint3
   if (N) {
     hipLaunchKernelGGL(k_kmer_emit, dim3(nb), dim3(256), 0, c->st, R, bgn, nb, d_posbase.as<uint64_t>(), d_loaded.as<uint8_t>(), K,
                        invalid, keys, vals);
-    RadixTemp tmp{c->s_k2.as<uint64_t>(), c->s_v2.as<uint64_t>(), c->s_hist.as<uint32_t>(), c->s_offs.as<uint64_t>(), c->s_scan.as<uint64_t>()};
-    const int nbits = ((2 * K + 1) + 7) / 8 * 8;
-    HIPCHK(c, radix_sort(&keys, &vals, N, nbits, tmp, c->st));
+    const int rc = sort_pairs_rocprim(c, &keys, &vals, N, 2 * K + 1);   // stable: positions stay in (read, offset) order
+    if (rc) return rc;
     // the sorted arrays may have moved into the temp buffers: swap ownership so x_occ holds vals
-    if (keys != c->s_k1.as<uint64_t>()) std::swap(c->s_k1, c->s_k2);
-    if (vals != c->x_occ.as<uint64_t>()) std::swap(c->x_occ, c->s_v2);
+    if (keys != c->s_k1.as<uint64_t>()) c->s_k1.swap(c->s_k2);
+    if (vals != c->x_occ.as<uint64_t>()) c->x_occ.swap(c->s_v2);
     keys = c->s_k1.as<uint64_t>();
     vals = c->x_occ.as<uint64_t>();
   }
@@ -792,8 +812,8 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
       uint64_t *tk = c->s_k1.as<uint64_t>(), *tv = c->s_v1.as<uint64_t>();
       hipLaunchKernelGGL(k_task_keys, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->st, c->b_spec.as<FwdTask>(), nt,
                          c->p_pairs.as<PairRec>(), c->p_nodes.as<int4>(), tk, tv);
-      RadixTemp tmp{c->s_k2.as<uint64_t>(), c->s_v2.as<uint64_t>(), c->s_hist.as<uint32_t>(), c->s_offs.as<uint64_t>(), c->s_scan.as<uint64_t>()};
-      HIPCHK(c, radix_sort(&tk, &tv, nt, 16, tmp, c->st));
+      const int src = sort_pairs_rocprim(c, &tk, &tv, nt, 16);
+      if (src) return src;
       FA.tasks = (const FwdTask *)tv;
     }
     ALLOC(c, c->b_ovf, 2 * nt * sizeof(FwdTask));
@@ -1056,12 +1076,12 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
                      c->g_phi.as<uint64_t>(), c->g_pval.as<uint64_t>(), P, H, GA.X.occ, GA.tbits, hk, hv);
   HIPCHK(c, hipGetLastError());
   {
-    RadixTemp tmp{c->s_k2.as<uint64_t>(), c->s_v2.as<uint64_t>(), c->s_hist.as<uint32_t>(), c->s_offs.as<uint64_t>(), c->s_scan.as<uint64_t>()};
     // Hits arrive in (S read, orientation) order (windows laid out by pbase).  A stable sort on the
     // digits holding the T bits keeps equal sorted digits in that order, so each (S, orientation, T)
     // pair ends up contiguous, its hits in generation order, without sorting the S bits above them.
     const int nbits = (GA.tbits + 7) / 8 * 8;
-    HIPCHK(c, radix_sort(&hk, &hv, H, nbits, tmp, c->st));
+    const int rc = sort_pairs_rocprim(c, &hk, &hv, H, nbits);
+    if (rc) return rc;
   }
   ALLOC(c, c->s_flag, H * 4);
   ALLOC(c, c->s_idx, H * 8);
