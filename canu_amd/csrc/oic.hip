@@ -694,13 +694,30 @@ static void launch_fwd(oic_ctx *c, const FwdArgs &FA, uint32_t waves) {
   else hipLaunchKernelGGL((k_fwd_tasks<G, CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
 }
 
+// persistent wavefronts for a packed launch: as many as fit on every CU at once (registers and LDS
+// decide), but no more than the tasks can use
+template <int G, int CAP, bool STATS>
+static uint32_t fwd_waves(oic_ctx *c, uint64_t max_tasks) {
+  static int cached[2] = {0, 0};   // per loop form; the same on every device of one architecture
+  const int sm = (G < 64 && c->tune.fwd_loop == 0) ? 1 : 0;
+  int per_cu = cached[sm];
+  if (per_cu < 1) {
+    const hipError_t e = sm ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_sm<G, CAP, STATS>, OIC_WAVE, 0)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_tasks<G, CAP, STATS>, OIC_WAVE, 0);
+    if (e != hipSuccess || per_cu < 1) { (void)hipGetLastError(); per_cu = 16; }
+    cached[sm] = per_cu;
+  }
+  const uint64_t need = (max_tasks + (OIC_WAVE / G) - 1) / (OIC_WAVE / G);
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->ncu * (uint64_t)per_cu, need));
+}
+
 template <int G, int CAP, bool STATS>
 static int run_fwd(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
   if (n == 0) return OIC_OK;
   FA.ntasks = n;
   HIPCHK(c, hipMemsetAsync(FA.queue, 0, 4, c->st));
   HIPCHK(c, hipMemsetAsync(FA.stats, 0, 64, c->st));
-  const uint32_t waves = (uint32_t)std::min<uint64_t>((uint64_t)c->ncu * 16, (n + (OIC_WAVE / G) - 1) / (OIC_WAVE / G));
+  const uint32_t waves = fwd_waves<G, CAP, STATS>(c, n);
   HIPCHK(c, hipEventRecord(c->ev0, c->st));
   launch_fwd<G, CAP, STATS>(c, FA, waves);
   HIPCHK(c, hipGetLastError());
@@ -832,7 +849,7 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
 // e0 / e1 (recorded only outside graph capture) time the launch
 template <int G, int CAP, bool STATS>
 static void enqueue_fwd(oic_ctx *c, const FwdArgs &FA, uint64_t max_tasks, hipEvent_t e0, hipEvent_t e1) {
-  const uint32_t waves = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->ncu * 16, (max_tasks + (OIC_WAVE / G) - 1) / (OIC_WAVE / G)));
+  const uint32_t waves = fwd_waves<G, CAP, STATS>(c, max_tasks);
   if (e0) hipEventRecord(e0, c->st);
   launch_fwd<G, CAP, STATS>(c, FA, waves);
   if (e1) hipEventRecord(e1, c->st);
