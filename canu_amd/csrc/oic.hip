@@ -1127,16 +1127,18 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     LS.fl = (uint8_t *)(b + 8 * H);
     // pairs with many hits: one wavefront each; the rest (and any handed back): one thread each
     ALLOC(c, c->l_done, np);
-    ALLOC(c, c->l_big, np * 8);
+    ALLOC(c, c->l_big, np * 4 * (LW_BUCKETS + 1));
     ALLOC(c, c->l_cnt, 64);
     HIPCHK(c, hipMemsetAsync(c->l_cnt.p, 0, 64, c->st));
     uint32_t *cnt = c->l_cnt.as<uint32_t>();
     hipLaunchKernelGGL(k_big_select, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np,
-                       (uint64_t)c->tune.list_wave_min_hits, c->l_done.as<uint8_t>(), c->l_big.as<uint32_t>(), cnt);
+                       (uint64_t)c->tune.list_wave_min_hits, c->l_done.as<uint8_t>(), c->l_big.as<uint32_t>(), cnt, cnt + 12);
     BigListArgs BA;
     BA.pairs = c->p_pairs.as<PairRec>();
     BA.big = c->l_big.as<uint32_t>();
     BA.nbig = cnt;
+    BA.bcnt = cnt + 12;
+    BA.bstride = np;
     BA.next = cnt + 1;
     BA.handed_back = cnt + 2;
     BA.done = c->l_done.as<uint8_t>();
@@ -1146,13 +1148,14 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     BA.nodes = c->p_nodes.as<int4>();
     BA.K = K;
     // first level: up to 128 hits per offset; pairs with a more crowded offset go to the 512-hit level
-    BA.up = c->l_big.as<uint32_t>() + np;
+    BA.up = c->l_big.as<uint32_t>() + LW_BUCKETS * np;
     BA.nup = cnt + 8;
     const uint64_t ncu = (uint64_t)std::max(c->ncu, 1);
     hipLaunchKernelGGL(k_build_lists_wave<LW_CAP_FIRST>, dim3((unsigned)std::min<uint64_t>(np, 16 * ncu)), dim3(64), 0, c->st, BA);
     BigListArgs B2 = BA;
     B2.big = BA.up;
     B2.nbig = BA.nup;
+    B2.bcnt = nullptr;
     B2.next = cnt + 9;
     B2.up = nullptr;
     B2.nup = nullptr;

@@ -563,6 +563,8 @@ struct BigListArgs {
   PairRec *pairs;
   const uint32_t *big;      // indices of pairs with many hits
   const uint32_t *nbig;
+  const uint32_t *bcnt;     // if not null: big holds LW_BUCKETS lists of bstride entries, bcnt[b]
+  uint64_t bstride;         //   of them used, taken in bucket order (largest pairs first)
   uint32_t *next;           // work counter, starts at 0
   uint32_t *handed_back;    // pairs left to the one-thread builder
   uint32_t *up;             // if not null: pairs too crowded for this level go here, for the next one
@@ -575,13 +577,20 @@ struct BigListArgs {
   int32_t K;
 };
 
+// Pairs for the wavefront builder, bucketed by hit count (>= 2^17, 2^14, 2^11, the rest) so that
+// the largest, whose offsets run one after another, start first and do not end the launch late.
+#define LW_BUCKETS 4
 __global__ void k_big_select(const PairRec *__restrict__ pairs, uint64_t np, uint64_t min_hits, uint8_t *__restrict__ done,
-                             uint32_t *__restrict__ big, uint32_t *__restrict__ nbig) {
+                             uint32_t *__restrict__ big, uint32_t *__restrict__ nbig, uint32_t *__restrict__ bcnt) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const bool b = pairs[p].he - pairs[p].hb >= min_hits;
+  const uint64_t nh = pairs[p].he - pairs[p].hb;
+  const bool b = nh >= min_hits;
   done[p] = b ? 1 : 0;
-  if (b) big[atomicAdd(nbig, 1u)] = (uint32_t)p;
+  if (!b) return;
+  const int k = nh >= (1u << 17) ? 0 : nh >= (1u << 14) ? 1 : nh >= (1u << 11) ? 2 : 3;
+  big[(uint64_t)k * np + atomicAdd(&bcnt[k], 1u)] = (uint32_t)p;
+  atomicAdd(nbig, 1u);
 }
 
 template <int CAP>
@@ -607,7 +616,15 @@ __global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
     const uint32_t w = sWork;
     __syncthreads();
     if (w >= nbig) break;
-    const uint32_t p = A.big[w];
+    uint32_t p;
+    if (A.bcnt) {
+      uint32_t r = w;
+      int k = 0;
+      while (k < LW_BUCKETS - 1 && r >= A.bcnt[k]) r -= A.bcnt[k++];
+      p = A.big[(uint64_t)k * A.bstride + r];
+    } else {
+      p = A.big[w];
+    }
     PairRec pr = A.pairs[p];
     const uint64_t hb = pr.hb, he = pr.he;
     int4 *dq = A.tmp + hb;
