@@ -35,7 +35,13 @@ WORKLOADS = {
                desc="50 Mbp uniform genome, 30x lognormal 12k/5k (max 60k), 1.5/1.5/1.0% sub/ins/del"),
     "c3": dict(preset="c3", hashbits=26, hashload=0.8, hashdatalen=2_000_000_000,
                desc="c2 with 25% repeats: 300 families 3-8 kbp (1-5% diverged), 20 satellite arrays of a 171 bp unit"),
+    "c5": dict(preset="c5", hashbits=27, hashload=0.8, hashdatalen=4_000_000_000, maxerate=0.01, minlength=1000,
+               desc="100 Mbp genome with the c3 repeat mix, 30x lognormal 18k/4k, 0.2% errors mostly homopolymer "
+                    "indels, homopolymer-compressed reads"),
 }
+for _w in WORKLOADS.values():
+    _w.setdefault("maxerate", 0.045)
+    _w.setdefault("minlength", 500)
 
 
 def log(*a):
@@ -54,13 +60,14 @@ def traffic_from_profiles(kernel):
     return None
 
 
-def cpu_baseline(preset, genome_len, threads):
+def cpu_baseline(W, genome_len, threads):
     """The oracle ('port') on the same generator at a smaller genome, timed on host cores."""
     from canu_amd import readsim
     from oracle import oracle
+    preset = W["preset"]
     cfg = readsim.PRESETS[preset].scaled(genome_len, seed=readsim.PRESETS[preset].seed)
     R = readsim.simulate(cfg)
-    out = oracle.run(R, kmer=22, maxErate=0.045, minOlap=500, nThreads=threads, ompThreads=threads)
+    out = oracle.run(R, kmer=22, maxErate=W["maxerate"], minOlap=W["minlength"], nThreads=threads, ompThreads=threads)
     return dict(value=out["cells_used"] / out["t_search"], unit="DP cells/s", cores=threads, kind="port",
                 sample=f"oracle on the {preset} generator at a {genome_len/1e6:g} Mbp genome ({R.n} reads, all-vs-all): "
                        f"{out['cells_used']:.3g} used cells ({out['cells']:.3g} incl. unused reverse DPs) in "
@@ -147,7 +154,7 @@ def main():
     log(f"simulated {R.n} reads, {int(R.length.sum())/1e9:.3f} Gbp in {time.perf_counter()-t0:.1f} s")
 
     sched = dict(plain=oic.SCHED_PLAIN, batched=oic.SCHED_BATCHED, learned=oic.SCHED_LEARNED)
-    kw = dict(kmer=22, maxerate=0.045, minlength=500, hashbits=W["hashbits"], hashload=W["hashload"],
+    kw = dict(kmer=22, maxerate=W["maxerate"], minlength=W["minlength"], hashbits=W["hashbits"], hashload=W["hashload"],
               hashdatalen=W["hashdatalen"], threads=1)
     eng = oic.Engine(schedule=sched[args.schedule], **kw)
     t0 = time.perf_counter()
@@ -200,7 +207,7 @@ def main():
         "value": cells_s, "unit": "DP cells/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * tot / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int32", "data": f"synthetic ({W['desc']})",
-        "config": {"workload": f"{args.workload}: {W['desc']}; k=22 --maxerate 0.045 --minlength 500",
+        "config": {"workload": f"{args.workload}: {W['desc']}; k=22 --maxerate {W['maxerate']} --minlength {W['minlength']}",
                    "ref_reads_per_step": args.batch, "reads": R.n, "schedule": args.schedule},
         "overlaps_per_s": olap_s,
         "dp_cells_per_step": d["dp_cells"] / steps,
@@ -242,7 +249,7 @@ def main():
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
         log(f"cpu baseline: oracle, {threads} threads")
-        line["cpu_baseline"] = cpu_baseline(W["preset"], args.cpu_genome, threads)
+        line["cpu_baseline"] = cpu_baseline(W, args.cpu_genome, threads)
     print(json.dumps(line), flush=True)
 
 

@@ -669,6 +669,55 @@ __global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
         ndq += nA;
         nA = 0;
       }
+      if (c == 1) {
+        // One hit (most offsets of plain overlaps): its node -- the live one on its diagonal, extended,
+        // or a new one -- is the only node left live, at list position 0 (it moves to the front when
+        // untouched nodes precede it), and every other live node dies, in list order.
+        const uint64_t v = hit(h);
+        const int32_t g = (int32_t)((v >> 21) & 0x1fffff) - o;
+        maxm = max(maxm, (int32_t)(v >> 42));
+        int32_t q = -1;
+        for (int32_t k0 = 0; k0 < nA; k0 += 64) {
+          const uint64_t bm = __ballot(k0 + lane < nA && Ld[k0 + lane] == g);
+          if (bm) { q = k0 + __builtin_ctzll(bm); break; }
+        }
+        const int32_t kp = (q >= 0) ? R[q].w : -1;   // the hit node's list position
+        const int32_t sz = (q >= 0) ? nA - 1 : nA;   // nodes that die
+        for (int32_t k = lane; k < nA; k += 64) {
+          if (k == kp) continue;
+          const int32_t r = (kp >= 0 && k > kp) ? k - 1 : k;
+          int4 rec = R[Lq[k]];
+          rec.w = ndq + sz - r;
+          dq[ndq + r] = rec;
+        }
+        int4 nrec;
+        if (q >= 0) {
+          nrec = R[q];
+          nrec.y += 1;
+        } else {
+          nrec = make_int4(g + o, K, o, 0);
+          bool bad = nA > 0;   // a created node makes the pair inconsistent unless no node is live
+          if (nA == 0 && nnode > 0) {
+            const int32_t es = head.z + head.y - K + 1, dg = head.x - head.z;
+            if (abs(dg - g) > 3 || o < es + K - 2) bad = true;
+          }
+          if (bad) consistent = false;
+          nnode += 1;
+        }
+        nrec.w = 0;
+        ndq += sz;
+        if (lane == 0) {
+          R2[0] = nrec;
+          Ld2[0] = g;
+          Lq2[0] = 0;
+        }
+        __syncthreads();
+        nA = 1;
+        prev_o = o;
+        h += 1;
+        cur ^= 1;
+        continue;
+      }
       // hits -> live nodes
       for (int32_t j = lane; j < c; j += 64) {
         const uint64_t v = hit(h + j);
