@@ -457,6 +457,32 @@ __device__ __forceinline__ int32_t sm_match16(const SMDP &s, const uint32_t *__r
   return x ? (int32_t)(__builtin_ctz(x) >> 1) : 16;
 }
 
+// 64 bases of both tails from positions pa and pb, compared: the number of leading equal bases (64
+// if all are).  Reads five words per tail (the packed reads carry 64 guard bases per strand and slack
+// after the last one, so the words past a view's end exist).
+__device__ __forceinline__ int32_t sm_match64(const SMDP &s, const uint32_t *__restrict__ words,
+                                              const uint32_t *__restrict__ nmask, int32_t pa, int32_t pb) {
+  const int32_t qa = s.ta.r0 + pa, qb = s.tb.r0 + pb;
+  const int32_t wa = qa >> 4, wb = qb >> 4;
+  // the five words of each tail, from its LDS copy when they lie inside it
+  const uint32_t *A = (wa + 4 < OIC_SEQ_WIN) ? s.ta.lds + wa : words + (s.ta.w0 + wa);
+  const uint32_t *B = (wb + 4 < OIC_SEQ_WIN) ? s.tb.lds + wb : words + (s.tb.w0 + wb);
+  const uint32_t sa = (uint32_t)(qa & 15) * 2u, sb = (uint32_t)(qb & 15) * 2u;
+  uint32_t a0 = A[0], b0 = B[0];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const uint32_t a1 = A[k + 1], b1 = B[k + 1];
+    uint32_t x = __builtin_amdgcn_alignbit(a1, a0, sa) ^ __builtin_amdgcn_alignbit(b1, b0, sb);
+    x = (x | (x >> 1)) & 0x55555555u;
+    if (s.nA) x &= ~nwin16(nmask, s.a_b0 + pa + 16 * k);
+    if (s.nB) x &= ~nwin16(nmask, s.b_b0 + pb + 16 * k);
+    if (x) return 16 * k + (int32_t)(__builtin_ctz(x) >> 1);
+    a0 = a1;
+    b0 = b1;
+  }
+  return 64;
+}
+
 // Stage the tails and run row 0 (the exact-prefix slide, prefixEditDistance-forward.C:113-132).
 // Returns true when the DP is already over (res holds its outcome).
 template <int G>
@@ -532,20 +558,34 @@ __device__ __forceinline__ bool sm_row(SMDP &s, SMRes &res, const uint32_t *__re
       const int32_t x1 = 1 + pbuf[d & MASK], x2 = pbuf[(d - 1) & MASK], x3 = 1 + pbuf[(d + 1) & MASK];
       Row = x1 > x2 ? x1 : x2;
       Row = Row > x3 ? Row : x3;
-      // slide along diagonal d while the bases match (prefixEditDistance-forward.C:155-167)
+      // slide along diagonal d while the bases match (prefixEditDistance-forward.C:155-167): a
+      // 16-base window first (most slides end in it), then 64-base windows (long matches)
       const int32_t r1 = s.m - Row, r2 = s.n - Row - d;
       int32_t rem = r1 < r2 ? r1 : r2;
       int32_t adv = 0;
-      while (rem > 0) {
+      if (rem > 0) {
         if (STATS) {
           res.siters++;
           if (lane_id() == __builtin_ctzll(__ballot(1))) res.wsiters++;
         }
-        const int32_t kk = sm_match16(s, words, nmask, Row + adv, Row + d + adv);
-        if (kk >= rem) { adv += rem; break; }
-        adv += kk;
-        if (kk < 16) break;
-        rem -= 16;
+        const int32_t kk = sm_match16(s, words, nmask, Row, Row + d);
+        if (kk >= rem) {
+          adv = rem;
+        } else {
+          adv = kk;
+          rem = (kk == 16) ? rem - 16 : 0;
+          while (rem > 0) {
+            if (STATS) {
+              res.siters++;
+              if (lane_id() == __builtin_ctzll(__ballot(1))) res.wsiters++;
+            }
+            const int32_t k64 = sm_match64(s, words, nmask, Row + adv, Row + d + adv);
+            if (k64 >= rem) { adv += rem; break; }
+            adv += k64;
+            if (k64 < 64) break;
+            rem -= 64;
+          }
+        }
       }
       res.bases += (uint32_t)adv + 1u;
       Row += adv;
@@ -600,8 +640,9 @@ __device__ __forceinline__ bool sm_row(SMDP &s, SMRes &res, const uint32_t *__re
   return false;
 }
 
+// five wavefronts per SIMD (at most 102 VGPRs): the occupancy its LDS use allows at 8 x 64
 template <int G, int CAP, bool STATS>
-__global__ void __launch_bounds__(OIC_WAVE) k_fwd_sm(FwdArgs A) {
+__global__ void __launch_bounds__(OIC_WAVE) __attribute__((amdgpu_waves_per_eu(5, 8))) k_fwd_sm(FwdArgs A) {
   constexpr int NG = OIC_WAVE / G;
   __shared__ int32_t tile[NG][2][CAP];
   __shared__ uint32_t seqA[NG][OIC_SEQ_WIN], seqT[NG][OIC_SEQ_WIN];

@@ -282,14 +282,15 @@ def test_forward_launch_shapes_identical(c3s):
 
 def test_forward_loop_forms_do_the_same_work(c3s):
     """The row-stepped loop runs exactly the per-task loop's DPs: the same cells, rows, lane slots
-    and bases per speculative pass, in fewer wavefront rows."""
+    and bases per speculative pass, in fewer wavefront rows (its slides compare 64-base windows after
+    the first 16 bases, so window counts differ)."""
     eng = oic.Engine(kmer=22, maxerate=0.045, minlength=500, threads=4, schedule=oic.SCHED_BATCHED)
     eng.load_reads(c3s)
     got = {}
     for loop in (1, 0):
         eng.set_tuning(fwd_lanes=8, fwd_tile=64, fwd_loop=loop, fwd_counters=1)
         got[loop] = eng.run().counters
-    for k in ("spec_cells", "spec_rows", "spec_lane_slots", "spec_bases", "spec_slide_iters", "dp_cells"):
+    for k in ("spec_cells", "spec_rows", "spec_lane_slots", "spec_bases", "dp_cells"):
         assert got[0][k] == got[1][k], k
     assert got[0]["spec_rows"] > 0
     assert got[0]["spec_wave_rows"] < got[1]["spec_wave_rows"]
