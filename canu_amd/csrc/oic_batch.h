@@ -9,12 +9,15 @@
 //   cache[node] = 1  not a dovetail extension (Process_Matches skips the DP, removes the node)
 //                 2  dovetail (Process_Matches runs the whole extension for the deltas)
 //                 0  not computed (band outgrew the tile)
-// Tasks run 4 to a wavefront (16 lanes, 256-cell band tile); those whose band outgrows the tile are
-// queued and re-run one to a wavefront with an 8192-cell tile, which holds any band the error
-// limit allows (2 x 2700 + 3 cells at 60 kbp and 4.5 %).
-// The DP below is wave_prefix_dp's row loop without row storage or traceback, on a G-lane group,
-// with the first 1024 bases of both tails staged in LDS.  When the forward DP reaches the end, the
-// reverse DP on the left tails runs too, so the cached outcome is the extension's kind.
+// Tasks run 64/G to a wavefront (G lanes each, a T-cell band tile in LDS; the autotuner picks G and
+// T, 8 x 64 on repeat-rich reads); those whose band outgrows the tile are queued and re-run one to a
+// wavefront with a 512-cell tile, then an 8192-cell one, which holds any band the error limit
+// allows (2 x 2700 + 3 cells at 60 kbp and 4.5 %).
+// The DP is wave_prefix_dp's row loop without row storage or traceback, on a G-lane group, with the
+// first 1024 bases of both tails staged in LDS.  When the forward DP reaches the end, the reverse DP
+// on the left tails runs too, so the cached outcome is the extension's kind.  Two loop forms:
+// k_fwd_tasks runs each DP to its end (group_prefix_dp); k_fwd_sm advances every DP of the
+// wavefront by one row per iteration and starts a finished DP's successor at once (sm_start, sm_row).
 #pragma once
 
 #include "oic_search.h"
