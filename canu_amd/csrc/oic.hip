@@ -690,8 +690,13 @@ static int predict_pairs(oic_ctx *c, uint64_t np, std::vector<float> &pred) {
 // the packed kernel in the loop form the tuning selects (one DP per wavefront: the per-task loop)
 template <int G, int CAP, bool STATS>
 static void launch_fwd(oic_ctx *c, const FwdArgs &FA, uint32_t waves) {
-  if (G < 64 && c->tune.fwd_loop == 0) hipLaunchKernelGGL((k_fwd_sm<G, CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
-  else hipLaunchKernelGGL((k_fwd_tasks<G, CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
+  if constexpr (G < 64) {
+    if (c->tune.fwd_loop == 0) {
+      hipLaunchKernelGGL((k_fwd_sm<G, CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_fwd_tasks<G, CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
 }
 
 // persistent wavefronts for a packed launch: as many as fit on every CU at once (registers and LDS
@@ -702,8 +707,11 @@ static uint32_t fwd_waves(oic_ctx *c, uint64_t max_tasks) {
   const int sm = (G < 64 && c->tune.fwd_loop == 0) ? 1 : 0;
   int per_cu = cached[sm];
   if (per_cu < 1) {
-    const hipError_t e = sm ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_sm<G, CAP, STATS>, OIC_WAVE, 0)
-                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_tasks<G, CAP, STATS>, OIC_WAVE, 0);
+    hipError_t e = hipSuccess;
+    if constexpr (G < 64) {
+      if (sm) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_sm<G, CAP, STATS>, OIC_WAVE, 0);
+    }
+    if (!sm) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_tasks<G, CAP, STATS>, OIC_WAVE, 0);
     if (e != hipSuccess || per_cu < 1) { (void)hipGetLastError(); per_cu = 16; }
     cached[sm] = per_cu;
   }
@@ -790,7 +798,7 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
   ALLOC(c, c->b_stats, 64);
   HIPCHK(c, hipMemsetAsync(c->b_cache.p, 0, H, c->st));
   const unsigned g = (unsigned)((np + 255) / 256);
-  hipLaunchKernelGGL(k_probe_tasks, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->p_nodes.as<int4>(), c->b_probe.as<FwdTask>());
+  hipLaunchKernelGGL(k_probe_tasks, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>());
   FwdArgs FA;
   FA.R = dev_reads(c);
   FA.tb = dev_tables(c);
@@ -905,7 +913,7 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, (uint32_t *)tst, (uint64_t)8, 0u);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->st, (uint32_t *)tst, (uint64_t)2, 0xffffffffu);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->st, (uint32_t *)(tst + 2), (uint64_t)2, 0xffffffffu);
-  hipLaunchKernelGGL(k_probe_tasks, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->p_nodes.as<int4>(), c->b_probe.as<FwdTask>());
+  hipLaunchKernelGGL(k_probe_tasks, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>());
   FwdArgs FA;
   FA.R = dev_reads(c);
   FA.tb = dev_tables(c);

@@ -643,9 +643,18 @@ __device__ __forceinline__ bool sm_row(SMDP &s, SMRes &res, const uint32_t *__re
   return false;
 }
 
-// five wavefronts per SIMD (at most 102 VGPRs): the occupancy its LDS use allows at 8 x 64
+// Wavefronts per SIMD the kernel's LDS allows (160 KB per CU over 4 SIMDs, rounded up), at most 5:
+// registers are held to that occupancy (5: at most 102 VGPRs), no further.
+template <int G, int CAP>
+constexpr int sm_waves_per_eu() {
+  constexpr int lds = (OIC_WAVE / G) * (2 * CAP * 4 + 2 * OIC_SEQ_WIN * 4 + 4);
+  constexpr int w = (163840 / lds + 3) / 4;
+  return w < 1 ? 1 : (w > 5 ? 5 : w);
+}
+
 template <int G, int CAP, bool STATS>
-__global__ void __launch_bounds__(OIC_WAVE) __attribute__((amdgpu_waves_per_eu(5, 8))) k_fwd_sm(FwdArgs A) {
+__global__ void __launch_bounds__(OIC_WAVE) __attribute__((amdgpu_waves_per_eu(sm_waves_per_eu<G, CAP>(), 8)))
+k_fwd_sm(FwdArgs A) {
   constexpr int NG = OIC_WAVE / G;
   __shared__ int32_t tile[NG][2][CAP];
   __shared__ uint32_t seqA[NG][OIC_SEQ_WIN], seqT[NG][OIC_SEQ_WIN];
@@ -799,7 +808,7 @@ __global__ void __launch_bounds__(OIC_WAVE) __attribute__((amdgpu_waves_per_eu(5
 }
 
 // probe tasks: each pair's first-longest node (Process_String_Overlaps.C:424-431)
-__global__ void k_probe_tasks(const PairRec *__restrict__ pairs, uint64_t np, const int4 *__restrict__ nodes, FwdTask *__restrict__ tasks) {
+__global__ void k_probe_tasks(const PairRec *__restrict__ pairs, uint64_t np, FwdTask *__restrict__ tasks) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   tasks[p].pair = (uint32_t)p;
