@@ -84,16 +84,46 @@ __device__ __forceinline__ uint32_t mism16(const SeqV &A, int64_t ia, const SeqV
   return x;
 }
 
-// number of equal bases (with 'n' matching anything) from A[ia..], T[it..], at most rem
+// equal bases (with 'n' matching anything) among the 64 from A[ia..], T[it..]: five words of each
+// loaded together (each strand carries 4 guard words past its last base, so they exist)
+__device__ __forceinline__ int32_t match64(const SeqV &A, int64_t ia, const SeqV &T, int64_t it) {
+  const int64_t pa = A.b0 + ia, pt = T.b0 + it;
+  const uint32_t *wa = A.w + (pa >> 4), *wt = T.w + (pt >> 4);
+  const uint32_t sa = (uint32_t)(pa & 15) * 2u, st = (uint32_t)(pt & 15) * 2u;
+  uint32_t a[5], b[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    a[k] = wa[k];
+    b[k] = wt[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    uint32_t x = __builtin_amdgcn_alignbit(a[k + 1], a[k], sa) ^ __builtin_amdgcn_alignbit(b[k + 1], b[k], st);
+    x = (x | (x >> 1)) & 0x55555555u;
+    if (A.nm) x &= ~nwin16(A.nm, pa + 16 * k);
+    if (T.nm) x &= ~nwin16(T.nm, pt + 16 * k);
+    if (x) return 16 * k + (int32_t)(__builtin_ctz(x) >> 1);
+  }
+  return 64;
+}
+
+// number of equal bases (with 'n' matching anything) from A[ia..], T[it..], at most rem: a 16-base
+// window first (most slides end in it), then 64-base windows (long matches: one memory round trip
+// per 64 bases)
 __device__ __forceinline__ int32_t lane_slide(const SeqV &A, int64_t ia, const SeqV &T, int64_t it, int32_t rem) {
-  int32_t adv = 0;
+  if (rem <= 0) return 0;
+  const uint32_t x = mism16(A, ia, T, it);
+  const int32_t k = x ? (int32_t)(__builtin_ctz(x) >> 1) : 16;
+  if (k >= rem) return rem;
+  if (k < 16) return k;
+  int32_t adv = 16;
+  rem -= 16;
   while (rem > 0) {
-    const uint32_t x = mism16(A, ia + adv, T, it + adv);
-    const int32_t k = x ? (int32_t)(__builtin_ctz(x) >> 1) : 16;
-    if (k >= rem) return adv + rem;
-    adv += k;
-    if (k < 16) return adv;
-    rem -= 16;
+    const int32_t k64 = match64(A, ia + adv, T, it + adv);
+    if (k64 >= rem) return adv + rem;
+    adv += k64;
+    if (k64 < 64) return adv;
+    rem -= 64;
   }
   return adv;
 }
