@@ -80,6 +80,8 @@ struct oic_ctx {
   oic::Tables T;
   std::string err;
   hipStream_t st = nullptr;
+  hipStream_t st2 = nullptr;      // second stream: the 512-hit list level beside the 128-hit one
+  hipEvent_t evl[2] = {};
   int ncu = 0;
 
   DevBuf d_eb, d_ml, d_minkmers;
@@ -98,7 +100,7 @@ struct oic_ctx {
   DevBuf s_k1, s_v1, s_k2, s_v2, s_scan, s_flag, s_idx, s_small, s_rp;
   // search
   DevBuf g_cnt, g_off, g_pbase, g_searched, g_sdsum, g_phi, g_pval;
-  DevBuf l_scr, l_done, l_big, l_cnt;
+  DevBuf l_scr, l_done, l_big, l_cnt, l_crowd;
   std::vector<uint64_t> skip;           // -k skip k-mers, both strands, sorted, unique
   DevBuf d_skip, x_tscreen, g_sscreen;
   oic_tuning tune{};
@@ -220,6 +222,8 @@ extern "C" int oic_create(const oic_params *p, oic_ctx **out) {
   hipEventCreate(&c->ev0);
   hipEventCreate(&c->ev1);
   for (auto &e : c->evf) hipEventCreate(&e);
+  if (hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess) { delete c; return OIC_EHIP; }
+  for (auto &e : c->evl) hipEventCreateWithFlags(&e, hipEventDisableTiming);
   c->tune.fwd_graph = 1;
   c->T.build(p->max_erate, p->align_noise);
   // overlapInCore.C:400-401
@@ -260,6 +264,10 @@ extern "C" void oic_destroy(oic_ctx *c) {
   if (c->ev1) hipEventDestroy(c->ev1);
   for (auto &e : c->evf)
     if (e) hipEventDestroy(e);
+  if (c->st2) hipStreamSynchronize(c->st2);
+  for (auto &e : c->evl)
+    if (e) hipEventDestroy(e);
+  if (c->st2) hipStreamDestroy(c->st2);
   if (c->st) hipStreamDestroy(c->st);
   delete c;
 }
@@ -1135,12 +1143,18 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     LS.fl = (uint8_t *)(b + 8 * H);
     // pairs with many hits: one wavefront each; the rest (and any handed back): one thread each
     ALLOC(c, c->l_done, np);
-    ALLOC(c, c->l_big, np * 4 * (LW_BUCKETS + 1));
+    ALLOC(c, c->l_big, np * 4 * (LW_BUCKETS + 2));
+    ALLOC(c, c->l_crowd, np);
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->st, c->l_crowd.as<uint8_t>(), np);
+    hipLaunchKernelGGL(k_crowded_pairs, dim3((unsigned)((H + 255) / 256)), dim3(256), 0, c->st, hk, hv, c->s_flag.as<uint32_t>(),
+                       c->s_idx.as<uint64_t>(), H, c->l_crowd.as<uint8_t>());
     ALLOC(c, c->l_cnt, 64);
     HIPCHK(c, hipMemsetAsync(c->l_cnt.p, 0, 64, c->st));
     uint32_t *cnt = c->l_cnt.as<uint32_t>();
+    uint32_t *lists = c->l_big.as<uint32_t>();
     hipLaunchKernelGGL(k_big_select, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np,
-                       (uint64_t)c->tune.list_wave_min_hits, c->l_done.as<uint8_t>(), c->l_big.as<uint32_t>(), cnt, cnt + 12);
+                       (uint64_t)c->tune.list_wave_min_hits, c->l_done.as<uint8_t>(), lists, cnt, cnt + 12,
+                       c->l_crowd.as<uint8_t>(), lists + LW_BUCKETS * np, cnt + 8);
     BigListArgs BA;
     BA.pairs = c->p_pairs.as<PairRec>();
     BA.big = c->l_big.as<uint32_t>();
@@ -1155,19 +1169,35 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     BA.tmp = c->p_tmp.as<int4>();
     BA.nodes = c->p_nodes.as<int4>();
     BA.K = K;
-    // first level: up to 128 hits per offset; pairs with a more crowded offset go to the 512-hit level
-    BA.up = c->l_big.as<uint32_t>() + LW_BUCKETS * np;
-    BA.nup = cnt + 8;
     const uint64_t ncu = (uint64_t)std::max(c->ncu, 1);
-    hipLaunchKernelGGL(k_build_lists_wave<LW_CAP_FIRST>, dim3((unsigned)std::min<uint64_t>(np, 16 * ncu)), dim3(64), 0, c->st, BA);
+    // crowded pairs (an offset of more than 128 hits): the 512-hit level, on the second stream,
+    // beside the first level; the two take disjoint pairs
     BigListArgs B2 = BA;
-    B2.big = BA.up;
-    B2.nbig = BA.nup;
+    B2.big = lists + LW_BUCKETS * np;
+    B2.nbig = cnt + 8;
     B2.bcnt = nullptr;
     B2.next = cnt + 9;
     B2.up = nullptr;
     B2.nup = nullptr;
-    hipLaunchKernelGGL(k_build_lists_wave<LW_CAP>, dim3((unsigned)std::min<uint64_t>(np, 4 * ncu)), dim3(64), 0, c->st, B2);
+    uint32_t ncrowd = 0;   // launched only when there are such pairs, with one wavefront each
+    HIPCHK(c, hipMemcpyAsync(&ncrowd, cnt + 8, 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (ncrowd) {
+      HIPCHK(c, hipEventRecord(c->evl[0], c->st));
+      HIPCHK(c, hipStreamWaitEvent(c->st2, c->evl[0], 0));
+      hipLaunchKernelGGL(k_build_lists_wave<LW_CAP>, dim3((unsigned)std::min<uint64_t>(ncrowd, 4 * ncu)), dim3(64), 0, c->st2, B2);
+      HIPCHK(c, hipEventRecord(c->evl[1], c->st2));
+    }
+    // first level: up to 128 hits per offset; a pair it still finds crowded runs at 512 hits after it
+    BA.up = lists + (LW_BUCKETS + 1) * np;
+    BA.nup = cnt + 10;
+    hipLaunchKernelGGL(k_build_lists_wave<LW_CAP_FIRST>, dim3((unsigned)std::min<uint64_t>(np, 16 * ncu)), dim3(64), 0, c->st, BA);
+    BigListArgs B3 = B2;
+    B3.big = BA.up;
+    B3.nbig = BA.nup;
+    B3.next = cnt + 11;
+    hipLaunchKernelGGL(k_build_lists_wave<LW_CAP>, dim3((unsigned)std::min<uint64_t>(np, 4 * ncu)), dim3(64), 0, c->st, B3);
+    if (ncrowd) HIPCHK(c, hipStreamWaitEvent(c->st, c->evl[1], 0));
     hipLaunchKernelGGL(k_build_lists, dim3((unsigned)((np + 127) / 128)), dim3(128), 0, c->st, c->p_pairs.as<PairRec>(), np, hv,
                        c->p_tmp.as<int4>(), c->p_nodes.as<int4>(), K, LS, c->l_done.as<uint8_t>(), BA.nnodes_total);
   }
@@ -1178,7 +1208,7 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->ctr.list_wave_pairs += lc[0];
     c->ctr.list_handed_back += lc[2];
-    c->ctr.list_second_level += lc[8];
+    c->ctr.list_second_level += lc[8] + lc[10];
     memcpy(&batch_nodes, lc + 4, 8);
     c->ctr.nodes += batch_nodes;
   }

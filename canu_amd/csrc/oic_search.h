@@ -577,17 +577,43 @@ struct BigListArgs {
   int32_t K;
 };
 
+__global__ void k_fill_u8(uint8_t *__restrict__ p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Pairs with an S offset of more than LW_CAP_FIRST hits (crowded[p] = 1).  Within a pair, hits are in
+// S-offset order, so an offset holds more than LW_CAP_FIRST of them exactly when some hit i and hit
+// i + LW_CAP_FIRST have the same pair key and the same offset.  keys are the pair-grouped hit keys,
+// flag marks each pair's first hit and idx is the exclusive scan of flag.
+__global__ void k_crowded_pairs(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ hval,
+                                const uint32_t *__restrict__ flag, const uint64_t *__restrict__ idx, uint64_t n,
+                                uint8_t *__restrict__ crowded) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + LW_CAP_FIRST >= n) return;
+  const uint64_t v = hval[i];
+  if ((v >> 42) <= LW_CAP_FIRST) return;   // a k-mer with no more occurrences cannot crowd an offset
+  const uint64_t j = i + LW_CAP_FIRST;
+  if (keys[j] == keys[i] && (hval[j] & 0x1fffff) == (v & 0x1fffff)) crowded[idx[i] + flag[i] - 1] = 1;
+}
+
 // Pairs for the wavefront builder, bucketed by hit count (>= 2^17, 2^14, 2^11, the rest) so that
 // the largest, whose offsets run one after another, start first and do not end the launch late.
+// Crowded pairs go to the 512-hit level's list (up) directly.
 #define LW_BUCKETS 4
 __global__ void k_big_select(const PairRec *__restrict__ pairs, uint64_t np, uint64_t min_hits, uint8_t *__restrict__ done,
-                             uint32_t *__restrict__ big, uint32_t *__restrict__ nbig, uint32_t *__restrict__ bcnt) {
+                             uint32_t *__restrict__ big, uint32_t *__restrict__ nbig, uint32_t *__restrict__ bcnt,
+                             const uint8_t *__restrict__ crowded, uint32_t *__restrict__ up, uint32_t *__restrict__ nup) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
   const uint64_t nh = pairs[p].he - pairs[p].hb;
   const bool b = nh >= min_hits;
   done[p] = b ? 1 : 0;
   if (!b) return;
+  if (crowded[p]) {
+    up[atomicAdd(nup, 1u)] = (uint32_t)p;
+    return;
+  }
   const int k = nh >= (1u << 17) ? 0 : nh >= (1u << 14) ? 1 : nh >= (1u << 11) ? 2 : 3;
   big[(uint64_t)k * np + atomicAdd(&bcnt[k], 1u)] = (uint32_t)p;
   atomicAdd(nbig, 1u);
