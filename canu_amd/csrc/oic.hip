@@ -1179,15 +1179,18 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     B2.next = cnt + 9;
     B2.up = nullptr;
     B2.nup = nullptr;
-    uint32_t ncrowd = 0;   // launched only when there are such pairs, with one wavefront each
+    // Second stream, beside the 128-hit level: the one-thread builder for the pairs no wavefront
+    // takes, then (when there are any) the crowded pairs at 512 hits, one wavefront each.
+    uint32_t ncrowd = 0;
     HIPCHK(c, hipMemcpyAsync(&ncrowd, cnt + 8, 4, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    if (ncrowd) {
-      HIPCHK(c, hipEventRecord(c->evl[0], c->st));
-      HIPCHK(c, hipStreamWaitEvent(c->st2, c->evl[0], 0));
+    HIPCHK(c, hipEventRecord(c->evl[0], c->st));
+    HIPCHK(c, hipStreamWaitEvent(c->st2, c->evl[0], 0));
+    hipLaunchKernelGGL(k_build_lists, dim3((unsigned)((np + 127) / 128)), dim3(128), 0, c->st2, c->p_pairs.as<PairRec>(), np, hv,
+                       c->p_tmp.as<int4>(), c->p_nodes.as<int4>(), K, LS, c->l_done.as<uint8_t>(), (uint8_t)0, BA.nnodes_total);
+    if (ncrowd)
       hipLaunchKernelGGL(k_build_lists_wave<LW_CAP>, dim3((unsigned)std::min<uint64_t>(ncrowd, 4 * ncu)), dim3(64), 0, c->st2, B2);
-      HIPCHK(c, hipEventRecord(c->evl[1], c->st2));
-    }
+    HIPCHK(c, hipEventRecord(c->evl[1], c->st2));
     // first level: up to 128 hits per offset; a pair it still finds crowded runs at 512 hits after it
     BA.up = lists + (LW_BUCKETS + 1) * np;
     BA.nup = cnt + 10;
@@ -1197,9 +1200,10 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     B3.nbig = BA.nup;
     B3.next = cnt + 11;
     hipLaunchKernelGGL(k_build_lists_wave<LW_CAP>, dim3((unsigned)std::min<uint64_t>(np, 4 * ncu)), dim3(64), 0, c->st, B3);
-    if (ncrowd) HIPCHK(c, hipStreamWaitEvent(c->st, c->evl[1], 0));
+    HIPCHK(c, hipStreamWaitEvent(c->st, c->evl[1], 0));
+    // pairs a wavefront handed back (an offset of more than 512 hits)
     hipLaunchKernelGGL(k_build_lists, dim3((unsigned)((np + 127) / 128)), dim3(128), 0, c->st, c->p_pairs.as<PairRec>(), np, hv,
-                       c->p_tmp.as<int4>(), c->p_nodes.as<int4>(), K, LS, c->l_done.as<uint8_t>(), BA.nnodes_total);
+                       c->p_tmp.as<int4>(), c->p_nodes.as<int4>(), K, LS, c->l_done.as<uint8_t>(), (uint8_t)2, BA.nnodes_total);
   }
   HIPCHK(c, hipGetLastError());
   {

@@ -418,11 +418,12 @@ struct ListScratch {
   uint8_t *fl;
 };
 
+// Pairs with done[p] == want: 0 for pairs never given to a wavefront, 2 for pairs a wavefront handed back.
 __global__ void k_build_lists(PairRec *__restrict__ pairs, uint64_t np, const uint64_t *__restrict__ hval, int4 *__restrict__ tmp,
-                              int4 *__restrict__ nodes, int32_t K, ListScratch X, const uint8_t *__restrict__ done,
+                              int4 *__restrict__ nodes, int32_t K, ListScratch X, const uint8_t *__restrict__ done, uint8_t want,
                               unsigned long long *__restrict__ nnodes_total) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np || done[p]) return;
+  if (p >= np || done[p] != want) return;
   PairRec pr = pairs[p];
   const uint64_t hb = pr.hb, he = pr.he;
   int4 *N = tmp + hb;                     // node records: x Offset, y Len, z Start
@@ -533,7 +534,7 @@ __global__ void k_build_lists(PairRec *__restrict__ pairs, uint64_t np, const ui
 // Dead nodes go to tmp in blocks as they die, each record tagged so that its final position
 // (live nodes, then blocks newest first, each in list order) is nlive + ndead - tag.  LDS holds up
 // to CAP hits of one offset: a pair with a more crowded offset goes on to the next level's list
-// (the 512-hit kernel), or, from the last level, back to the one-thread builder (done[p] = 0).
+// (the 512-hit kernel), or, from the last level, back to the one-thread builder (done[p] = 2).
 #define LW_CAP 512        // hits per offset of the last wavefront level
 #define LW_CAP_FIRST 128  // of the first: a quarter of the LDS, four times the wavefronts per CU
 
@@ -569,7 +570,7 @@ struct BigListArgs {
   uint32_t *handed_back;    // pairs left to the one-thread builder
   uint32_t *up;             // if not null: pairs too crowded for this level go here, for the next one
   uint32_t *nup;
-  uint8_t *done;            // per pair: 1 = list built by k_build_lists_wave
+  uint8_t *done;            // per pair: 0 = one-thread builder, 1 = a wavefront builds it, 2 = handed back
   unsigned long long *nnodes_total;
   const uint64_t *hval;
   int4 *tmp;
@@ -854,7 +855,7 @@ __global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
         if (A.up) {
           A.up[atomicAdd(A.nup, 1u)] = p;
         } else {
-          A.done[p] = 0;
+          A.done[p] = 2;
           atomicAdd(A.handed_back, 1u);
         }
       }
