@@ -65,17 +65,21 @@ __device__ __forceinline__ void wave_fence() {
 
 // A tail staged in LDS: its first OIC_SEQ_WIN words, from the word holding base 0 of the view.
 // Positions are 32-bit offsets from that word's first base; HBM is read only past the copy.
+// The copy is held through an LDS-typed pointer: a pointer that may address either LDS or HBM
+// (one chosen by a select) compiles to flat loads, which wait on both memory counters.
+typedef const __attribute__((address_space(3))) uint32_t *lds_words;
+
 struct Staged {
   int64_t w0;               // first staged word
   int32_t r0;               // base 0 of the view within word w0 (0..15)
-  const uint32_t *lds;
+  lds_words lds;
 };
 
 __device__ __forceinline__ Staged make_staged(int64_t b0, const uint32_t *lds) {
   Staged t;
   t.w0 = b0 >> 4;
   t.r0 = (int32_t)(b0 & 15);
-  t.lds = lds;
+  t.lds = (lds_words)lds;
   return t;
 }
 
@@ -460,28 +464,36 @@ __device__ __forceinline__ int32_t sm_match16(const SMDP &s, const uint32_t *__r
   return x ? (int32_t)(__builtin_ctz(x) >> 1) : 16;
 }
 
+// the five words from word wi of a staged tail: from its LDS copy when they lie inside it, else
+// from HBM (the packed reads carry 64 guard bases per strand and slack after the last one, so the
+// words past a view's end exist)
+__device__ __forceinline__ void words5_staged(const uint32_t *__restrict__ words, const Staged &t, int32_t wi, uint32_t (&w)[5]) {
+  if (wi + 4 < OIC_SEQ_WIN) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) w[k] = t.lds[wi + k];
+  } else {
+    const uint32_t *g = words + (t.w0 + wi);
+#pragma unroll
+    for (int k = 0; k < 5; k++) w[k] = g[k];
+  }
+}
+
 // 64 bases of both tails from positions pa and pb, compared: the number of leading equal bases (64
-// if all are).  Reads five words per tail (the packed reads carry 64 guard bases per strand and slack
-// after the last one, so the words past a view's end exist).
+// if all are).  The five words of each tail are loaded together.
 __device__ __forceinline__ int32_t sm_match64(const SMDP &s, const uint32_t *__restrict__ words,
                                               const uint32_t *__restrict__ nmask, int32_t pa, int32_t pb) {
   const int32_t qa = s.ta.r0 + pa, qb = s.tb.r0 + pb;
-  const int32_t wa = qa >> 4, wb = qb >> 4;
-  // the five words of each tail, from its LDS copy when they lie inside it
-  const uint32_t *A = (wa + 4 < OIC_SEQ_WIN) ? s.ta.lds + wa : words + (s.ta.w0 + wa);
-  const uint32_t *B = (wb + 4 < OIC_SEQ_WIN) ? s.tb.lds + wb : words + (s.tb.w0 + wb);
+  uint32_t a[5], b[5];
+  words5_staged(words, s.ta, qa >> 4, a);
+  words5_staged(words, s.tb, qb >> 4, b);
   const uint32_t sa = (uint32_t)(qa & 15) * 2u, sb = (uint32_t)(qb & 15) * 2u;
-  uint32_t a0 = A[0], b0 = B[0];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    const uint32_t a1 = A[k + 1], b1 = B[k + 1];
-    uint32_t x = __builtin_amdgcn_alignbit(a1, a0, sa) ^ __builtin_amdgcn_alignbit(b1, b0, sb);
+    uint32_t x = __builtin_amdgcn_alignbit(a[k + 1], a[k], sa) ^ __builtin_amdgcn_alignbit(b[k + 1], b[k], sb);
     x = (x | (x >> 1)) & 0x55555555u;
     if (s.nA) x &= ~nwin16(nmask, s.a_b0 + pa + 16 * k);
     if (s.nB) x &= ~nwin16(nmask, s.b_b0 + pb + 16 * k);
     if (x) return 16 * k + (int32_t)(__builtin_ctz(x) >> 1);
-    a0 = a1;
-    b0 = b1;
   }
   return 64;
 }
