@@ -102,6 +102,10 @@ inline void homopoly_compress(std::string &s) {
 
 class Store {
  public:
+  Store() = default;
+  Store(const Store &) = delete;              // owns the open blob files
+  Store &operator=(const Store &) = delete;
+
   std::string path;
   uint32_t nreads = 0;     // last read ID
   uint32_t nlibs = 0;
