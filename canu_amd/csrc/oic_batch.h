@@ -554,8 +554,9 @@ __device__ __forceinline__ bool sm_row(SMDP &s, SMRes &res, const uint32_t *__re
   if (s.Right - s.Left + 3 > CAP) { res.overflow = true; return true; }
   wave_fence();
   {
-    // sentinels at Left, Left - 1, Right, Right + 1 (prefixEditDistance-forward.C:147-153)
-    const int32_t sd = ((gl & 2) ? s.Right : s.Left) + ((gl & 1) ? ((gl & 2) ? 1 : -1) : 0);
+    // sentinels at Left - 1, Left, Right, Right + 1 (prefixEditDistance-forward.C:147-153), from
+    // lanes 0-3 (positions from gl itself: no per-lane constant is kept across rows)
+    const int32_t sd = gl + ((gl < 2) ? s.Left - 1 : s.Right - 2);
     if (gl < 4) pbuf[sd & MASK] = -2;
   }
   wave_fence();
