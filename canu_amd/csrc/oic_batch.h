@@ -308,7 +308,6 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
   constexpr int NG = OIC_WAVE / G;
   __shared__ int32_t tile[NG][2][CAP];
   __shared__ uint32_t seqA[NG][OIC_SEQ_WIN], seqT[NG][OIC_SEQ_WIN];
-  __shared__ uint32_t fetched[NG];
   const int lane = lane_id();
   const int g = lane / G, gl = lane % G;
   // per-launch totals per lane stay far below 2^32 (about 10^6 per group at the c3 workload)
@@ -320,10 +319,9 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
 
   while (true) {
     if (ti == tend) {                        // group-uniform: take the next FWD_CHUNK tasks
-      if (gl == 0) fetched[g] = atomicAdd(A.queue, (uint32_t)FWD_CHUNK);
-      wave_fence();
-      ti = fetched[g];
-      wave_fence();
+      uint32_t f = 0;
+      if (gl == 0) f = atomicAdd(A.queue, (uint32_t)FWD_CHUNK);
+      ti = (uint32_t)__shfl((int)f, g * G);  // from the group's first lane
       tend = ti + FWD_CHUNK < ntasks ? ti + FWD_CHUNK : ntasks;
     }
     if (ti >= ntasks) break;                 // group-uniform
@@ -657,10 +655,11 @@ __device__ __forceinline__ bool sm_row(SMDP &s, SMRes &res, const uint32_t *__re
 }
 
 // Wavefronts per SIMD the kernel's LDS allows (160 KB per CU over 4 SIMDs, rounded up), at most 5:
-// registers are held to that occupancy (5: at most 102 VGPRs), no further.
+// registers are held to that occupancy (5: at most 102 VGPRs), no further.  (8 x 64: 8 KiB per
+// wavefront, 20 per CU.)
 template <int G, int CAP>
 constexpr int sm_waves_per_eu() {
-  constexpr int lds = (OIC_WAVE / G) * (2 * CAP * 4 + 2 * OIC_SEQ_WIN * 4 + 4);
+  constexpr int lds = (OIC_WAVE / G) * (2 * CAP * 4 + 2 * OIC_SEQ_WIN * 4);
   constexpr int w = (163840 / lds + 3) / 4;
   return w < 1 ? 1 : (w > 5 ? 5 : w);
 }
@@ -671,7 +670,6 @@ k_fwd_sm(FwdArgs A) {
   constexpr int NG = OIC_WAVE / G;
   __shared__ int32_t tile[NG][2][CAP];
   __shared__ uint32_t seqA[NG][OIC_SEQ_WIN], seqT[NG][OIC_SEQ_WIN];
-  __shared__ uint32_t fetched[NG];
   const int lane = lane_id();
   const int g = lane / G, gl = lane % G;
   int32_t *pbuf = tile[g][0], *cbuf = tile[g][1];
@@ -747,10 +745,9 @@ k_fwd_sm(FwdArgs A) {
   while (true) {
     while (state == 0) {
       if (ti == tend) {                      // group-uniform: take the next FWD_CHUNK tasks
-        if (gl == 0) fetched[g] = atomicAdd(A.queue, (uint32_t)FWD_CHUNK);
-        wave_fence();
-        ti = fetched[g];
-        wave_fence();
+        uint32_t f = 0;
+        if (gl == 0) f = atomicAdd(A.queue, (uint32_t)FWD_CHUNK);
+        ti = (uint32_t)__shfl((int)f, g * G);  // from the group's first lane
         tend = ti + FWD_CHUNK < ntasks ? ti + FWD_CHUNK : ntasks;
       }
       if (ti >= ntasks) { state = 3; break; }
