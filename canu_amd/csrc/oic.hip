@@ -119,6 +119,10 @@ struct oic_ctx {
   uint32_t sc_waves = 0;
   oic_counters ctr{};
   bool have_model = false;
+  // records of the last oic_search / oic_run that did not fit the caller's sink, held until
+  // oic_fetch_pending or the next search
+  std::vector<oic_overlap> pending;
+  void *static_scratch = nullptr;       // this device's copy of the no-allocation entry point's scratch
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evf[12] = {};
 };
@@ -911,57 +915,68 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
   const unsigned g = (unsigned)((np + 255) / 256);
   const bool graph = c->tune.fwd_graph != 0;
   hipEvent_t *ev = graph ? nullptr : c->evf;
-  hipGraph_t gr = nullptr;
   if (graph) HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-  const uint64_t cw = (H + 4) / 4;
-  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((cw + 255) / 256)), dim3(256), 0, c->st, (uint32_t *)c->b_cache.p, cw, 0u);
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, cnt, (uint64_t)16, 0u);
-  unsigned long long *tst = stats + 48;      // wall-clock [start, end] of the two packed launches
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, (uint32_t *)stats, (uint64_t)(6 * 16), 0u);
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, (uint32_t *)tst, (uint64_t)8, 0u);
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->st, (uint32_t *)tst, (uint64_t)2, 0xffffffffu);
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->st, (uint32_t *)(tst + 2), (uint64_t)2, 0xffffffffu);
-  hipLaunchKernelGGL(k_probe_tasks, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>());
-  FwdArgs FA;
-  FA.R = dev_reads(c);
-  FA.tb = dev_tables(c);
-  FA.pairs = c->p_pairs.as<PairRec>();
-  FA.nodes = c->p_nodes.as<int4>();
-  FA.cache = c->b_cache.as<uint8_t>();
-  FA.ncells = c->b_ncells.as<uint32_t>();
-  FA.nexit = c->b_nexit.as<int32_t>();
-  FwdTask *ovf = c->b_ovf.as<FwdTask>();
-  // one pass: launch slot k (0 probe, 1 spec) uses queues/stats 3k..3k+2 and overflow counts 8+2k, 9+2k
-  auto pass = [&](int k, const FwdTask *tasks, uint64_t n, const uint32_t *n_dev, uint64_t cap, FwdTask *ov1, FwdTask *ov2) -> int {
-    FA.tasks = tasks; FA.ntasks = n; FA.ntasks_dev = n_dev;
-    FA.queue = cnt + 3 * k; FA.stats = stats + 8 * (3 * k); FA.ovf = ov1; FA.novf = cnt + 8 + 2 * k;
-    FA.tstamp = tst + 2 * k;
-    int rc = enqueue_fwd_narrow(c, FA, cap, ev ? ev[6 * k] : nullptr, ev ? ev[6 * k + 1] : nullptr);
+  // everything up to the end of the capture; an error inside it still ends the capture below, so
+  // the stream never stays in capture mode
+  auto enqueue_all = [&]() -> int {
+    const uint64_t cw = (H + 4) / 4;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((cw + 255) / 256)), dim3(256), 0, c->st, (uint32_t *)c->b_cache.p, cw, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, cnt, (uint64_t)16, 0u);
+    unsigned long long *tst = stats + 48;      // wall-clock [start, end] of the two packed launches
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, (uint32_t *)stats, (uint64_t)(6 * 16), 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, (uint32_t *)tst, (uint64_t)8, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->st, (uint32_t *)tst, (uint64_t)2, 0xffffffffu);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->st, (uint32_t *)(tst + 2), (uint64_t)2, 0xffffffffu);
+    hipLaunchKernelGGL(k_probe_tasks, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>());
+    FwdArgs FA;
+    FA.R = dev_reads(c);
+    FA.tb = dev_tables(c);
+    FA.pairs = c->p_pairs.as<PairRec>();
+    FA.nodes = c->p_nodes.as<int4>();
+    FA.cache = c->b_cache.as<uint8_t>();
+    FA.ncells = c->b_ncells.as<uint32_t>();
+    FA.nexit = c->b_nexit.as<int32_t>();
+    FwdTask *ovf = c->b_ovf.as<FwdTask>();
+    // one pass: launch slot k (0 probe, 1 spec) uses queues/stats 3k..3k+2 and overflow counts 8+2k, 9+2k
+    auto pass = [&](int k, const FwdTask *tasks, uint64_t n, const uint32_t *n_dev, uint64_t cap, FwdTask *ov1, FwdTask *ov2) -> int {
+      FA.tasks = tasks; FA.ntasks = n; FA.ntasks_dev = n_dev;
+      FA.queue = cnt + 3 * k; FA.stats = stats + 8 * (3 * k); FA.ovf = ov1; FA.novf = cnt + 8 + 2 * k;
+      FA.tstamp = tst + 2 * k;
+      int rc = enqueue_fwd_narrow(c, FA, cap, ev ? ev[6 * k] : nullptr, ev ? ev[6 * k + 1] : nullptr);
+      if (rc) return rc;
+      FA.tstamp = nullptr;
+      FA.tasks = ov1; FA.ntasks = 0; FA.ntasks_dev = cnt + 8 + 2 * k;
+      FA.queue = cnt + 3 * k + 1; FA.stats = stats + 8 * (3 * k + 1); FA.ovf = ov2; FA.novf = cnt + 9 + 2 * k;
+      if (c->tune.fwd_counters) enqueue_fwd<64, 512, true>(c, FA, cap, ev ? ev[6 * k + 2] : nullptr, ev ? ev[6 * k + 3] : nullptr);
+      else enqueue_fwd<64, 512, false>(c, FA, cap, ev ? ev[6 * k + 2] : nullptr, ev ? ev[6 * k + 3] : nullptr);
+      FA.tasks = ov2; FA.ntasks = 0; FA.ntasks_dev = cnt + 9 + 2 * k;
+      FA.queue = cnt + 3 * k + 2; FA.stats = stats + 8 * (3 * k + 2); FA.ovf = nullptr; FA.novf = nullptr;
+      if (c->tune.fwd_counters) enqueue_fwd<64, 8192, true>(c, FA, cap, ev ? ev[6 * k + 4] : nullptr, ev ? ev[6 * k + 5] : nullptr);
+      else enqueue_fwd<64, 8192, false>(c, FA, cap, ev ? ev[6 * k + 4] : nullptr, ev ? ev[6 * k + 5] : nullptr);
+      return OIC_OK;
+    };
+    int rc = pass(0, c->b_probe.as<FwdTask>(), np, nullptr, np, ovf, ovf + np);
     if (rc) return rc;
-    FA.tstamp = nullptr;
-    FA.tasks = ov1; FA.ntasks = 0; FA.ntasks_dev = cnt + 8 + 2 * k;
-    FA.queue = cnt + 3 * k + 1; FA.stats = stats + 8 * (3 * k + 1); FA.ovf = ov2; FA.novf = cnt + 9 + 2 * k;
-    if (c->tune.fwd_counters) enqueue_fwd<64, 512, true>(c, FA, cap, ev ? ev[6 * k + 2] : nullptr, ev ? ev[6 * k + 3] : nullptr);
-    else enqueue_fwd<64, 512, false>(c, FA, cap, ev ? ev[6 * k + 2] : nullptr, ev ? ev[6 * k + 3] : nullptr);
-    FA.tasks = ov2; FA.ntasks = 0; FA.ntasks_dev = cnt + 9 + 2 * k;
-    FA.queue = cnt + 3 * k + 2; FA.stats = stats + 8 * (3 * k + 2); FA.ovf = nullptr; FA.novf = nullptr;
-    if (c->tune.fwd_counters) enqueue_fwd<64, 8192, true>(c, FA, cap, ev ? ev[6 * k + 4] : nullptr, ev ? ev[6 * k + 5] : nullptr);
-    else enqueue_fwd<64, 8192, false>(c, FA, cap, ev ? ev[6 * k + 4] : nullptr, ev ? ev[6 * k + 5] : nullptr);
+    hipLaunchKernelGGL(k_spec_count, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
+                       c->b_cache.as<uint8_t>(), c->p_order.as<uint32_t>(), c->b_cnt.as<uint32_t>());
+    HIPCHK(c, scan_excl<uint32_t>(c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), np, c->s_scan.as<uint64_t>(), c->st, nullptr));
+    hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, c->st, c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), np, cnt + 6);
+    hipLaunchKernelGGL(k_spec_write, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
+                       c->p_order.as<uint32_t>(), c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), c->b_spec.as<FwdTask>());
+    rc = pass(1, c->b_spec.as<FwdTask>(), 0, cnt + 6, nnodes, ovf + 2 * np, ovf + 2 * np + nnodes);
+    if (rc) return rc;
+    HIPCHK(c, hipGetLastError());
     return OIC_OK;
   };
-  int rc = pass(0, c->b_probe.as<FwdTask>(), np, nullptr, np, ovf, ovf + np);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_spec_count, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
-                     c->b_cache.as<uint8_t>(), c->p_order.as<uint32_t>(), c->b_cnt.as<uint32_t>());
-  HIPCHK(c, scan_excl<uint32_t>(c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), np, c->s_scan.as<uint64_t>(), c->st, nullptr));
-  hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, c->st, c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), np, cnt + 6);
-  hipLaunchKernelGGL(k_spec_write, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
-                     c->p_order.as<uint32_t>(), c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), c->b_spec.as<FwdTask>());
-  rc = pass(1, c->b_spec.as<FwdTask>(), 0, cnt + 6, nnodes, ovf + 2 * np, ovf + 2 * np + nnodes);
-  if (rc) return rc;
-  HIPCHK(c, hipGetLastError());
+  const int erc = enqueue_all();
   if (graph) {
-    HIPCHK(c, hipStreamEndCapture(c->st, &gr));
+    hipGraph_t gr = nullptr;
+    const hipError_t ce = hipStreamEndCapture(c->st, &gr);
+    if (erc != OIC_OK || ce != hipSuccess) {
+      if (gr) hipGraphDestroy(gr);
+      (void)hipGetLastError();
+      return erc != OIC_OK ? erc : fail(c, OIC_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+    }
     hipGraphExec_t ge = nullptr;
     const hipError_t ie = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
     if (ie != hipSuccess) { hipGraphDestroy(gr); return fail(c, OIC_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ie)); }
@@ -974,6 +989,7 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
     if (le != hipSuccess || se != hipSuccess) return fail(c, OIC_EHIP, "forward-pass graph: %s", hipGetErrorString(le != hipSuccess ? le : se));
     c->ctr.graphs_launched++;
   } else {
+    if (erc != OIC_OK) return erc;
     HIPCHK(c, hipStreamSynchronize(c->st));
   }
   unsigned long long st[52];
@@ -1286,7 +1302,12 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     HIPCHK(c, hipMemcpyAsync(pout.data(), c->p_pout.p, np * sizeof(PairOut), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     for (uint64_t p = 0; p < np; p++)
-      if (pout[p].status == 1) return fail(c, OIC_ECAPACITY, "pair %lu exceeds the widest DP tile", (unsigned long)p);
+      if (pout[p].status == 1) {
+        PairRec pr;
+        HIPCHK(c, hipMemcpy(&pr, c->p_pairs.as<PairRec>() + p, sizeof pr, hipMemcpyDeviceToHost));
+        return fail(c, OIC_EUNSUPPORTED, "pair S %u (%s) x T %u needs a DP band wider than the widest tile (%d cells)", pr.s_id,
+                    pr.dir ? "reverse" : "forward", pr.t_id, kCapLarge);
+      }
   }
   c->ctr.ms_dp_kernel += ms;
   c->ctr.ms_extend += now_ms() - t2;
@@ -1312,16 +1333,20 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   c->ctr.extensions += stats[STAT_EXT];
   c->ctr.dp_calls += stats[STAT_DPCALLS];
   c->ctr.dp_bases += stats[STAT_BASES];
-  if (sink->count + nrec > sink->capacity || !sink->records) {
-    sink->records_needed = sink->count + nrec;
-    // keep counting so the caller learns the full size
-    sink->count += nrec;
-    return OIC_OK;
+  // records that fit go straight into the caller's sink, the rest are held by the context
+  // (oic_fetch_pending hands them over without searching again)
+  const uint64_t room = (sink->records && sink->capacity > sink->count) ? sink->capacity - sink->count : 0;
+  const uint64_t direct = c->pending.empty() ? std::min<uint64_t>(room, nrec) : 0;
+  if (direct)
+    HIPCHK(c, hipMemcpy(sink->records + sink->count, c->p_outrec.p, direct * sizeof(OutRec), hipMemcpyDeviceToHost));
+  if (nrec > direct) {
+    const size_t o = c->pending.size();
+    c->pending.resize(o + (nrec - direct));
+    HIPCHK(c, hipMemcpy(c->pending.data() + o, c->p_outrec.as<OutRec>() + direct, (nrec - direct) * sizeof(OutRec),
+                        hipMemcpyDeviceToHost));
   }
-  if (nrec)
-    HIPCHK(c, hipMemcpy(sink->records + sink->count, c->p_outrec.p, nrec * sizeof(OutRec), hipMemcpyDeviceToHost));
-  sink->count += nrec;
-  sink->records_needed = sink->count;
+  sink->count += direct;
+  sink->records_needed = sink->count + c->pending.size();
   if (c->tracing) {
     ALLOC(c, c->m_feat, np * MLP_IN * 4);
     hipLaunchKernelGGL(k_pair_features, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np,
@@ -1336,6 +1361,8 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   }
   return OIC_OK;
 }
+
+static_assert(sizeof(OutRec) == sizeof(oic_overlap), "device records are ovOverlap records");
 
 static int search_range(oic_ctx *c, uint32_t bgn, uint32_t end, oic_sink *sink) {
   if (!c->have_index) return fail(c, OIC_ESTATE, "no index built");
@@ -1367,17 +1394,40 @@ extern "C" int oic_search(oic_ctx *c, uint32_t bgn_ref, uint32_t end_ref, oic_si
   if (!c || !sink) return OIC_EINVAL;
   if (bgn_ref < 1) bgn_ref = 1;
   if (end_ref > c->nreads) end_ref = c->nreads;
+  c->pending.clear();
+  if (sink->count > sink->capacity) return fail(c, OIC_EINVAL, "sink count %lu exceeds its capacity %lu", (unsigned long)sink->count,
+                                                (unsigned long)sink->capacity);
   const uint32_t e = ref_end_searched(bgn_ref, end_ref, c->P.n_threads);
   const double t0 = now_ms();
   int rc = (e >= bgn_ref) ? search_range(c, bgn_ref, e, sink) : OIC_OK;
   c->ctr.ms_total += now_ms() - t0;
+  if (rc == OIC_OK && !c->pending.empty())
+    return fail(c, OIC_ECAPACITY, "sink holds %lu records, %lu needed", (unsigned long)sink->capacity,
+                (unsigned long)sink->records_needed);
   return rc;
+}
+
+extern "C" int oic_fetch_pending(oic_ctx *c, oic_sink *sink) {
+  if (!c || !sink) return OIC_EINVAL;
+  const uint64_t n = c->pending.size();
+  if (n == 0) return OIC_OK;
+  if (!sink->records || sink->count > sink->capacity || sink->capacity - sink->count < n) {
+    sink->records_needed = sink->count + n;
+    return fail(c, OIC_ECAPACITY, "%lu held records need a sink of %lu", (unsigned long)n, (unsigned long)sink->records_needed);
+  }
+  memcpy(sink->records + sink->count, c->pending.data(), n * sizeof(oic_overlap));
+  sink->count += n;
+  sink->records_needed = sink->count;
+  c->pending.clear();
+  c->pending.shrink_to_fit();
+  return OIC_OK;
 }
 
 extern "C" int oic_run(oic_ctx *c, uint32_t bgn_hash, uint32_t end_hash, uint32_t bgn_ref, uint32_t end_ref, oic_sink *sink) {
   if (!c || !sink) return OIC_EINVAL;
   const double t0 = now_ms();
   c->ctr = oic_counters{};
+  c->pending.clear();
   sink->count = 0;
   sink->records_needed = 0;
   memset(sink->stats, 0, sizeof sink->stats);
@@ -1401,8 +1451,9 @@ extern "C" int oic_run(oic_ctx *c, uint32_t bgn_hash, uint32_t end_hash, uint32_
     eh = end_hash;
   }
   c->ctr.ms_total = now_ms() - t0;
-  if (sink->count > sink->capacity) return fail(c, OIC_ECAPACITY, "sink holds %lu records, %lu needed", (unsigned long)sink->capacity,
-                                                 (unsigned long)sink->count);
+  if (!c->pending.empty())
+    return fail(c, OIC_ECAPACITY, "sink holds %lu records, %lu needed", (unsigned long)sink->capacity,
+                (unsigned long)sink->records_needed);
   return OIC_OK;
 }
 

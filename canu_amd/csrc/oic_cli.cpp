@@ -286,11 +286,11 @@ int main(int argc, char **argv) {
   sink.capacity = recs.size();
   rc = oic_run(ctx, bgn_hash, end_hash, bgn_ref, end_ref, &sink);
   if (rc == OIC_ECAPACITY) {
+    // the job ran to its end; the engine holds the records that did not fit
     recs.resize(sink.records_needed);
-    sink = {};
     sink.records = recs.data();
     sink.capacity = recs.size();
-    rc = oic_run(ctx, bgn_hash, end_hash, bgn_ref, end_ref, &sink);
+    rc = oic_fetch_pending(ctx, &sink);
   }
   if (rc != OIC_OK) die(oic_last_error(ctx));
 

@@ -27,7 +27,11 @@ static inline int make_order(int schedule, const PairRec *d_pairs, uint64_t np, 
   if (hipMemcpyAsync(pr.data(), d_pairs, np * sizeof(PairRec), hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
   if (hipStreamSynchronize(st) != hipSuccess) return -1;
   std::vector<double> cost(np);
-  if (schedule == 2 && pred) {
+  // a prediction that is not a finite number (a diverged or corrupt model) would make the
+  // comparison below an invalid ordering: the whole batch then falls back to the static estimate
+  bool use_pred = schedule == 2 && pred;
+  for (uint64_t p = 0; use_pred && p < np; p++) use_pred = std::isfinite(pred[p]);
+  if (use_pred) {
     for (uint64_t p = 0; p < np; p++) cost[p] = pred[p];
   } else {
     for (uint64_t p = 0; p < np; p++) cost[p] = static_cost(pr[p], len);

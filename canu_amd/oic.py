@@ -25,7 +25,7 @@ SCHED_PLAIN, SCHED_BATCHED, SCHED_LEARNED = 0, 1, 2
 
 EXPORTS = [
     "oic_params_default", "oic_create", "oic_destroy", "oic_last_error", "oic_load_reads", "oic_index_build",
-    "oic_search", "oic_run", "oic_get_counters", "oic_extend_batch", "oic_prefix_dp_batch", "oic_tables",
+    "oic_search", "oic_run", "oic_fetch_pending", "oic_get_counters", "oic_extend_batch", "oic_prefix_dp_batch", "oic_tables",
     "oic_dp_tables", "oic_set_tracing", "oic_get_traces", "oic_model_train", "oic_model_get", "oic_model_set",
     "oic_model_predict", "oic_model_grad", "oic_get_tuning", "oic_set_tuning", "oic_autotune", "oic_set_skip_kmers",
 ]
@@ -114,6 +114,7 @@ def lib():
         L.oic_index_build.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
         L.oic_search.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Sink)]
         L.oic_run.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(Sink)]
+        L.oic_fetch_pending.argtypes = [vp, ctypes.POINTER(Sink)]
         L.oic_get_counters.argtypes = [vp, ctypes.POINTER(Counters)]
         L.oic_extend_batch.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32]
         L.oic_prefix_dp_batch.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32]
@@ -207,23 +208,27 @@ class Engine:
         self._check(lib().oic_index_build(self._ctx, bgn, end, ctypes.byref(last)))
         return last.value
 
-    def _collect(self, call):
-        cap = 1 << 16
-        while True:
-            buf = np.zeros(cap, RECORD_DTYPE)
-            s = Sink(buf.ctypes.data, cap, 0, 0)
-            rc = call(s)
-            if rc == OIC_ECAPACITY or s.count > cap:
-                cap = int(max(s.records_needed, s.count)) + 1024
-                continue
-            self._check(rc)
-            return Result(buf[:s.count].copy(), list(s.stats), self.counters())
+    def _collect(self, call, cap=1 << 16):
+        """Run a search into a sink of cap records; when they do not fit, take the records the
+        engine holds into a larger buffer (oic_fetch_pending) instead of searching again."""
+        buf = np.zeros(cap, RECORD_DTYPE)
+        s = Sink(buf.ctypes.data, cap, 0, 0)
+        rc = call(s)
+        if rc == OIC_ECAPACITY:
+            need = int(s.records_needed)
+            big = np.zeros(need, RECORD_DTYPE)
+            big[:s.count] = buf[:s.count]
+            buf = big
+            s.records, s.capacity = buf.ctypes.data, need
+            rc = lib().oic_fetch_pending(self._ctx, ctypes.byref(s))
+        self._check(rc)
+        return Result(buf[:s.count].copy(), list(s.stats), self.counters())
 
-    def search(self, bgn_ref, end_ref):
-        return self._collect(lambda s: lib().oic_search(self._ctx, bgn_ref, end_ref, ctypes.byref(s)))
+    def search(self, bgn_ref, end_ref, cap=1 << 16):
+        return self._collect(lambda s: lib().oic_search(self._ctx, bgn_ref, end_ref, ctypes.byref(s)), cap)
 
-    def run(self, bgn_hash=1, end_hash=0xFFFFFFFF, bgn_ref=1, end_ref=0xFFFFFFFF):
-        return self._collect(lambda s: lib().oic_run(self._ctx, bgn_hash, end_hash, bgn_ref, end_ref, ctypes.byref(s)))
+    def run(self, bgn_hash=1, end_hash=0xFFFFFFFF, bgn_ref=1, end_ref=0xFFFFFFFF, cap=1 << 16):
+        return self._collect(lambda s: lib().oic_run(self._ctx, bgn_hash, end_hash, bgn_ref, end_ref, ctypes.byref(s)), cap)
 
     def counters(self):
         c = Counters()

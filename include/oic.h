@@ -43,7 +43,8 @@ extern "C" {
 #define OIC_ENOMEM       -3   /* device allocation failed */
 #define OIC_ESTATE       -4   /* call out of order (e.g. search before index) */
 #define OIC_ECAPACITY    -5   /* caller buffer too small; *_needed reports the size */
-#define OIC_EUNSUPPORTED -6   /* option the engine does not implement */
+#define OIC_EUNSUPPORTED -6   /* option or input size the engine does not implement (e.g. a DP band
+                                 wider than its widest tile) */
 
 typedef struct oic_ctx oic_ctx;
 
@@ -88,8 +89,11 @@ typedef struct oic_overlap {
   uint64_t dat[2];
 } oic_overlap;
 
-/* Caller-owned output sink.  If the records do not fit, the call fails with OIC_ECAPACITY,
- * sets records_needed, and may be repeated with a larger buffer. */
+/* Caller-owned output sink.  records[count ..] receive the call's records while they fit.  If
+ * they do not all fit, the search still runs to its end: stats and counters are complete, count
+ * is the number of records written (at most capacity), records_needed the total, the context
+ * holds the rest and the call returns OIC_ECAPACITY; oic_fetch_pending then appends them to a
+ * sink with room, without searching again.  records may be NULL (capacity 0) to learn the size. */
 typedef struct oic_sink {
   oic_overlap *records;
   uint64_t     capacity;
@@ -202,6 +206,13 @@ int oic_search(oic_ctx *ctx, uint32_t bgn_ref, uint32_t end_ref, oic_sink *sink)
 /* The whole overlapInCore job: hash-block loop over [bgn_hash, end_hash] x [bgn_ref, end_ref]. */
 int oic_run(oic_ctx *ctx, uint32_t bgn_hash, uint32_t end_hash, uint32_t bgn_ref, uint32_t end_ref,
             oic_sink *sink);
+
+/* After OIC_ECAPACITY from oic_search / oic_run: append the records the context holds to
+ * records[count ..] once capacity - count >= records_needed - count; OIC_ECAPACITY (with
+ * records_needed) while it is smaller.  The next oic_search / oic_run drops held records.  No
+ * reference counterpart: overlapInCore writes each thread's records to its ovFile as it goes
+ * (overlapInCore-Process_Overlaps.C:96-112). */
+int oic_fetch_pending(oic_ctx *ctx, oic_sink *sink);
 
 int oic_get_counters(const oic_ctx *ctx, oic_counters *out);
 

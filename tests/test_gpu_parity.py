@@ -402,3 +402,58 @@ def test_run_c1_full_bit_exact():
     res, ref = run_both(R, threads=8)
     assert len(res.records) > 50000
     assert_same(res, ref)
+
+
+def test_short_sink_keeps_records_for_fetch(c1s):
+    """A sink too small for a search: the call runs to its end and returns OIC_ECAPACITY with
+    records_needed; oic_fetch_pending hands over the held records without a second search, and
+    the counters are those of one search."""
+    import ctypes
+    eng = oic.Engine(kmer=22, maxerate=0.045, minlength=500, threads=4, schedule=oic.SCHED_BATCHED)
+    eng.load_reads(c1s)
+    full = eng.run()
+    n = len(full.records)
+    assert n > 100
+    eng.build_hash_index(1, c1s.n)
+    c0 = eng.counters()
+    small = np.zeros(7, oic.RECORD_DTYPE)
+    s = oic.Sink(small.ctypes.data, 7, 0, 0)
+    rc = oic.lib().oic_search(eng._ctx, 1, c1s.n, ctypes.byref(s))
+    assert rc == oic.OIC_ECAPACITY
+    assert s.count == 7 and s.records_needed == n
+    assert list(s.stats) == list(full.stats)
+    c1 = eng.counters()
+    # still too small: refused, nothing lost
+    rc = oic.lib().oic_fetch_pending(eng._ctx, ctypes.byref(s))
+    assert rc == oic.OIC_ECAPACITY and s.records_needed == n and s.count == 7
+    big = np.zeros(n, oic.RECORD_DTYPE)
+    big[:7] = small
+    s.records, s.capacity = big.ctypes.data, n
+    assert oic.lib().oic_fetch_pending(eng._ctx, ctypes.byref(s)) == oic.OIC_OK
+    assert s.count == n
+    assert eng.counters()["dp_cells"] == c1["dp_cells"]
+    assert c1["dp_cells"] - c0["dp_cells"] == full.counters["dp_cells"]
+    a, b = oic.sort_records(big), oic.sort_records(full.records)
+    assert all(np.array_equal(a[f], b[f]) for f in ("a_iid", "b_iid", "w0", "w1"))
+    # a NULL sink only learns the size
+    s2 = oic.Sink(None, 0, 0, 0)
+    assert oic.lib().oic_search(eng._ctx, 1, c1s.n, ctypes.byref(s2)) == oic.OIC_ECAPACITY
+    assert s2.count == 0 and s2.records_needed == n
+    # the Python wrapper takes the held records the same way
+    r = eng.search(1, c1s.n, cap=5)
+    a = oic.sort_records(r.records)
+    assert len(a) == n and all(np.array_equal(a[f], b[f]) for f in ("a_iid", "b_iid", "w0", "w1"))
+
+
+def test_learned_schedule_with_nan_model_falls_back(c2s):
+    """A model whose predictions are NaN cannot order pairs: the learned schedule falls back to the
+    static estimate for the batch and the output stays the oracle's."""
+    eng = oic.Engine(kmer=22, maxerate=0.045, minlength=500, threads=4, schedule=oic.SCHED_LEARNED)
+    w = np.full(oic.NPARAM, np.nan, np.float32)
+    eng.set_model(w, np.zeros(oic.NFEAT, np.float32), np.ones(oic.NFEAT, np.float32))
+    assert np.isnan(eng.predict(np.ones((3, oic.NFEAT), np.float32))).all()
+    eng.load_reads(c2s)
+    res = eng.run()
+    ref = oracle.run(c2s, kmer=22, maxErate=0.045, minOlap=500, nThreads=4, ompThreads=16)
+    assert_same(res, ref)
+    assert res.counters["dp_cells"] == ref["cells_used"]
