@@ -108,6 +108,7 @@ class Reads:
     pos: np.ndarray        # uint64 [n+1] genome start of the template
     strand: np.ndarray     # uint8  [n+1]
     genome_len: int
+    tlen: np.ndarray = None  # uint32 [n+1] template length on the genome (the read covers [pos, pos + tlen))
 
     @property
     def n(self):
@@ -121,7 +122,8 @@ class Reads:
         """The first n reads as their own 1-based set."""
         end = int(self.start[n] + self.length[n] + 1)
         return Reads(self.bases[:end].copy(), self.start[:n + 1].copy(), self.length[:n + 1].copy(),
-                     self.pos[:n + 1].copy(), self.strand[:n + 1].copy(), self.genome_len)
+                     self.pos[:n + 1].copy(), self.strand[:n + 1].copy(), self.genome_len,
+                     None if self.tlen is None else self.tlen[:n + 1].copy())
 
 
 _lib = None
@@ -151,10 +153,11 @@ def simulate(cfg: SimConfig) -> Reads:
         length = np.ctypeslib.as_array(r.len, (n + 1,)).copy()
         pos = np.ctypeslib.as_array(r.pos, (n + 1,)).copy()
         strand = np.ctypeslib.as_array(r.strand, (n + 1,)).copy()
+        tlen = np.ctypeslib.as_array(r.tlen, (n + 1,)).copy()
         glen = int(r.genome_len)
     finally:
         lib.rs_free(ctypes.byref(r))
-    return Reads(bases, start, length, pos, strand, glen)
+    return Reads(bases, start, length, pos, strand, glen, tlen)
 
 
 def from_strings(seqs):

@@ -737,6 +737,49 @@ typedef struct oic_index {
 
 static inline uint64_t occ_pack(uint32_t tl, uint32_t off) { return ((uint64_t)tl << 32) | off; }
 
+/* Mark_Skip_Kmers / Hash_Mark_Empty / Mark_Screened_Ends_Chain (Build_Hash_Index.C:98-257): each
+ * skip k-mer and then its reverse complement is marked Empty.  One present in the block marks
+ * the ends of the hash reads holding it within HOPELESS_MATCH of an end as screened (first time
+ * only); an absent one is added as an Empty entry when the hopeless check is on. */
+static void mark_skip_kmers(const oo_params *P, oic_index *X) {
+  const uint32_t k = P->kmer;
+  X->tscreen = (uint8_t *)calloc(X->nstr ? X->nstr : 1, 1);
+  kmap_init(&X->empty, 1024);
+  for (uint64_t q = 0; q < P->nSkip; q++) {
+    char km[64], rc[64];
+    memcpy(km, P->skipKmers + q * k, k);
+    for (uint32_t i = 0; i < k; i++) {
+      const char c = (char)tolower((unsigned char)km[i]);
+      rc[k - 1 - i] = (c == 'a') ? 't' : (c == 'c') ? 'g' : (c == 'g') ? 'c' : (c == 't') ? 'a' : c;
+    }
+    for (int pass = 0; pass < 2; pass++) {
+      const char *s = pass ? rc : km;
+      uint64_t key = 0;
+      int ok = 1;
+      for (uint32_t i = 0; i < k; i++) {
+        const int c = code_of[(unsigned char)s[i]];
+        if (c < 0) { ok = 0; break; }
+        key = (key << 2) | (uint64_t)c;
+      }
+      if (!ok) continue;
+      const int64_t h = kmap_find(&X->map, key);
+      if (h >= 0 && X->map.cnt[h] > 0) {
+        if (kmap_find(&X->empty, key) < 0) {
+          const uint64_t f = X->map.first[h], n = X->map.cnt[h];
+          for (uint64_t o = 0; o < n; o++) {
+            const uint32_t tl = (uint32_t)(X->occ[f + o] >> 32), off = (uint32_t)X->occ[f + o];
+            if (off < HOPELESS_MATCH) X->tscreen[tl] |= 1;
+            if ((int64_t)X->strLen[tl] - off - k + 1 < HOPELESS_MATCH) X->tscreen[tl] |= 2;
+          }
+        }
+        int nw; kmap_insert(&X->empty, key, &nw);
+      } else if (P->hopeless) {
+        int nw; kmap_insert(&X->empty, key, &nw);
+      }
+    }
+  }
+}
+
 /* Build_Hash_Index.C:414-631 restated: returns the last hash ID considered (curID - 1). */
 static uint32_t build_index(const oo_params *P, const char *bases, const uint64_t *start,
                             const uint32_t *len, uint32_t bgnID, uint32_t endID, oic_index *X) {
@@ -813,45 +856,147 @@ static uint32_t build_index(const oo_params *P, const char *bases, const uint64_
   free(fill);
   (void)nkmers;
 
-  /* Mark_Skip_Kmers / Hash_Mark_Empty / Mark_Screened_Ends_Chain (Build_Hash_Index.C:98-257): each
-   * skip k-mer and then its reverse complement is marked Empty.  One present in the block marks
-   * the ends of the hash reads holding it within HOPELESS_MATCH of an end as screened (first time
-   * only); an absent one is added as an Empty entry when the hopeless check is on. */
-  X->tscreen = (uint8_t *)calloc(X->nstr ? X->nstr : 1, 1);
-  kmap_init(&X->empty, 1024);
+  mark_skip_kmers(P, X);
+  return X->endID;
+}
+
+static void reverse_complement(char *s, int len);
+
+/* rolling keys of every window of s[0..L) whose k bases are all ACGT, as build_index and
+ * Find_Overlaps roll them, inserted into Q */
+static void insert_windows(kmap *Q, const char *s, uint32_t L, uint32_t k) {
+  const uint64_t kmask = (k == 32) ? ~0ULL : (((uint64_t)1 << (2 * k)) - 1);
+  uint64_t key = 0; uint32_t valid = 0;
+  for (uint32_t i = 0; i < L; i++) {
+    int c = code_of[(unsigned char)s[i]];
+    if (c < 0) { valid = 0; key = 0; continue; }
+    key = ((key << 2) | (uint64_t)c) & kmask;
+    if (++valid >= k) { int nw; kmap_insert(Q, key, &nw); }
+  }
+}
+
+typedef struct occ_run { uint64_t *v; uint64_t n, cap; } occ_run;   /* (slot, packed occurrence) pairs */
+
+static void occ_push(occ_run *r, uint64_t slot, uint64_t occ) {
+  if (r->n + 2 > r->cap) {
+    r->cap = r->cap ? 2 * r->cap : 1 << 16;
+    r->v = (uint64_t *)realloc(r->v, sizeof(uint64_t) * r->cap);
+  }
+  r->v[r->n++] = slot;
+  r->v[r->n++] = occ;
+}
+
+/* Checker-side shortcut for small ref ranges against large hash ranges (no reference counterpart).
+ * Find_Overlaps looks up only the k-mers of the ref reads it searches (both orientations) and
+ * Mark_Skip_Kmers only the skip k-mers (both orientations), so an index holding just those keys,
+ * each with its complete occurrence list in the order build_index gives it, yields the same
+ * search.  The hash reads are scanned by nthreads threads in contiguous ranges.  Block limits: the
+ * block must be the whole range [bgnID, endID], which holds when every window counted as a distinct
+ * k-mer stays under the load limit and the data stays under the length limit at every read
+ * (build_index's loop condition); otherwise -1 is returned and nothing is built. */
+static int64_t build_index_restricted(const oo_params *P, const char *bases, const uint64_t *start, const uint32_t *len,
+                                      uint32_t bgnID, uint32_t endID, uint32_t refBgn, int64_t refEnd, int nthreads,
+                                      oic_index *X) {
+  const uint32_t k = P->kmer;
+  const uint64_t hash_entry_limit = (uint64_t)(P->hashLoad * (double)((uint64_t)1 << P->hashBits) * ENTRIES_PER_BUCKET);
+  const uint64_t maxData = P->hashDataLen + OO_MAX_READLEN;
+  const uint64_t kmask = (k == 32) ? ~0ULL : (((uint64_t)1 << (2 * k)) - 1);
+  uint64_t total_len = 0, windows = 0;
+  for (uint32_t id = bgnID; id <= endID; id++) {
+    if (!(total_len < maxData && windows < hash_entry_limit)) return -1;
+    const uint32_t L = len[id];
+    if ((int64_t)L < P->minOlap) continue;
+    total_len += L + 1;
+    windows += L >= k ? L - k + 1 : 0;
+  }
+
+  memset(X, 0, sizeof(*X));
+  X->bgnID = bgnID;
+  X->endID = endID;
+  X->nstr = endID - bgnID + 1;
+  X->strStart = (int64_t *)malloc(sizeof(int64_t) * X->nstr);
+  X->strLen = (int32_t *)calloc(X->nstr, sizeof(int32_t));
+  X->lower = (char *)malloc(total_len + 1);
+  uint64_t off = 0;
+  for (uint32_t t = 0; t < X->nstr; t++) {
+    const uint32_t L = len[bgnID + t];
+    X->strStart[t] = -1;
+    if ((int64_t)L < P->minOlap) continue;
+    X->strStart[t] = (int64_t)off;
+    X->strLen[t] = (int32_t)L;
+    off += L + 1;
+  }
+
+  /* the query keys */
+  kmap_init(&X->map, 1 << 16);
+  char *buf = (char *)malloc(OO_MAX_READLEN + 2);
+  for (int64_t fi = refBgn; fi <= refEnd; fi++) {
+    const uint32_t L = len[fi];
+    if ((int64_t)L < P->minOlap) continue;
+    const char *s = bases + start[fi];
+    for (uint32_t i = 0; i < L; i++) buf[i] = (char)tolower((unsigned char)s[i]);
+    insert_windows(&X->map, buf, L, k);
+    reverse_complement(buf, (int)L);
+    insert_windows(&X->map, buf, L, k);
+  }
+  free(buf);
   for (uint64_t q = 0; q < P->nSkip; q++) {
-    char km[64], rc[64];
-    memcpy(km, P->skipKmers + q * k, k);
-    for (uint32_t i = 0; i < k; i++) {
-      const char c = (char)tolower((unsigned char)km[i]);
-      rc[k - 1 - i] = (c == 'a') ? 't' : (c == 'c') ? 'g' : (c == 'g') ? 'c' : (c == 't') ? 'a' : c;
-    }
-    for (int pass = 0; pass < 2; pass++) {
-      const char *s = pass ? rc : km;
-      uint64_t key = 0;
-      int ok = 1;
-      for (uint32_t i = 0; i < k; i++) {
-        const int c = code_of[(unsigned char)s[i]];
-        if (c < 0) { ok = 0; break; }
-        key = (key << 2) | (uint64_t)c;
-      }
-      if (!ok) continue;
-      const int64_t h = kmap_find(&X->map, key);
-      if (h >= 0) {
-        if (kmap_find(&X->empty, key) < 0) {
-          const uint64_t f = X->map.first[h], n = X->map.cnt[h];
-          for (uint64_t o = 0; o < n; o++) {
-            const uint32_t tl = (uint32_t)(X->occ[f + o] >> 32), off = (uint32_t)X->occ[f + o];
-            if (off < HOPELESS_MATCH) X->tscreen[tl] |= 1;
-            if ((int64_t)X->strLen[tl] - off - k + 1 < HOPELESS_MATCH) X->tscreen[tl] |= 2;
-          }
+    char km[64];
+    for (uint32_t i = 0; i < k; i++) km[i] = (char)tolower((unsigned char)P->skipKmers[q * k + i]);
+    insert_windows(&X->map, km, k, k);
+    reverse_complement(km, (int)k);
+    insert_windows(&X->map, km, k, k);
+  }
+
+  /* one pass over the hash reads: lowercase copy and the occurrences of query keys, each thread's
+   * range in ascending (T, offset) order */
+  if (nthreads < 1) nthreads = 1;
+  occ_run *runs = (occ_run *)calloc((size_t)nthreads, sizeof(occ_run));
+#pragma omp parallel num_threads(nthreads)
+  {
+    const int th = omp_get_thread_num(), nth = omp_get_num_threads();
+    const uint32_t t0 = (uint32_t)((uint64_t)X->nstr * th / nth), t1 = (uint32_t)((uint64_t)X->nstr * (th + 1) / nth);
+    occ_run *r = runs + th;
+    for (uint32_t t = t0; t < t1; t++) {
+      if (X->strStart[t] < 0) continue;
+      const uint32_t L = (uint32_t)X->strLen[t];
+      const char *s = bases + start[bgnID + t];
+      char *dst = X->lower + X->strStart[t];
+      uint64_t key = 0; uint32_t valid = 0;
+      for (uint32_t i = 0; i < L; i++) {
+        const char ch = (char)tolower((unsigned char)s[i]);
+        dst[i] = ch;
+        const int c = code_of[(unsigned char)ch];
+        if (c < 0) { valid = 0; key = 0; continue; }
+        key = ((key << 2) | (uint64_t)c) & kmask;
+        if (++valid >= k) {
+          const int64_t h = kmap_find(&X->map, key);
+          if (h >= 0) occ_push(r, (uint64_t)h, occ_pack(t, i + 1 - k));
         }
-        int nw; kmap_insert(&X->empty, key, &nw);
-      } else if (P->hopeless) {
-        int nw; kmap_insert(&X->empty, key, &nw);
       }
+      dst[L] = 0;
     }
   }
+  /* lay the lists out as build_index does: counts, then each list filled backwards */
+  for (int th = 0; th < nthreads; th++)
+    for (uint64_t j = 0; j < runs[th].n; j += 2) X->map.cnt[runs[th].v[j]]++;
+  X->map.first = (uint64_t *)malloc(sizeof(uint64_t) * X->map.cap);
+  uint64_t acc = 0;
+  for (uint64_t h = 0; h < X->map.cap; h++) { X->map.first[h] = acc; acc += X->map.cnt[h]; }
+  X->nocc = acc;
+  X->occ = (uint64_t *)malloc(sizeof(uint64_t) * (acc ? acc : 1));
+  uint32_t *fill = (uint32_t *)calloc(X->map.cap, sizeof(uint32_t));
+  for (int th = 0; th < nthreads; th++) {
+    for (uint64_t j = 0; j < runs[th].n; j += 2) {
+      const uint64_t h = runs[th].v[j];
+      X->occ[X->map.first[h] + X->map.cnt[h] - 1 - fill[h]++] = runs[th].v[j + 1];
+    }
+    free(runs[th].v);
+  }
+  free(runs);
+  free(fill);
+
+  mark_skip_kmers(P, X);
   return X->endID;
 }
 
@@ -1635,7 +1780,18 @@ int oo_run(const oo_params *Pin, const char *bases, const uint64_t *start, const
     if (endHashID > P.endHash) endHashID = P.endHash;
     oic_index X;
     double t0 = now_s();
-    endHashID = build_index(&P, bases, start, len, bgnHashID, endHashID, &X);
+    if (P.restrictIndex) {
+      const int64_t e = build_index_restricted(&P, bases, start, len, bgnHashID, endHashID, P.bgnRef, refEnd, nomp, &X);
+      if (e < 0) {
+        for (int i = 0; i < nomp; i++) wa_free(WAs + i);
+        free(WAs);
+        oo_tables_free(tb);
+        return -2;
+      }
+      endHashID = (uint32_t)e;
+    } else {
+      endHashID = build_index(&P, bases, start, len, bgnHashID, endHashID, &X);
+    }
     out->t_index += now_s() - t0;
     if (out->nblocks < 64) out->hashEnd[out->nblocks] = endHashID;
     out->nblocks++;

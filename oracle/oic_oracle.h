@@ -94,6 +94,9 @@ typedef struct oo_params {
   const char *skipKmers;       /* -k <file>: nSkip k-mers of kmer ACGT letters each, back to back, in
                                   file order (parsed as Mark_Skip_Kmers does); NULL = no file */
   uint64_t  nSkip;
+  int32_t   restrictIndex;     /* checker shortcut, no reference counterpart: index only the k-mers the
+                                  searched ref reads and the skip k-mers hold (same search; the hash range
+                                  must fit one block, else oo_run returns -2) */
 } oo_params;
 
 void oo_params_default(oo_params *P);

@@ -23,7 +23,7 @@ class Params(ctypes.Structure):
         ("bgnHash", ctypes.c_uint32), ("endHash", ctypes.c_uint32),
         ("bgnRef", ctypes.c_uint32), ("endRef", ctypes.c_uint32),
         ("nThreads", ctypes.c_uint32), ("ompThreads", ctypes.c_uint32),
-        ("skipKmers", ctypes.c_char_p), ("nSkip", ctypes.c_uint64),
+        ("skipKmers", ctypes.c_char_p), ("nSkip", ctypes.c_uint64), ("restrictIndex", ctypes.c_int32),
     ]
 
 

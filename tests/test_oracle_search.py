@@ -238,3 +238,77 @@ def test_skip_kmers_remove_hits_both_strands(small):
         assert got[key]["n_hits"] == len(hits), key
         assert got[key]["n_nodes"] == len(runs(hits)), key
     assert out["hits"] < small[1]["hits"]
+
+
+def _same_run(a, b):
+    ra, rb = oracle.sort_records(a["records"]), oracle.sort_records(b["records"])
+    assert len(ra) == len(rb)
+    for f in ("a_iid", "b_iid", "w0", "w1"):
+        assert np.array_equal(ra[f], rb[f]), f
+    assert a["stats"] == b["stats"]
+    for k in ("cells", "cells_used", "bases", "dp_calls", "extensions", "hits", "pairs"):
+        assert a[k] == b[k], k
+
+
+@pytest.mark.parametrize("case", ["c3_mid_refs", "c3_low_refs_skip", "c5_partial", "c2_no_hopeless_skip"])
+def test_restricted_index_searches_like_the_full_index(case):
+    """The checker's restricted index (only the k-mers of the searched ref reads and the skip
+    k-mers) gives the full index's records, .stats and work counters."""
+    preset, glen, seed, kw = {
+        "c3_mid_refs": ("c3", 400_000, 41, dict(bgnRef=300, endRef=340)),
+        "c3_low_refs_skip": ("c3", 300_000, 42, dict(bgnRef=1, endRef=25)),
+        "c5_partial": ("c5", 300_000, 43, dict(bgnRef=100, endRef=130, partial=1)),
+        "c2_no_hopeless_skip": ("c2", 300_000, 44, dict(bgnRef=50, endRef=90, hopeless=0)),
+    }[case]
+    R = readsim.simulate(readsim.PRESETS[preset].scaled(glen, seed=seed))
+    rate, minolap = (0.01, 1000) if preset == "c5" else (0.045, 500)
+    base = dict(kmer=K, maxErate=rate, minOlap=minolap, hashBits=26, hashLoad=0.8, hashDataLen=int(2e9),
+                nThreads=4, ompThreads=4, collect_trace=False, **kw)
+    if "skip" in case:
+        # frequent k-mers of a few reads, plus one absent and one with an N
+        s = R.read(kw["endRef"]).upper().decode()
+        base["skip"] = [s[i:i + K] for i in (0, 40, 500, 1000)] + ["A" * K, "ACGTN" + "A" * (K - 5)]
+    full = oracle.run(R, **base)
+    part = oracle.run(R, restrictIndex=1, **base)
+    assert full["hash_end"] == part["hash_end"] == [R.n]
+    assert full["pairs"] > 0 and full["cells"] > 0
+    assert len(full["records"]) > 0 or preset != "c5"
+    _same_run(full, part)
+
+
+def test_restricted_index_refuses_more_than_one_block():
+    R = readsim.simulate(readsim.PRESETS["c1"].scaled(100_000, seed=45))
+    with pytest.raises(RuntimeError):
+        oracle.run(R, kmer=K, maxErate=0.045, minOlap=500, hashBits=10, hashLoad=0.6, bgnRef=1, endRef=5, restrictIndex=1)
+
+
+def test_record_orientation_and_hangs_match_truth():
+    """Output_Overlap's A/B choice, N/I/O orientation and hangs (overlapInCore-Output.C:71-150), read
+    back as ovOverlap's a_hang()/b_hang()/flipped (ovOverlap.H:162-166), describe where the two
+    reads lie on the simulated genome: B is reverse-complemented against A exactly when their
+    strands differ, and in A's frame B starts a_hang and ends b_hang bases after A does (read
+    coordinates drift from genome coordinates by the reads' own indels)."""
+    cfg = readsim.PRESETS["c1"].scaled(60_000, seed=23)
+    cfg.ln_mu, cfg.ln_sigma, cfg.min_len = 7.8, 0.35, 800
+    R = readsim.simulate(cfg)
+    out = oracle.run(R, kmer=K, maxErate=0.045, minOlap=500, nThreads=4, ompThreads=4)
+    recs = out["records"]
+    f = decode(recs)
+    ah = f["ahg5"].astype(np.int64) - f["bhg5"].astype(np.int64)
+    bh = f["bhg3"].astype(np.int64) - f["ahg3"].astype(np.int64)
+    pos, tl, st = R.pos.astype(np.int64), R.tlen.astype(np.int64), R.strand.astype(np.int64)
+    kinds = set()
+    for k in range(len(recs)):
+        a, b = int(recs["a_iid"][k]), int(recs["b_iid"][k])
+        assert int(f["flipped"][k]) == int(st[a] != st[b]), k
+        if st[a] == 0:
+            ea, eb = pos[b] - pos[a], (pos[b] + tl[b]) - (pos[a] + tl[a])
+        else:
+            ea, eb = (pos[a] + tl[a]) - (pos[b] + tl[b]), pos[a] - pos[b]
+        assert abs(ah[k] - ea) <= 0.03 * max(abs(ea), 1) + 60, (k, a, b, ah[k], ea)
+        assert abs(bh[k] - eb) <= 0.03 * max(abs(eb), 1) + 60, (k, a, b, bh[k], eb)
+        kinds.add((int(f["flipped"][k]), int(np.sign(ah[k])), int(np.sign(bh[k])), int(a < b)))
+    # every orientation, both hang signs (dovetails and containments) and both ID orders occur
+    assert len(recs) > 300
+    assert {x[0] for x in kinds} == {0, 1} and {x[3] for x in kinds} == {0, 1}
+    assert {(x[1], x[2]) for x in kinds} >= {(1, 1), (-1, -1), (1, -1)}
