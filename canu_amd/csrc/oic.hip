@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -137,6 +138,8 @@ static int fail(oic_ctx *c, int code, const char *fmt, ...) {
   return code;
 }
 
+static int lookup_scratch(oic_ctx *c, int dev);
+
 #define HIPCHK(c, x)                                                                      \
   do {                                                                                    \
     hipError_t _e = (x);                                                                  \
@@ -206,6 +209,7 @@ extern "C" int oic_create(const oic_params *p, oic_ctx **out) {
     return OIC_EHIP;
   }
   if (hipSetDevice(p->device) != hipSuccess) { delete c; return OIC_EHIP; }
+  if (lookup_scratch(c, p->device) != OIC_OK) { delete c; return OIC_EHIP; }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, p->device) != hipSuccess) { delete c; return OIC_EHIP; }
   c->ncu = prop.multiProcessorCount;
@@ -1705,6 +1709,120 @@ extern "C" int oic_prefix_dp_batch(oic_ctx *c, const oic_dp_task *tasks, uint32_
   if (rc) return rc;
   HIPCHK(c, hipMemcpy(res, dr.p, (size_t)n * sizeof(oic_dp_result), hipMemcpyDeviceToHost));
   if (delta) HIPCHK(c, hipMemcpy(delta, dd.p, (size_t)n * stride * 4, hipMemcpyDeviceToHost));
+  return OIC_OK;
+}
+
+// ------------------------------------------------------------ DP units without allocation
+//
+// oic_prefix_dp_batch_noalloc runs oic_prefix_dp_batch's kernel on memory the library holds from
+// load time: one file-scope device array per GPU, carved into the task and result staging areas,
+// the delta area, the work-queue counter and the per-wavefront row store of the traceback.  Its
+// bounds are fixed at compile time (na::kMax* below).  Every call on a device shares that device's
+// array, so calls that overlap on one GPU race; each call clears the counter and the delta area it
+// uses before its kernel runs.
+
+namespace na {
+constexpr size_t up256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr uint32_t kMaxTasks = 16384;                  // tasks per call
+constexpr uint64_t kMaxDeltaInts = (uint64_t)1 << 20;  // tasks x delta_stride per call
+constexpr uint32_t kWaves = 64;                        // persistent wavefronts of the kernel
+constexpr int32_t kRows = 1024;                        // rows stored per DP: error limits up to kRows - 3
+constexpr int64_t kVals = (int64_t)1 << 17;            // band cells stored per DP (else status OIC_ECAPACITY)
+constexpr int32_t kMaxLimit = kRows - 3;
+constexpr size_t kQueueOff = 0;
+constexpr size_t kTaskOff = 256;
+constexpr size_t kResOff = kTaskOff + up256((size_t)kMaxTasks * sizeof(oic_dp_task));
+constexpr size_t kDeltaOff = kResOff + up256((size_t)kMaxTasks * sizeof(oic_dp_result));
+constexpr size_t kRowLOff = kDeltaOff + up256((size_t)kMaxDeltaInts * 4);
+constexpr size_t kRowROff = kRowLOff + up256((size_t)kWaves * kRows * 4);
+constexpr size_t kRowOffOff = kRowROff + up256((size_t)kWaves * kRows * 4);
+constexpr size_t kRowValOff = kRowOffOff + up256((size_t)kWaves * kRows * 8);
+constexpr size_t kDbufOff = kRowValOff + up256((size_t)kWaves * kVals * 4);
+constexpr size_t kLdOff = kDbufOff + up256((size_t)kWaves * (kRows + 4) * 4);
+constexpr size_t kRdOff = kLdOff + up256((size_t)kWaves * (kRows + 4) * 4);
+constexpr size_t kBytes = kRdOff + up256((size_t)kWaves * (kRows + 4) * 4);   // about 41 MB
+}  // namespace na
+
+__device__ __align__(256) uint8_t scratch[na::kBytes];
+
+// scratch's device address on each GPU, looked up once per device (oic_create)
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[256] = {};
+
+static int lookup_scratch(oic_ctx *c, int dev) {
+  if (dev < 0 || dev >= 256) return fail(c, OIC_EINVAL, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(c, OIC_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  c->static_scratch = g_scratch_ptr[dev];
+  return OIC_OK;
+}
+
+extern "C" int oic_prefix_dp_batch_noalloc(oic_ctx *c, const oic_dp_task *tasks, uint32_t n, oic_dp_result *res, int32_t *delta,
+                                           uint32_t stride) {
+  if (!c || (!tasks && n) || (!res && n)) return OIC_EINVAL;
+  if (n == 0) return OIC_OK;
+  if (!c->static_scratch) return fail(c, OIC_ESTATE, "no static scratch on this device");
+  if (n > na::kMaxTasks) return fail(c, OIC_EUNSUPPORTED, "%u tasks: at most %u per call without allocation", n, na::kMaxTasks);
+  if (delta && (uint64_t)n * stride > na::kMaxDeltaInts)
+    return fail(c, OIC_EUNSUPPORTED, "%u tasks x delta stride %u: at most %lu delta entries per call without allocation", n, stride,
+                (unsigned long)na::kMaxDeltaInts);
+  for (uint32_t i = 0; i < n; i++) {
+    const oic_dp_task &t = tasks[i];
+    if (t.a_id < 1 || t.a_id > c->nreads || t.b_id < 1 || t.b_id > c->nreads) return fail(c, OIC_EINVAL, "task %u: read id", i);
+    if (t.m < 0 || t.n < 0 || t.m > t.n || t.error_limit < 0 || t.error_limit >= (int32_t)c->T.max_errors)
+      return fail(c, OIC_EINVAL, "task %u: need 0 <= m <= n, 0 <= limit < max errors", i);
+    if (t.error_limit > na::kMaxLimit)
+      return fail(c, OIC_EUNSUPPORTED, "task %u: error limit %d, at most %d without allocation", i, t.error_limit, na::kMaxLimit);
+    const int32_t la = c->h_len[t.a_id], lb = c->h_len[t.b_id];
+    if (t.dir == 0) {
+      if (t.a_pos < 0 || t.a_pos + t.m > la || t.b_pos < 0 || t.b_pos + t.n > lb) return fail(c, OIC_EINVAL, "task %u: window", i);
+    } else {
+      if (t.a_pos >= la || t.a_pos - t.m + 1 < 0 || t.b_pos >= lb || t.b_pos - t.n + 1 < 0) return fail(c, OIC_EINVAL, "task %u: window", i);
+    }
+  }
+  uint8_t *base = (uint8_t *)c->static_scratch;
+  oic_dp_task *dt = (oic_dp_task *)(base + na::kTaskOff);
+  oic_dp_result *dr = (oic_dp_result *)(base + na::kResOff);
+  int32_t *dd = (int32_t *)(base + na::kDeltaOff);
+  uint32_t *queue = (uint32_t *)(base + na::kQueueOff);
+  HIPCHK(c, hipMemcpyAsync(dt, tasks, (size_t)n * sizeof(oic_dp_task), hipMemcpyHostToDevice, c->st));
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->st, queue, (uint64_t)1, 0u);
+  if (delta) {
+    const uint64_t nd = (uint64_t)n * stride;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, c->st, (uint32_t *)dd, nd, 0u);
+  }
+  BatchArgs A{};
+  A.R = dev_reads(c);
+  A.tb = dev_tables(c);
+  A.dtasks = dt;
+  A.n = n;
+  A.dres = dr;
+  A.delta = delta ? dd : nullptr;
+  A.delta_stride = stride;
+  A.queue = queue;
+  A.sc0.rowL = (int32_t *)(base + na::kRowLOff);
+  A.sc0.rowR = (int32_t *)(base + na::kRowROff);
+  A.sc0.rowOff = (int64_t *)(base + na::kRowOffOff);
+  A.sc0.rowVal = (int32_t *)(base + na::kRowValOff);
+  A.sc0.val_cap = na::kVals;
+  A.sc0.max_rows = na::kRows - 2;
+  A.sc0.dbuf = (int32_t *)(base + na::kDbufOff);
+  A.sc0.ldelta = (int32_t *)(base + na::kLdOff);
+  A.sc0.rdelta = (int32_t *)(base + na::kRdOff);
+  A.row_stride = na::kRows;
+  A.val_stride = na::kVals;
+  A.delta_stride_sc = na::kRows + 4;
+  const uint32_t waves = std::min<uint32_t>(n, na::kWaves);
+  hipLaunchKernelGGL(k_dp_batch<kCapLarge>, dim3(waves), dim3(OIC_WAVE), 0, c->st, A);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(res, dr, (size_t)n * sizeof(oic_dp_result), hipMemcpyDeviceToHost, c->st));
+  if (delta) HIPCHK(c, hipMemcpyAsync(delta, dd, (size_t)n * stride * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
   return OIC_OK;
 }
 

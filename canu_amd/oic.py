@@ -25,7 +25,8 @@ SCHED_PLAIN, SCHED_BATCHED, SCHED_LEARNED = 0, 1, 2
 
 EXPORTS = [
     "oic_params_default", "oic_create", "oic_destroy", "oic_last_error", "oic_load_reads", "oic_index_build",
-    "oic_search", "oic_run", "oic_fetch_pending", "oic_get_counters", "oic_extend_batch", "oic_prefix_dp_batch", "oic_tables",
+    "oic_search", "oic_run", "oic_fetch_pending", "oic_get_counters", "oic_extend_batch", "oic_prefix_dp_batch",
+    "oic_prefix_dp_batch_noalloc", "oic_tables",
     "oic_dp_tables", "oic_set_tracing", "oic_get_traces", "oic_model_train", "oic_model_get", "oic_model_set",
     "oic_model_predict", "oic_model_grad", "oic_get_tuning", "oic_set_tuning", "oic_autotune", "oic_set_skip_kmers",
 ]
@@ -118,6 +119,7 @@ def lib():
         L.oic_get_counters.argtypes = [vp, ctypes.POINTER(Counters)]
         L.oic_extend_batch.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32]
         L.oic_prefix_dp_batch.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32]
+        L.oic_prefix_dp_batch_noalloc.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32]
         L.oic_tables.argtypes = [vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
                                  ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)), ctypes.POINTER(ctypes.c_uint32)]
         L.oic_dp_tables.argtypes = [ctypes.c_double, ctypes.c_double, vp, vp, ctypes.POINTER(ctypes.c_uint32)]

@@ -260,6 +260,18 @@ typedef struct oic_dp_result {
 int oic_prefix_dp_batch(oic_ctx *ctx, const oic_dp_task *tasks, uint32_t n, oic_dp_result *results,
                         int32_t *delta, uint32_t delta_stride);
 
+/* oic_prefix_dp_batch without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same tasks, results and deltas as oic_prefix_dp_batch, on one
+ * device array the library holds on each GPU it runs on (about 41 MB, from library load to process
+ * end).  Limits per call: at most 16384 tasks, tasks x delta_stride at most 2^20, error limits at
+ * most 1021 (OIC_EUNSUPPORTED beyond, nothing run); a DP whose traceback needs more than 131072
+ * stored band cells gets status OIC_ECAPACITY, as one beyond oic_prefix_dp_batch's larger store
+ * does.  delta entries past each result's delta_len are zero.  All calls on one GPU share its array:
+ * calls that overlap on one GPU race. */
+int oic_prefix_dp_batch_noalloc(oic_ctx *ctx, const oic_dp_task *tasks, uint32_t n, oic_dp_result *results,
+                                int32_t *delta, uint32_t delta_stride);
+
 /* Host copies of the DP tables the context uses. error_bound has 2^21 entries,
  * match_limit has max_errors entries. */
 int oic_tables(const oic_ctx *ctx, const int32_t **error_bound, const int32_t **match_limit,
