@@ -39,9 +39,18 @@ WORKLOADS = {
                desc="100 Mbp genome with the c3 repeat mix, 30x lognormal 18k/4k, 0.2% errors mostly homopolymer "
                     "indels, homopolymer-compressed reads"),
 }
-for _w in WORKLOADS.values():
+# The CPU baseline's sample: the first ref reads of the instance.  On c3 it stops before read 22, a
+# read from a satellite array whose search alone takes 3.2e9 DP cells (about 280 s on one core).
+CPU_SAMPLE = {
+    "c1": ((1, 0), "c1 is searched whole (all reads against all)."),
+    "c2": ((1, 24), ""),
+    "c3": ((1, 21), "Read 22, from a satellite array, alone needs 3.2e9 DP cells (about 280 s on one core) and is left out."),
+    "c5": ((1, 16), ""),
+}
+for _k, _w in WORKLOADS.items():
     _w.setdefault("maxerate", 0.045)
     _w.setdefault("minlength", 500)
+    _w["cpu_refs"], _w["cpu_note"] = CPU_SAMPLE[_k]
 
 
 def log(*a):
@@ -60,19 +69,32 @@ def traffic_from_profiles(kernel):
     return None
 
 
-def cpu_baseline(W, genome_len, threads):
-    """The oracle ('port') on the same generator at a smaller genome, timed on host cores."""
-    from canu_amd import readsim
+def host_cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(W, R, refs, threads):
+    """The oracle ('port') on the same instance: ref reads refs[0]..refs[1] (both orientations)
+    against an index over every read, timed on host cores.  The oracle indexes only the k-mers
+    those ref reads hold (its restricted-index mode), which searches exactly like the full
+    index; the index build is not timed."""
     from oracle import oracle
-    preset = W["preset"]
-    cfg = readsim.PRESETS[preset].scaled(genome_len, seed=readsim.PRESETS[preset].seed)
-    R = readsim.simulate(cfg)
-    out = oracle.run(R, kmer=22, maxErate=W["maxerate"], minOlap=W["minlength"], nThreads=threads, ompThreads=threads)
+    b, e = refs
+    out = oracle.run(R, kmer=22, maxErate=W["maxerate"], minOlap=W["minlength"], hashBits=28, hashLoad=0.8,
+                     hashDataLen=8_000_000_000, bgnRef=b, endRef=e, nThreads=threads, ompThreads=threads,
+                     restrictIndex=1)
     return dict(value=out["cells_used"] / out["t_search"], unit="DP cells/s", cores=threads, kind="port",
-                sample=f"oracle on the {preset} generator at a {genome_len/1e6:g} Mbp genome ({R.n} reads, all-vs-all): "
-                       f"{out['cells_used']:.3g} used cells ({out['cells']:.3g} incl. unused reverse DPs) in "
-                       f"{out['t_search']:.1f} s of search (+{out['t_index']:.1f} s single-thread index build, not counted); "
-                       f"{len(out['records'])} overlaps",
+                cpu=host_cpu_model(),
+                sample=f"oracle (C restatement, OpenMP over ref reads) searching ref reads {b}-{e} of the same "
+                       f"{W['preset']} instance against all {R.n} reads: {out['cells_used']:.3g} DP cells in "
+                       f"{out['t_search']:.1f} s on {threads} threads ({out['pairs']} candidate pairs, "
+                       f"{len(out['records'])} overlaps); index build {out['t_index']:.1f} s, not counted. " + W["cpu_note"],
                 overlaps_per_s=len(out["records"]) / out["t_search"])
 
 
@@ -124,7 +146,6 @@ def main():
     ap.add_argument("--fwd-counters", action="store_true", help="count rows, lane slots and slide iterations of the packed DPs")
     ap.add_argument("--tune-refs", type=int, default=200, help="ref reads the autotuner times each launch shape on")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-genome", type=int, default=300_000)
     ap.add_argument("--train-genome", type=int, default=12_000_000, help="genome of the c3 instance the model learns from")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES between two device buffers (the reference kernel for HBM byte counters)")
@@ -193,15 +214,31 @@ def main():
     cells_s = d["dp_cells"] / tot
     olap_s = recs / tot
     steps = args.steps
-    if args.schedule == "plain":
-        kname, kms, ops, kbases, nl = "k_process_pairs<512>", d["ms_dp_kernel"], 4 * d["dp_cells"], d["dp_bases"], None
+    # the dominant DP kernel by measured device time: the packed forward kernel or Process_Matches
+    lanes, tile = tuning["fwd_lanes"], tuning["fwd_tile"]
+    stats_flag = "true" if tuning.get("fwd_counters") else "false"
+    if lanes == 1:
+        fwd_name = f"k_fwd_lane<{tile}, {stats_flag}>"
     else:
-        # the packed narrow-band forward-DP kernel does almost all of the DP work of the scheduled paths
-        kform = "k_fwd_tasks" if tuning.get("fwd_loop") else "k_fwd_sm"
-        kname = f"{kform}<{tuning['fwd_lanes']}, {tuning['fwd_tile']}, {'true' if tuning.get('fwd_counters') else 'false'}>"
-        kms, ops, kbases, nl = d["ms_fwd_narrow"], 4 * d["fwd_cells"], d["fwd_bases"], d["fwd_launches"]
-    ops += 2 * kbases / 16.0
+        fwd_name = f"{'k_fwd_tasks' if tuning.get('fwd_loop') else 'k_fwd_sm'}<{lanes}, {tile}, {stats_flag}>"
+    kernels = [dict(kernel="k_process_pairs<512>", ms=d["ms_pp_kernel"], cells=d["pp_cells"], bases=d["pp_bases"],
+                    launches=d["pp_launches"],
+                    timing="HIP events around each launch on its stream")]
+    if args.schedule != "plain":
+        kernels.append(dict(kernel=fwd_name, ms=d["ms_fwd_narrow"], cells=d["fwd_cells"], bases=d["fwd_bases"],
+                            launches=d["fwd_launches"],
+                            timing="HIP events around each forward-pass hipGraph on its stream; inside it, each packed "
+                                   "launch's span from its first wavefront's start to its last one's end (device wall clock)"))
+    dom = max(kernels, key=lambda k: k["ms"])
+    kname, kms, nl = dom["kernel"], dom["ms"], dom["launches"]
+    ops = 4 * dom["cells"] + 2 * dom["bases"] / 16.0
     achieved = ops / (kms / 1e3) / 1e12 if kms > 0 else 0.0
+    traffic = traffic_from_profiles(kname)
+    hbm = None
+    if traffic and nl:
+        gbs = traffic / (kms / nl / 1e3) / 1e9
+        hbm = {"achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+               "bytes_per_launch": traffic, "source": "profiles/pmc_traffic.txt (rocprofv3 FETCH_SIZE, WRITE_SIZE)"}
     line = {
         "metric": "overlaps/s + DP cells/s on 1 GPU, learned-binned vs plain, bit-exact",
         "value": cells_s, "unit": "DP cells/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
@@ -215,15 +252,13 @@ def main():
         "tuning": tuning,
         "cost_model": info,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT_TOPS, "unit": "TOPS",
-                     "frac": achieved / PEAK_INT_TOPS, "traffic": traffic_from_profiles(kname),
+                     "frac": achieved / PEAK_INT_TOPS, "traffic": traffic, "hbm": hbm,
                      "kernel": kname, "kernel_ms_per_step": kms / steps, "int_ops_per_step": ops / steps,
-                     "launches_per_step": (nl / steps) if nl else None,
-                     "timing": ("HIP events around each forward-pass hipGraph on its stream; inside it, each packed "
-                                "launch's span from its first wavefront's start to its last one's end (device wall clock)")
-                               if args.schedule != "plain" else "HIP events around each launch on its stream",
+                     "launches_per_step": (nl / steps) if nl else None, "timing": dom["timing"],
                      "graph_ms_per_step": d.get("ms_fwd_graph", 0.0) / steps,
                      "avg_launch_ms": (kms / nl) if nl else None,
-                     "int_ops_per_launch": (ops / nl) if nl else None},
+                     "int_ops_per_launch": (ops / nl) if nl else None,
+                     "kernels_ms_per_step": {k["kernel"]: k["ms"] / steps for k in kernels}},
         "counters_per_step": {k: (v / steps) for k, v in d.items()},
     }
     if not args.no_plain and args.schedule != "plain":
@@ -248,8 +283,9 @@ def main():
     eng.close()
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))
-        log(f"cpu baseline: oracle, {threads} threads")
-        line["cpu_baseline"] = cpu_baseline(W, args.cpu_genome, threads)
+        refs = W["cpu_refs"] if W["cpu_refs"][1] else (1, R.n)
+        log(f"cpu baseline: oracle on refs {refs[0]}-{refs[1]}, {threads} threads")
+        line["cpu_baseline"] = cpu_baseline(W, R, refs, threads)
     print(json.dumps(line), flush=True)
 
 

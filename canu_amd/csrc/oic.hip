@@ -23,6 +23,7 @@
 #include "oic_search.h"
 #include "oic_sched.h"
 #include "oic_batch.h"
+#include "oic_lane.h"
 #include "oic_mlp.h"
 #include <random>
 #include <numeric>
@@ -224,8 +225,8 @@ extern "C" int oic_create(const oic_params *p, oic_ctx **out) {
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) tot = (size_t)64 << 30;
     c->hit_cap = std::min<uint64_t>(1200000000ull, std::max<uint64_t>(50000000ull, (uint64_t)(0.45 * (double)tot) / kPassBytesPerHit));
   }
-  c->tune.fwd_lanes = 16;
-  c->tune.fwd_tile = 256;
+  c->tune.fwd_lanes = 8;
+  c->tune.fwd_tile = 64;
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) { delete c; return OIC_EHIP; }
   hipEventCreate(&c->ev0);
   hipEventCreate(&c->ev1);
@@ -358,13 +359,16 @@ extern "C" int oic_get_tuning(const oic_ctx *c, oic_tuning *out) {
 
 extern "C" int oic_set_tuning(oic_ctx *c, const oic_tuning *t) {
   if (!c || !t) return OIC_EINVAL;
-  const bool lanes_ok = t->fwd_lanes == 4 || t->fwd_lanes == 8 || t->fwd_lanes == 16 || t->fwd_lanes == 32;
-  const bool tile_ok = t->fwd_tile == 128 || t->fwd_tile == 256 || (t->fwd_tile == 512 && t->fwd_lanes != 4) ||
-                       ((t->fwd_tile == 64 || t->fwd_tile == 32) && (t->fwd_lanes == 8 || t->fwd_lanes == 16));
+  const bool lanes_ok = t->fwd_lanes == 1 || t->fwd_lanes == 4 || t->fwd_lanes == 8 || t->fwd_lanes == 16 || t->fwd_lanes == 32;
+  const bool tile_ok = t->fwd_lanes == 1
+                           ? (t->fwd_tile == 16 || t->fwd_tile == 32 || t->fwd_tile == 64)
+                           : (t->fwd_tile == 128 || t->fwd_tile == 256 || (t->fwd_tile == 512 && t->fwd_lanes != 4) ||
+                              ((t->fwd_tile == 64 || t->fwd_tile == 32) && (t->fwd_lanes == 8 || t->fwd_lanes == 16)));
   if (t->fwd_order > 1) return fail(c, OIC_EINVAL, "fwd_order must be 0 or 1");
   if (t->fwd_loop > 1) return fail(c, OIC_EINVAL, "fwd_loop must be 0 or 1");
   if (!lanes_ok || !tile_ok)
-    return fail(c, OIC_EINVAL, "fwd_lanes must be 4, 8, 16 or 32 and fwd_tile 32 or 64 (8 or 16 lanes), 128, 256 or 512 (not with 4 lanes)");
+    return fail(c, OIC_EINVAL, "fwd_lanes must be 1 (tile 16, 32 or 64), or 4, 8, 16 or 32 with fwd_tile 32 or 64 (8 or 16 lanes), "
+                               "128, 256 or 512 (not with 4 lanes)");
   c->tune = *t;
   return OIC_OK;
 }
@@ -668,6 +672,7 @@ static int run_pairs(oic_ctx *c, PairKernelArgs A, uint64_t n, uint32_t waves, i
   HIPCHK(c, hipEventRecord(c->ev0, c->st));
   hipLaunchKernelGGL(k_process_pairs<CAP>, dim3(waves), dim3(OIC_WAVE), 0, c->st, A);
   HIPCHK(c, hipGetLastError());
+  c->ctr.pp_launches++;
   HIPCHK(c, hipEventRecord(c->ev1, c->st));
   HIPCHK(c, hipEventSynchronize(c->ev1));
   float f = 0;
@@ -706,7 +711,10 @@ static int predict_pairs(oic_ctx *c, uint64_t np, std::vector<float> &pred) {
 // the packed kernel in the loop form the tuning selects (one DP per wavefront: the per-task loop)
 template <int G, int CAP, bool STATS>
 static void launch_fwd(oic_ctx *c, const FwdArgs &FA, uint32_t waves) {
-  if constexpr (G < 64) {
+  if constexpr (G == 1) {
+    hipLaunchKernelGGL((k_fwd_lane<CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
+    return;
+  } else if constexpr (G < 64) {
     if (c->tune.fwd_loop == 0) {
       hipLaunchKernelGGL((k_fwd_sm<G, CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
       return;
@@ -724,10 +732,14 @@ static uint32_t fwd_waves(oic_ctx *c, uint64_t max_tasks) {
   int per_cu = cached[sm];
   if (per_cu < 1) {
     hipError_t e = hipSuccess;
-    if constexpr (G < 64) {
-      if (sm) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_sm<G, CAP, STATS>, OIC_WAVE, 0);
+    if constexpr (G == 1) {
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_lane<CAP, STATS>, OIC_WAVE, 0);
+    } else {
+      if constexpr (G < 64) {
+        if (sm) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_sm<G, CAP, STATS>, OIC_WAVE, 0);
+      }
+      if (!sm) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_tasks<G, CAP, STATS>, OIC_WAVE, 0);
     }
-    if (!sm) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_tasks<G, CAP, STATS>, OIC_WAVE, 0);
     if (e != hipSuccess || per_cu < 1) { (void)hipGetLastError(); per_cu = 16; }
     cached[sm] = per_cu;
   }
@@ -773,6 +785,7 @@ static int run_fwd_narrow(oic_ctx *c, const FwdArgs &FA, uint64_t n, double *ms)
   const uint32_t g = c->tune.fwd_lanes, t = c->tune.fwd_tile;
 #define FWD_CASE(G_, T_) \
   if (g == G_ && t == T_) return c->tune.fwd_counters ? run_fwd<G_, T_, true>(c, FA, n, ms) : run_fwd<G_, T_, false>(c, FA, n, ms);
+  FWD_CASE(1, 16) FWD_CASE(1, 32) FWD_CASE(1, 64)
   FWD_CASE(4, 128) FWD_CASE(4, 256)
   FWD_CASE(8, 32) FWD_CASE(8, 64) FWD_CASE(8, 128) FWD_CASE(8, 256) FWD_CASE(8, 512)
   FWD_CASE(16, 32) FWD_CASE(16, 64) FWD_CASE(16, 128) FWD_CASE(16, 256) FWD_CASE(16, 512)
@@ -887,6 +900,7 @@ static int enqueue_fwd_narrow(oic_ctx *c, const FwdArgs &FA, uint64_t max_tasks,
     else enqueue_fwd<G_, T_, false>(c, FA, max_tasks, e0, e1);                                             \
     return OIC_OK;                                                                                         \
   }
+  FWD_CASE(1, 16) FWD_CASE(1, 32) FWD_CASE(1, 64)
   FWD_CASE(4, 128) FWD_CASE(4, 256)
   FWD_CASE(8, 32) FWD_CASE(8, 64) FWD_CASE(8, 128) FWD_CASE(8, 256) FWD_CASE(8, 512)
   FWD_CASE(16, 32) FWD_CASE(16, 64) FWD_CASE(16, 128) FWD_CASE(16, 256) FWD_CASE(16, 512)
@@ -1253,6 +1267,9 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     if (c->P.schedule == OIC_SCHED_LEARNED && c->have_model) {
       int rc = predict_pairs(c, np, pred);
       if (rc) return rc;
+      bool finite = true;
+      for (float v : pred) finite = finite && std::isfinite(v);
+      if (finite) c->ctr.learned_pairs += np;
     }
     int rc = make_order(c->P.schedule, c->p_pairs.as<PairRec>(), np, c->h_len.data(), pred.empty() ? nullptr : pred.data(), c->st, order);
     if (rc) return fail(c, OIC_EHIP, "scheduling failed");
@@ -1314,6 +1331,7 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
       }
   }
   c->ctr.ms_dp_kernel += ms;
+  c->ctr.ms_pp_kernel += ms;
   c->ctr.ms_extend += now_ms() - t2;
 
   // records
@@ -1337,6 +1355,8 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   c->ctr.extensions += stats[STAT_EXT];
   c->ctr.dp_calls += stats[STAT_DPCALLS];
   c->ctr.dp_bases += stats[STAT_BASES];
+  c->ctr.pp_cells += stats[STAT_PPCELLS];
+  c->ctr.pp_bases += stats[STAT_BASES];
   // records that fit go straight into the caller's sink, the rest are held by the context
   // (oic_fetch_pending hands them over without searching again)
   const uint64_t room = (sink->records && sink->capacity > sink->count) ? sink->capacity - sink->count : 0;
@@ -1496,54 +1516,66 @@ extern "C" int oic_autotune(oic_ctx *c, uint32_t bgn_ref, uint32_t end_ref, oic_
   double ms = 0;
   int rc = trial(cur, 1, &ms);   // first run pays for buffer growth; not compared
   if (rc) { c->ctr = saved; return rc; }
+  // Each knob in turn: the incumbent (the current setting) is timed first and a candidate replaces
+  // it only when it is kTuneMargin faster, so timing noise does not flip the choice between runs.
+  constexpr double kTuneMargin = 0.03;
+  auto better = [&](double x, double incumbent) { return x < incumbent * (1.0 - kTuneMargin); };
   if (c->P.schedule != OIC_SCHED_PLAIN && !c->P.partial) {
-    double bms = 1e300;
     oic_tuning b = cur;
-    for (uint32_t g : {4u, 8u, 16u, 32u})
-      for (uint32_t t : {32u, 64u, 128u, 256u, 512u}) {
-        if ((g == 4 && t == 512) || (t <= 64 && g != 8 && g != 16)) continue;
+    double bms = 0;
+    rc = trial(cur, 0, &bms);
+    if (rc) { c->ctr = saved; return rc; }
+    for (uint32_t g : {1u, 4u, 8u, 16u, 32u})
+      for (uint32_t t : {16u, 32u, 64u, 128u, 256u, 512u}) {
+        if (g == 1 ? t > 64 : (t == 16 || (g == 4 && t == 512) || (t <= 64 && g != 8 && g != 16))) continue;
+        if (g == cur.fwd_lanes && t == cur.fwd_tile) continue;
         oic_tuning x = cur;
         x.fwd_lanes = g;
         x.fwd_tile = t;
         rc = trial(x, 0, &ms);
         if (rc) { c->ctr = saved; return rc; }
-        if (ms < bms) { bms = ms; b = x; }
+        if (better(ms, bms)) { bms = ms; b = x; }
       }
     cur = b;
   }
-  if (c->P.schedule != OIC_SCHED_PLAIN && !c->P.partial) {
-    double bms = 1e300;
+  if (c->P.schedule != OIC_SCHED_PLAIN && !c->P.partial && cur.fwd_lanes > 1) {
+    // the loop form (one DP per lane has a single form)
     oic_tuning b = cur;
-    for (uint32_t l : {0u, 1u}) {
-      oic_tuning x = cur;
-      x.fwd_loop = l;
-      rc = trial(x, 0, &ms);
-      if (rc) { c->ctr = saved; return rc; }
-      if (ms < bms) { bms = ms; b = x; }
-    }
+    double bms = 0;
+    rc = trial(cur, 0, &bms);
+    if (rc) { c->ctr = saved; return rc; }
+    oic_tuning x = cur;
+    x.fwd_loop = 1 - cur.fwd_loop;
+    rc = trial(x, 0, &ms);
+    if (rc) { c->ctr = saved; return rc; }
+    if (better(ms, bms)) b = x;
     cur = b;
   }
   if (c->P.schedule != OIC_SCHED_PLAIN && !c->P.partial) {
-    double bms = 1e300;
+    // seed-length order runs without the per-batch graph: the whole extension stage is timed
     oic_tuning b = cur;
-    for (uint32_t o : {0u, 1u}) {   // seed-length order runs without the per-batch graph: whole stage timed
-      oic_tuning x = cur;
-      x.fwd_order = o;
-      rc = trial(x, 2, &ms);
-      if (rc) { c->ctr = saved; return rc; }
-      if (ms < bms) { bms = ms; b = x; }
-    }
+    double bms = 0;
+    rc = trial(cur, 2, &bms);
+    if (rc) { c->ctr = saved; return rc; }
+    oic_tuning x = cur;
+    x.fwd_order = 1 - cur.fwd_order;
+    rc = trial(x, 2, &ms);
+    if (rc) { c->ctr = saved; return rc; }
+    if (better(ms, bms)) b = x;
     cur = b;
   }
   {
-    double bms = 1e300;
     oic_tuning b = cur;
-    for (uint32_t h : {256u, 1024u, 4096u, 16384u}) {
+    double bms = 0;
+    rc = trial(cur, 1, &bms);
+    if (rc) { c->ctr = saved; return rc; }
+    for (uint32_t h : {256u, 1024u, 2048u, 4096u, 16384u}) {
+      if (h == cur.list_wave_min_hits) continue;
       oic_tuning x = cur;
       x.list_wave_min_hits = h;
       rc = trial(x, 1, &ms);
       if (rc) { c->ctr = saved; return rc; }
-      if (ms < bms) { bms = ms; b = x; }
+      if (better(ms, bms)) { bms = ms; b = x; }
     }
     cur = b;
   }

@@ -1217,6 +1217,7 @@ struct PairOut {
 #define STAT_EXT     10
 #define STAT_DPCALLS 11
 #define STAT_BASES   12
+#define STAT_PPCELLS 13   // cells of the DPs this kernel runs itself (full extensions)
 #define STAT_NSTATS  16
 
 struct PairKernelArgs {
@@ -1263,7 +1264,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_process_pairs(PairKernelArgs A) {
   const DPScratch sc = wave_scratch<CAP>(A, w);
   const SearchConst &C = A.C;
 
-  uint64_t cells = 0, slots = 0, lane_bases = 0;
+  uint64_t cells = 0, slots = 0, lane_bases = 0, own_cells = 0;
   uint32_t dpcalls = 0, exts = 0;
   while (true) {
     if (lane == 0) next_item = atomicAdd(A.queue, 1u);
@@ -1361,7 +1362,9 @@ __global__ void __launch_bounds__(OIC_WAVE) k_process_pairs(PairKernelArgs A) {
         continue;
       }
       const int4 lm = nodes[bi];
+      const uint64_t cx = cells;
       ExtOut x = wave_extend<CAP>(lm.z, lm.x, lm.y, S, Src, S_Len, T, Trc, T_Len, C, bufA, bufB, sc, cells, slots, dpcalls, lane_bases);
+      own_cells += cells - cx;
       exts++;
       po.n_ext++;
       if (x.overflow) { overflow = true; break; }
@@ -1488,6 +1491,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_process_pairs(PairKernelArgs A) {
   if (lane == 0) {
     atomicAdd(&A.stats[STAT_BASES], (unsigned long long)lane_bases);
     atomicAdd(&A.stats[STAT_CELLS], (unsigned long long)cells);
+    atomicAdd(&A.stats[STAT_PPCELLS], (unsigned long long)own_cells);
     atomicAdd(&A.stats[STAT_SLOTS], (unsigned long long)slots);
     atomicAdd(&A.stats[STAT_EXT], (unsigned long long)exts);
     atomicAdd(&A.stats[STAT_DPCALLS], (unsigned long long)dpcalls);

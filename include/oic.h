@@ -140,6 +140,11 @@ typedef struct oic_counters {
                                   or, inside a forward-pass graph, the span from its first wavefront's
                                   start to its last one's end (device wall clock) */
   double   ms_fwd_graph;       /* forward-pass graphs, HIP events around their launches */
+  double   ms_pp_kernel;       /* the Process_Matches kernel (k_process_pairs), HIP events around its launches */
+  uint64_t pp_cells;           /* DP cells that kernel computes itself (full extensions with row storage) */
+  uint64_t pp_bases;           /* bases it compares in their slides, plus one per cell */
+  uint64_t learned_pairs;      /* pairs the learned cost model ordered (OIC_SCHED_LEARNED with a model) */
+  uint64_t pp_launches;        /* launches of the Process_Matches kernel */
 } oic_counters;
 
 /* Launch-shape knobs.  No setting changes a result; the defaults suit MI355X.  No reference
@@ -147,11 +152,13 @@ typedef struct oic_counters {
 typedef struct oic_tuning {
   uint32_t list_wave_min_hits;  /* pairs with at least this many seed hits build their match list
                                    with one wavefront, smaller ones with one thread (default 2048) */
-  uint32_t fwd_lanes;           /* lanes per packed forward DP: 4, 8, 16 or 32 (64/fwd_lanes DPs
-                                   per wavefront; default 16) */
-  uint32_t fwd_tile;            /* band cells a packed forward DP holds in LDS: 32 or 64 (8 or 16 lanes),
-                                   128, 256 or 512; wider bands re-run one per wavefront, with a
-                                   512-cell tile and then an 8192-cell one (default 256) */
+  uint32_t fwd_lanes;           /* lanes per packed forward DP: 1 (64 DPs per wavefront, each lane
+                                   walking its own band cell by cell), or 4, 8, 16 or 32 (64/fwd_lanes
+                                   DPs per wavefront, rows spread over the group; default 8) */
+  uint32_t fwd_tile;            /* band cells a packed forward DP holds in LDS: 16, 32 or 64 with one
+                                   lane per DP; 32 or 64 (8 or 16 lanes), 128, 256 or 512 otherwise;
+                                   wider bands re-run one per wavefront, with a 512-cell tile and then
+                                   an 8192-cell one (default 64) */
   uint32_t fwd_order;          /* speculative DP tasks in pair order (0) or by seed length,
                                    longest first (1); the autotuner picks */
   uint32_t fwd_graph;           /* 1 (default): each search batch's probe and speculation passes run
@@ -176,9 +183,11 @@ typedef struct oic_tune_trial {
 } oic_tune_trial;
 
 /* Autotuner: searches ref reads [bgn_ref, end_ref] against the current index under each candidate
- * launch shape (forward DP: 4/8/16/32 lanes x 32..512-cell tile, then loop form, then task order; list builder
- * threshold 256..16384 hits), keeps the fastest, and reports the trials (up to cap) when trials is not null.
- * Counters are left as they were.  No reference counterpart. */
+ * launch shape (forward DP: one DP per lane with a 16..64-cell ring, or 4/8/16/32 lanes per DP x
+ * 32..512-cell tile; then loop form, then task order; list builder threshold 256..16384 hits).  Each
+ * knob starts from the current setting, which a candidate replaces only when it is at least 3 %
+ * faster.  Reports the trials (up to cap) when trials is not null.  Counters are left as they were.
+ * No reference counterpart. */
 int oic_autotune(oic_ctx *ctx, uint32_t bgn_ref, uint32_t end_ref, oic_tuning *best, oic_tune_trial *trials,
                  uint32_t cap, uint32_t *ntrials);
 

@@ -930,7 +930,9 @@ static int64_t build_index_restricted(const oo_params *P, const char *bases, con
   /* the query keys */
   kmap_init(&X->map, 1 << 16);
   char *buf = (char *)malloc(OO_MAX_READLEN + 2);
-  for (int64_t fi = refBgn; fi <= refEnd; fi++) {
+  const int64_t nq = P->nRefList ? (int64_t)P->nRefList : refEnd - (int64_t)refBgn + 1;
+  for (int64_t qi = 0; qi < nq; qi++) {
+    const int64_t fi = P->nRefList ? (int64_t)P->refList[qi] : (int64_t)refBgn + qi;
     const uint32_t L = len[fi];
     if ((int64_t)L < P->minOlap) continue;
     const char *s = bases + start[fi];
@@ -1802,8 +1804,10 @@ int oo_run(const oo_params *Pin, const char *bases, const uint64_t *start, const
     {
       work_area *WA = WAs + omp_get_thread_num();
       char *buf = (char *)malloc(OO_MAX_READLEN + 2);
-#pragma omp for schedule(dynamic, 4)
-      for (int64_t fi = P.bgnRef; fi <= refEnd; fi++) {
+      const int64_t nq = P.nRefList ? (int64_t)P.nRefList : refEnd - (int64_t)P.bgnRef + 1;
+#pragma omp for schedule(dynamic, 1)
+      for (int64_t qi = 0; qi < nq; qi++) {
+        const int64_t fi = P.nRefList ? (int64_t)P.refList[qi] : (int64_t)P.bgnRef + qi;
         uint32_t L = len[fi];
         if ((int64_t)L < P.minOlap) continue;
         const char *s = bases + start[fi];

@@ -97,6 +97,9 @@ typedef struct oo_params {
   int32_t   restrictIndex;     /* checker shortcut, no reference counterpart: index only the k-mers the
                                   searched ref reads and the skip k-mers hold (same search; the hash range
                                   must fit one block, else oo_run returns -2) */
+  const uint32_t *refList;     /* checker shortcut: when nRefList > 0, search exactly these ref IDs (in
+                                  place of bgnRef..endRef and the -t chunk rule) */
+  uint64_t  nRefList;
 } oo_params;
 
 void oo_params_default(oo_params *P);

@@ -24,6 +24,7 @@ class Params(ctypes.Structure):
         ("bgnRef", ctypes.c_uint32), ("endRef", ctypes.c_uint32),
         ("nThreads", ctypes.c_uint32), ("ompThreads", ctypes.c_uint32),
         ("skipKmers", ctypes.c_char_p), ("nSkip", ctypes.c_uint64), ("restrictIndex", ctypes.c_int32),
+        ("refList", ctypes.c_void_p), ("nRefList", ctypes.c_uint64),
     ]
 
 
@@ -257,7 +258,14 @@ def run(reads, collect_trace=False, **kw):
 
     Returns dict(records=structured array, stats=[8], counters...)."""
     skip = kw.pop("skip", None)
+    refs = kw.pop("refs", None)          # explicit ref IDs (searched exactly, no -t chunk rule)
     p = default_params(**kw)
+    if refs is not None:
+        ref_arr = np.ascontiguousarray(refs, np.uint32)
+        if len(ref_arr) == 0 or ref_arr.min() < 1 or ref_arr.max() > reads.n:
+            raise ValueError("ref IDs out of range")
+        p.refList = ref_arr.ctypes.data
+        p.nRefList = len(ref_arr)
     if skip:
         buf = b"".join(x.encode() if isinstance(x, str) else bytes(x) for x in skip)
         p.skipKmers = buf

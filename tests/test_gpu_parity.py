@@ -254,6 +254,14 @@ def test_forward_launch_shapes_identical(c3s):
     ref = oracle.run(c3s, kmer=22, maxErate=0.045, minOlap=500, nThreads=4, ompThreads=16)
     eng = oic.Engine(kmer=22, maxerate=0.045, minlength=500, threads=4, schedule=oic.SCHED_BATCHED)
     eng.load_reads(c3s)
+    for tile in (16, 32, 64):
+        # one DP per lane; the 16-cell ring sends many bands to the retry levels
+        eng.set_tuning(fwd_lanes=1, fwd_tile=tile, fwd_loop=0)
+        res = eng.run()
+        assert_same(res, ref)
+        assert res.counters["dp_cells"] == ref["cells_used"], (1, tile)
+        if tile == 16:
+            assert res.counters["overflow_retries"] > 0
     for loop in (0, 1):
         for lanes in (4, 8, 16, 32):
             for tile in (32, 64, 128, 256, 512):
@@ -276,6 +284,8 @@ def test_forward_launch_shapes_identical(c3s):
         assert (res.counters["graphs_launched"] > 0) == bool(graph)
     with pytest.raises(oic.OicError):
         eng.set_tuning(fwd_lanes=64)
+    with pytest.raises(oic.OicError):
+        eng.set_tuning(fwd_lanes=1, fwd_tile=128)
     with pytest.raises(oic.OicError):
         eng.set_tuning(fwd_loop=2)
 
@@ -302,8 +312,11 @@ def test_autotune_keeps_results(c3s):
     eng.load_reads(c3s)
     eng.build_hash_index(1, c3s.n)
     best, trials = eng.autotune(1, min(c3s.n, 60))
-    assert len(trials) == 1 + 15 + 2 + 2 + 4
-    assert best["fwd_lanes"] in (4, 8, 16, 32) and best["fwd_tile"] in (128, 256, 512)
+    # warm-up, the incumbent and 17 other forward shapes, the loop form (shapes of more than one lane
+    # per DP), the task order, the incumbent list threshold and 4 others
+    assert len(trials) == 1 + 18 + (2 if best["fwd_lanes"] > 1 else 0) + 2 + 5
+    shapes = {(t["fwd_lanes"], t["fwd_tile"]) for t in trials if t["stage"] == "fwd"}
+    assert len(shapes) == 18 and (best["fwd_lanes"], best["fwd_tile"]) in shapes
     assert eng.tuning() == best
     assert_same(eng.run(), ref)
 
@@ -457,3 +470,26 @@ def test_learned_schedule_with_nan_model_falls_back(c2s):
     ref = oracle.run(c2s, kmer=22, maxErate=0.045, minOlap=500, nThreads=4, ompThreads=16)
     assert_same(res, ref)
     assert res.counters["dp_cells"] == ref["cells_used"]
+
+
+def test_learned_schedule_with_trained_model_matches_oracle(c3s):
+    """The learned path end to end: the cost model is trained on the GPU on traces of a fresh
+    C3-generator instance (seed 4), then orders the pairs and packed DP tasks of a search of another
+    C3-generator instance.  Records, .stats and DP cells equal the oracle's; every pair was ordered by
+    the model's prediction."""
+    from canu_amd import learned
+    eng = oic.Engine(kmer=22, maxerate=0.045, minlength=500, threads=4, schedule=oic.SCHED_LEARNED)
+    info = learned.train_for(eng, genome_len=2_000_000, seed=4, epochs=8, max_traces=400_000)
+    print(f"\ncost model: {info}")
+    assert info["traces"] > 20_000
+    assert info["learned_spearman"] > info["static_spearman"]
+    w, mu, sd = eng.model()
+    assert np.isfinite(w).all()
+    eng.load_reads(c3s)
+    res = eng.run()
+    assert res.counters["pairs"] > 1000
+    assert res.counters["learned_pairs"] == res.counters["pairs"]
+    ref = oracle.run(c3s, kmer=22, maxErate=0.045, minOlap=500, nThreads=4, ompThreads=16)
+    assert_same(res, ref)
+    assert res.counters["dp_cells"] == ref["cells_used"]
+    assert res.counters["extensions"] == ref["extensions"]

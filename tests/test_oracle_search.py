@@ -312,3 +312,19 @@ def test_record_orientation_and_hangs_match_truth():
     assert len(recs) > 300
     assert {x[0] for x in kinds} == {0, 1} and {x[3] for x in kinds} == {0, 1}
     assert {(x[1], x[2]) for x in kinds} >= {(1, 1), (-1, -1), (1, -1)}
+
+
+def test_ref_list_searches_exactly_the_listed_reads():
+    """The checker's explicit ref list (restricted index) gives the union of single-read searches
+    (a range [r, r + 1] under -t 1 searches read r only: the chunk rule skips r + 1)."""
+    R = readsim.simulate(readsim.PRESETS["c3"].scaled(300_000, seed=47))
+    refs = [5, 9, 17, 40]
+    base = dict(kmer=K, maxErate=0.045, minOlap=500, hashBits=26, hashLoad=0.8, hashDataLen=int(2e9), nThreads=1,
+                ompThreads=4)
+    got = oracle.run(R, refs=refs, restrictIndex=1, **base)
+    parts = [oracle.run(R, bgnRef=r, endRef=r + 1, **base) for r in refs]
+    recs = np.concatenate([p["records"] for p in parts])
+    _same_run(got, dict(records=recs, stats=list(np.sum([p["stats"] for p in parts], axis=0)),
+                        **{k: sum(p[k] for p in parts) for k in ("cells", "cells_used", "bases", "dp_calls", "extensions",
+                                                                  "hits", "pairs")}))
+    assert got["pairs"] > 0
