@@ -189,7 +189,10 @@ def main():
     if args.schedule == "learned":
         from canu_amd import learned
         t0 = time.perf_counter()
-        info = learned.train_for(eng, engine_kw=kw, genome_len=args.train_genome)
+        # the workload's own generator with a fresh seed (c3: the configured cost model's training set)
+        glen = min(args.train_genome, readsim.PRESETS[W["preset"]].genome_len)
+        info = learned.train_for(eng, engine_kw=kw, genome_len=glen, preset=W["preset"])
+        info["trained_on"] = f"{W['preset']} generator, seed 4, {glen / 1e6:g} Mbp genome"
         log(f"cost model: {info} ({time.perf_counter() - t0:.1f} s)")
 
     first = 1

@@ -5,7 +5,6 @@
 // Ref reads are processed in batches sized to bound device memory; every stage of a batch runs
 // on the context's stream.
 #include <hip/hip_runtime.h>
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -72,7 +71,7 @@ int ceil_log2(uint64_t x) {
 }
 
 constexpr int kCapSmall = 512;
-constexpr uint64_t kPassBytesPerHit = 140;   // device bytes a search pass holds per seed hit (hits, sort, lists, nodes, cache)
+constexpr uint64_t kPassBytesPerHit = 140;   // device bytes a search pass holds per seed hit (hits, sort, lists, nodes, outcomes)
 constexpr int kCapLarge = 8192;
 
 }  // namespace
@@ -110,7 +109,7 @@ struct oic_ctx {
   int wallclock_khz = 0;                // rate of wall_clock64()
   DevBuf p_pairs, p_tmp, p_nodes, p_recs, p_pout, p_order, p_queue, p_stats, p_nout, p_noff, p_outrec, p_retry;
   DevBuf sc_rowL, sc_rowR, sc_rowOff, sc_rowVal, sc_dbuf, sc_ld, sc_rd;
-  DevBuf b_cache, b_ncells, b_nexit, b_probe, b_spec, b_cnt, b_off, b_stats, b_order, b_ovf, b_novf;
+  DevBuf b_outc, b_probe, b_spec, b_cnt, b_off, b_stats, b_order, b_ovf, b_novf;
   // learned cost model
   bool tracing = false;
   std::vector<float> tr_feat, tr_targ;
@@ -121,6 +120,7 @@ struct oic_ctx {
   uint32_t sc_waves = 0;
   oic_counters ctr{};
   bool have_model = false;
+  bool batch_pred = false;              // m_pred holds finite predictions for the current batch's pairs
   // records of the last oic_search / oic_run that did not fit the caller's sink, held until
   // oic_fetch_pending or the next search
   std::vector<oic_overlap> pending;
@@ -437,7 +437,7 @@ extern "C" int oic_load_reads(oic_ctx *c, const oic_reads *rd) {
 
 // ------------------------------------------------------------------------------------ index
 
-// alternate key / value buffers for sort_pairs_rocprim, and scan scratch, for n elements
+// alternate key / value buffers for sort_pairs, and scan scratch, for n elements
 static int ensure_sort_temp(oic_ctx *c, uint64_t n) {
   ALLOC(c, c->s_k2, n * 8);
   ALLOC(c, c->s_v2, n * 8);
@@ -445,16 +445,47 @@ static int ensure_sort_temp(oic_ctx *c, uint64_t n) {
   return OIC_OK;
 }
 
-// Stable sort of n (key, value) pairs on key bits [0, nbits) with rocPRIM's device radix sort; the
-// result may land in the alternate buffers (s_k2 / s_v2), and *k / *v then point there.
-static int sort_pairs_rocprim(oic_ctx *c, uint64_t **k, uint64_t **v, uint64_t n, int nbits) {
-  rocprim::double_buffer<uint64_t> K(*k, c->s_k2.as<uint64_t>()), V(*v, c->s_v2.as<uint64_t>());
-  size_t bytes = 0;
-  HIPCHK(c, rocprim::radix_sort_pairs(nullptr, bytes, K, V, (size_t)n, 0u, (unsigned)nbits, c->st));
-  ALLOC(c, c->s_rp, bytes);
-  HIPCHK(c, rocprim::radix_sort_pairs(c->s_rp.p, bytes, K, V, (size_t)n, 0u, (unsigned)nbits, c->st));
-  *k = K.current();
-  *v = V.current();
+// Stable sort of n (key, value) pairs on key bits [0, nbits): LSD passes of 8 to 11-bit digits
+// (oic_sort.h; as few passes as 11-bit digits allow, the digit width then evened out), ping-ponging
+// with the alternate buffers (s_k2 / s_v2); *k / *v point at the result.
+template <int BITS>
+static int sort_pass(oic_ctx *c, const uint64_t *ki, const uint64_t *vi, uint64_t *ko, uint64_t *vo, uint64_t n, int shift) {
+  const uint64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+  const uint64_t ncount = ((uint64_t)1 << BITS) * ntiles;
+  uint32_t *counts = (uint32_t *)c->s_rp.p;
+  uint64_t *offs = (uint64_t *)((uint8_t *)c->s_rp.p + ncount * 4);
+  hipLaunchKernelGGL(k_rs_hist<BITS>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, c->st, ki, n, shift, counts, (uint32_t)ntiles);
+  HIPCHK(c, scan_excl<uint32_t>(counts, offs, ncount, c->s_small.as<uint64_t>(), c->st, nullptr));
+  hipLaunchKernelGGL(k_rs_scatter<BITS>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, c->st, ki, vi, ko, vo, n, shift, offs,
+                     (uint32_t)ntiles);
+  HIPCHK(c, hipGetLastError());
+  return OIC_OK;
+}
+
+static int sort_pairs(oic_ctx *c, uint64_t **k, uint64_t **v, uint64_t n, int nbits) {
+  if (n < 2 || nbits <= 0) return OIC_OK;
+  const uint64_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+  if (ntiles > 0xffffffffull) return fail(c, OIC_EINVAL, "sort of %lu keys has too many tiles", (unsigned long)n);
+  const int passes = (nbits + 10) / 11;
+  const int bits = std::max(8, (nbits + passes - 1) / passes);
+  const uint64_t ncount = ((uint64_t)1 << bits) * ntiles;
+  ALLOC(c, c->s_rp, ncount * 12);
+  ALLOC(c, c->s_small, scan_scratch_words(ncount) * 8);
+  uint64_t *ki = *k, *vi = *v, *ko = c->s_k2.as<uint64_t>(), *vo = c->s_v2.as<uint64_t>();
+  for (int shift = 0; shift < nbits; shift += bits) {
+    int rc = OIC_OK;
+    switch (bits) {
+      case 8: rc = sort_pass<8>(c, ki, vi, ko, vo, n, shift); break;
+      case 9: rc = sort_pass<9>(c, ki, vi, ko, vo, n, shift); break;
+      case 10: rc = sort_pass<10>(c, ki, vi, ko, vo, n, shift); break;
+      default: rc = sort_pass<11>(c, ki, vi, ko, vo, n, shift); break;
+    }
+    if (rc) return rc;
+    std::swap(ki, ko);
+    std::swap(vi, vo);
+  }
+  *k = ki;
+  *v = vi;
   return OIC_OK;
 }
 
@@ -489,7 +520,7 @@ static int build_table(oic_ctx *c, uint32_t bgn, uint32_t end, std::vector<uint3
   if (N) {
     hipLaunchKernelGGL(k_kmer_emit, dim3(nb), dim3(256), 0, c->st, R, bgn, nb, d_posbase.as<uint64_t>(), d_loaded.as<uint8_t>(), K,
                        invalid, keys, vals);
-    const int rc = sort_pairs_rocprim(c, &keys, &vals, N, 2 * K + 1);   // stable: positions stay in (read, offset) order
+    const int rc = sort_pairs(c, &keys, &vals, N, 2 * K + 1);   // stable: positions stay in (read, offset) order
     if (rc) return rc;
     // the sorted arrays may have moved into the temp buffers: swap ownership so x_occ holds vals
     if (keys != c->s_k1.as<uint64_t>()) c->s_k1.swap(c->s_k2);
@@ -708,6 +739,9 @@ static int predict_pairs(oic_ctx *c, uint64_t np, std::vector<float> &pred) {
   return OIC_OK;
 }
 
+// the cost model's per-pair predictions (on the device) when they order this batch, else null
+static const float *spec_pred(oic_ctx *c) { return c->batch_pred ? c->m_pred.as<float>() : nullptr; }
+
 // the packed kernel in the loop form the tuning selects (one DP per wavefront: the per-task loop)
 template <int G, int CAP, bool STATS>
 static void launch_fwd(oic_ctx *c, const FwdArgs &FA, uint32_t waves) {
@@ -816,16 +850,14 @@ static int run_fwd_binned(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
 }
 
 // probe pass over every pair's longest node, then every other node of the pairs whose probe
-// ended at a branch point; results land in the per-node cache
+// ended at a branch point; results land in the per-node outcome words
 static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
-  ALLOC(c, c->b_cache, H);
-  ALLOC(c, c->b_ncells, H * 4);
-  ALLOC(c, c->b_nexit, H * 4);
+  ALLOC(c, c->b_outc, H * 8);
   ALLOC(c, c->b_probe, np * sizeof(FwdTask));
   ALLOC(c, c->b_cnt, np * 4);
   ALLOC(c, c->b_off, np * 8);
   ALLOC(c, c->b_stats, 64);
-  HIPCHK(c, hipMemsetAsync(c->b_cache.p, 0, H, c->st));
+  HIPCHK(c, hipMemsetAsync(c->b_outc.p, 0, H * 8, c->st));
   const unsigned g = (unsigned)((np + 255) / 256);
   hipLaunchKernelGGL(k_probe_tasks, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>());
   FwdArgs FA;
@@ -834,9 +866,7 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
   FA.pairs = c->p_pairs.as<PairRec>();
   FA.nodes = c->p_nodes.as<int4>();
   FA.queue = c->p_queue.as<uint32_t>();
-  FA.cache = c->b_cache.as<uint8_t>();
-  FA.ncells = c->b_ncells.as<uint32_t>();
-  FA.nexit = c->b_nexit.as<int32_t>();
+  FA.outc = c->b_outc.as<uint64_t>();
   FA.stats = c->b_stats.as<unsigned long long>();
   FA.ntasks_dev = nullptr;
   FA.tstamp = nullptr;
@@ -849,7 +879,8 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
   int rc = run_fwd_binned(c, FA, np, &ms);
   if (rc) return rc;
   hipLaunchKernelGGL(k_spec_count, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
-                     c->b_cache.as<uint8_t>(), c->p_order.as<uint32_t>(), c->b_cnt.as<uint32_t>());
+                     c->b_outc.as<uint64_t>(), spec_pred(c), c->p_order.as<uint32_t>(),
+                     c->b_cnt.as<uint32_t>());
   uint64_t nt = 0;
   ALLOC(c, c->s_scan, scan_scratch_words(np) * 8);
   HIPCHK(c, scan_excl<uint32_t>(c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), np, c->s_scan.as<uint64_t>(), c->st, &nt));
@@ -866,7 +897,7 @@ static int run_forward_passes(oic_ctx *c, uint64_t np, uint64_t H) {
       uint64_t *tk = c->s_k1.as<uint64_t>(), *tv = c->s_v1.as<uint64_t>();
       hipLaunchKernelGGL(k_task_keys, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->st, c->b_spec.as<FwdTask>(), nt,
                          c->p_pairs.as<PairRec>(), c->p_nodes.as<int4>(), tk, tv);
-      const int src = sort_pairs_rocprim(c, &tk, &tv, nt, 16);
+      const int src = sort_pairs(c, &tk, &tv, nt, 16);
       if (src) return src;
       FA.tasks = (const FwdTask *)tv;
     }
@@ -916,9 +947,7 @@ static int enqueue_fwd_narrow(oic_ctx *c, const FwdArgs &FA, uint64_t max_tasks,
 // band outgrew its tile, one per wavefront with a 512-cell tile; those wider still with an
 // 8192-cell tile.  Results are the same as run_forward_passes'.
 static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_t nnodes) {
-  ALLOC(c, c->b_cache, H + 4);
-  ALLOC(c, c->b_ncells, H * 4);
-  ALLOC(c, c->b_nexit, H * 4);
+  ALLOC(c, c->b_outc, std::max<uint64_t>(H, 1) * 8);
   ALLOC(c, c->b_probe, np * sizeof(FwdTask));
   ALLOC(c, c->b_cnt, np * 4);
   ALLOC(c, c->b_off, np * 8);
@@ -937,8 +966,8 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
   // everything up to the end of the capture; an error inside it still ends the capture below, so
   // the stream never stays in capture mode
   auto enqueue_all = [&]() -> int {
-    const uint64_t cw = (H + 4) / 4;
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((cw + 255) / 256)), dim3(256), 0, c->st, (uint32_t *)c->b_cache.p, cw, 0u);
+    const uint64_t cw = 2 * H;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((cw + 255) / 256)), dim3(256), 0, c->st, (uint32_t *)c->b_outc.p, cw, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, cnt, (uint64_t)16, 0u);
     unsigned long long *tst = stats + 48;      // wall-clock [start, end] of the two packed launches
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, c->st, (uint32_t *)stats, (uint64_t)(6 * 16), 0u);
@@ -951,9 +980,7 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
     FA.tb = dev_tables(c);
     FA.pairs = c->p_pairs.as<PairRec>();
     FA.nodes = c->p_nodes.as<int4>();
-    FA.cache = c->b_cache.as<uint8_t>();
-    FA.ncells = c->b_ncells.as<uint32_t>();
-    FA.nexit = c->b_nexit.as<int32_t>();
+    FA.outc = c->b_outc.as<uint64_t>();
     FwdTask *ovf = c->b_ovf.as<FwdTask>();
     // one pass: launch slot k (0 probe, 1 spec) uses queues/stats 3k..3k+2 and overflow counts 8+2k, 9+2k
     auto pass = [&](int k, const FwdTask *tasks, uint64_t n, const uint32_t *n_dev, uint64_t cap, FwdTask *ov1, FwdTask *ov2) -> int {
@@ -976,7 +1003,8 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
     int rc = pass(0, c->b_probe.as<FwdTask>(), np, nullptr, np, ovf, ovf + np);
     if (rc) return rc;
     hipLaunchKernelGGL(k_spec_count, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
-                       c->b_cache.as<uint8_t>(), c->p_order.as<uint32_t>(), c->b_cnt.as<uint32_t>());
+                       c->b_outc.as<uint64_t>(), spec_pred(c), c->p_order.as<uint32_t>(),
+                       c->b_cnt.as<uint32_t>());
     HIPCHK(c, scan_excl<uint32_t>(c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), np, c->s_scan.as<uint64_t>(), c->st, nullptr));
     hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, c->st, c->b_cnt.as<uint32_t>(), c->b_off.as<uint64_t>(), np, cnt + 6);
     hipLaunchKernelGGL(k_spec_write, dim3(g), dim3(256), 0, c->st, c->p_pairs.as<PairRec>(), np, c->b_probe.as<FwdTask>(),
@@ -1146,8 +1174,7 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     // Hits arrive in (S read, orientation) order (windows laid out by pbase).  A stable sort on the
     // digits holding the T bits keeps equal sorted digits in that order, so each (S, orientation, T)
     // pair ends up contiguous, its hits in generation order, without sorting the S bits above them.
-    const int nbits = (GA.tbits + 7) / 8 * 8;
-    const int rc = sort_pairs_rocprim(c, &hk, &hv, H, nbits);
+    const int rc = sort_pairs(c, &hk, &hv, H, GA.tbits);
     if (rc) return rc;
   }
   ALLOC(c, c->s_flag, H * 4);
@@ -1262,6 +1289,7 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   HIPCHK(c, hipMemsetAsync(c->p_stats.p, 0, STAT_NSTATS * 8, c->st));
   HIPCHK(c, hipMemsetAsync(c->p_pout.p, 0, np * sizeof(PairOut), c->st));
   std::vector<uint32_t> order;
+  c->batch_pred = false;
   {
     std::vector<float> pred;
     if (c->P.schedule == OIC_SCHED_LEARNED && c->have_model) {
@@ -1270,6 +1298,7 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
       bool finite = true;
       for (float v : pred) finite = finite && std::isfinite(v);
       if (finite) c->ctr.learned_pairs += np;
+      c->batch_pred = finite;
     }
     int rc = make_order(c->P.schedule, c->p_pairs.as<PairRec>(), np, c->h_len.data(), pred.empty() ? nullptr : pred.data(), c->st, order);
     if (rc) return fail(c, OIC_EHIP, "scheduling failed");
@@ -1290,9 +1319,7 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     A.C.sscreen = c->g_sscreen.as<uint32_t>();
     A.C.s0 = s0;
   }
-  A.cache = batched ? c->b_cache.as<uint8_t>() : nullptr;
-  A.ncells = batched ? c->b_ncells.as<uint32_t>() : nullptr;
-  A.nexit = batched ? c->b_nexit.as<int32_t>() : nullptr;
+  A.outc = batched ? c->b_outc.as<uint64_t>() : nullptr;
   A.pairs = c->p_pairs.as<PairRec>();
   A.order = c->p_order.as<uint32_t>();
   A.nodes = c->p_nodes.as<int4>();

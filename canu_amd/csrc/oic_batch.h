@@ -5,10 +5,11 @@
 // that match node (overlapInCore-Process_String_Overlaps.C:457-490): the reverse DP, the deltas and
 // the coordinates are never used.  A node's forward DP depends on nothing but the node, so the
 // forward DPs of many nodes run here ahead of Process_Matches, as independent tasks, and their
-// outcome is cached per node:
-//   cache[node] = 1  not a dovetail extension (Process_Matches skips the DP, removes the node)
-//                 2  dovetail (Process_Matches runs the whole extension for the deltas)
-//                 0  not computed (band outgrew the tile)
+// outcome is cached per node, one 64-bit word (pack_outcome) holding its kind,
+//   1  not a dovetail extension (Process_Matches skips the DP, removes the node)
+//   2  dovetail (Process_Matches runs the whole extension for the deltas)
+//   0  not computed (band outgrew the tile),
+// with the run's DP cells and right-end diagonal, so that a finished task makes a single store.
 // Tasks run 64/G to a wavefront (G lanes each, a T-cell band tile in LDS; the autotuner picks G and
 // T, 8 x 64 on repeat-rich reads); those whose band outgrows the tile are queued and re-run one to a
 // wavefront with a 512-cell tile, then an 8192-cell one, which holds any band the error limit
@@ -38,9 +39,7 @@ struct FwdArgs {
   const FwdTask *tasks;
   uint64_t ntasks;
   uint32_t *queue;
-  uint8_t *cache;           // per node (global node index = pair.hb + node)
-  uint32_t *ncells;         // per node: DP cells of the cached forward run
-  int32_t *nexit;           // per node: right-end diagonal (T_Hi - S_Hi) of the cached run
+  uint64_t *outc;           // per node (global node index = pair.hb + node): the cached outcome (pack_outcome)
   unsigned long long *stats;
   FwdTask *ovf;             // tasks whose band outgrew the tile are appended here (if not null)
   uint32_t *novf;
@@ -92,20 +91,6 @@ __device__ __forceinline__ uint32_t win16_staged(const uint32_t *__restrict__ wo
   return __builtin_amdgcn_alignbit(words[t.w0 + wi + 1], words[t.w0 + wi], sh);
 }
 
-// lexicographic (max v, then min d) over the G lanes of a group, by DPP moves within rows of 16
-template <int G>
-__device__ __forceinline__ void group_argmax(int32_t &v, int32_t &d) {
-  auto take = [&](int32_t ov, int32_t od) {
-    if (ov > v || (ov == v && od < d)) { v = ov; d = od; }
-  };
-  if (G >= 2) take(__builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false), __builtin_amdgcn_mov_dpp(d, 0xB1, 0xF, 0xF, false));    // quad_perm [1,0,3,2]
-  if (G >= 4) take(__builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false), __builtin_amdgcn_mov_dpp(d, 0x4E, 0xF, 0xF, false));    // quad_perm [2,3,0,1]
-  if (G >= 8) take(__builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false), __builtin_amdgcn_mov_dpp(d, 0x141, 0xF, 0xF, false));  // row_half_mirror
-  if (G >= 16) take(__builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false), __builtin_amdgcn_mov_dpp(d, 0x140, 0xF, 0xF, false)); // row_mirror
-  if (G >= 32) take(__shfl_xor(v, 16), __shfl_xor(d, 16));
-  if (G >= 64) take(__shfl_xor(v, 32), __shfl_xor(d, 32));
-}
-
 // maximum over the G lanes of a group, by DPP moves within rows of 16
 template <int G>
 __device__ __forceinline__ uint64_t group_max_u64(uint64_t v) {
@@ -123,6 +108,35 @@ __device__ __forceinline__ uint64_t group_max_u64(uint64_t v) {
   if (G >= 32) step(__shfl_xor((uint32_t)v, 16), __shfl_xor((uint32_t)(v >> 32), 16));
   if (G >= 64) step(__shfl_xor((uint32_t)v, 32), __shfl_xor((uint32_t)(v >> 32), 32));
   return v;
+}
+
+// minimum of lo and maximum of hi over the G lanes of a group, by DPP moves within rows of 16
+template <int G>
+__device__ __forceinline__ void group_min_max(int32_t &lo, int32_t &hi) {
+#define OIC_DPPMM(ctl)                                                               \
+  {                                                                                  \
+    const int32_t ol = __builtin_amdgcn_mov_dpp(lo, ctl, 0xF, 0xF, false);           \
+    const int32_t oh = __builtin_amdgcn_mov_dpp(hi, ctl, 0xF, 0xF, false);           \
+    lo = ol < lo ? ol : lo;                                                          \
+    hi = oh > hi ? oh : hi;                                                          \
+  }
+  if (G >= 2) OIC_DPPMM(0xB1);    // quad_perm [1,0,3,2]
+  if (G >= 4) OIC_DPPMM(0x4E);    // quad_perm [2,3,0,1]
+  if (G >= 8) OIC_DPPMM(0x141);   // row_half_mirror
+  if (G >= 16) OIC_DPPMM(0x140);  // row_mirror
+#undef OIC_DPPMM
+  if (G >= 32) { const int32_t ol = __shfl_xor(lo, 16), oh = __shfl_xor(hi, 16); lo = ol < lo ? ol : lo; hi = oh > hi ? oh : hi; }
+  if (G >= 64) { const int32_t ol = __shfl_xor(lo, 32), oh = __shfl_xor(hi, 32); lo = ol < lo ? ol : lo; hi = oh > hi ? oh : hi; }
+}
+
+// lexicographic (max v, then min d) over the G lanes of a group, as one 64-bit key per lane
+// (v >= -1, |d| < 2^30; a lane's d matters only where its v is the group's maximum and >= 0)
+template <int G>
+__device__ __forceinline__ void group_argmax(int32_t &v, int32_t &d) {
+  uint64_t k = ((uint64_t)(uint32_t)(v + 1) << 32) | (uint32_t)(0x40000000 - d);
+  k = group_max_u64<G>(k);
+  v = (int32_t)(uint32_t)(k >> 32) - 1;
+  d = 0x40000000 - (int32_t)(uint32_t)k;
 }
 
 template <int G>
@@ -391,10 +405,8 @@ __global__ void __launch_bounds__(OIC_WAVE) k_fwd_tasks(FwdArgs A) {
     }
     if (gl == 0) {
       // 1: not a dovetail (Process_Matches only removes the node), 2: dovetail, 0: not computed
-      A.cache[gnode] = overflow ? 0 : ((rmte && lmte) ? 2 : 1);
+      A.outc[gnode] = pack_outcome(overflow ? 0 : ((rmte && lmte) ? 2 : 1), exit_d, cells);
       if (overflow && A.ovf) A.ovf[atomicAdd(A.novf, 1u)] = tk;
-      A.ncells[gnode] = cells;
-      A.nexit[gnode] = exit_d;
       if (!overflow) cells_acc += cells;
       rows_acc += rows;
       slots_acc += slots;
@@ -476,6 +488,38 @@ __device__ __forceinline__ void words5_staged(const uint32_t *__restrict__ words
   }
 }
 
+// the three words from word wi of a staged tail: read from its LDS copy (at a clamped index, so
+// always in bounds), and reloaded from HBM by the lanes whose words lie past the copy
+__device__ __forceinline__ void words3_staged(const uint32_t *__restrict__ words, const Staged &t, int32_t wi, uint32_t &w0,
+                                              uint32_t &w1, uint32_t &w2) {
+  const int32_t wl = wi < OIC_SEQ_WIN - 3 ? wi : OIC_SEQ_WIN - 3;
+  w0 = t.lds[wl]; w1 = t.lds[wl + 1]; w2 = t.lds[wl + 2];
+  if (wi > OIC_SEQ_WIN - 3) {
+    const uint32_t *g = words + (t.w0 + wi);
+    w0 = g[0]; w1 = g[1]; w2 = g[2];
+  }
+}
+
+// 32 bases of both tails from positions pa and pb, compared: the number of leading equal bases (32
+// if all are).  A slide's first window: at 8 % divergence about 2 % of cells slide past 16 bases,
+// and one such lane makes the whole wavefront run the 64-base continuation; past 32 bases far fewer.
+__device__ __forceinline__ int32_t sm_match32(const SMDP &s, const uint32_t *__restrict__ words,
+                                              const uint32_t *__restrict__ nmask, int32_t pa, int32_t pb) {
+  const int32_t qa = s.ta.r0 + pa, qb = s.tb.r0 + pb;
+  uint32_t a0, a1, a2, b0, b1, b2;
+  words3_staged(words, s.ta, qa >> 4, a0, a1, a2);
+  words3_staged(words, s.tb, qb >> 4, b0, b1, b2);
+  const uint32_t sa = (uint32_t)(qa & 15) * 2u, sb = (uint32_t)(qb & 15) * 2u;
+  uint32_t x0 = __builtin_amdgcn_alignbit(a1, a0, sa) ^ __builtin_amdgcn_alignbit(b1, b0, sb);
+  uint32_t x1 = __builtin_amdgcn_alignbit(a2, a1, sa) ^ __builtin_amdgcn_alignbit(b2, b1, sb);
+  x0 = (x0 | (x0 >> 1)) & 0x55555555u;
+  x1 = (x1 | (x1 >> 1)) & 0x55555555u;
+  if (s.nA) { x0 &= ~nwin16(nmask, s.a_b0 + pa); x1 &= ~nwin16(nmask, s.a_b0 + pa + 16); }
+  if (s.nB) { x0 &= ~nwin16(nmask, s.b_b0 + pb); x1 &= ~nwin16(nmask, s.b_b0 + pb + 16); }
+  // the first mismatch bit of x1:x0 (two bits a base), 64 when there is none
+  return (int32_t)(__builtin_ctzg(((uint64_t)x1 << 32) | x0, 64) >> 1);
+}
+
 // 64 bases of both tails from positions pa and pb, compared: the number of leading equal bases (64
 // if all are).  The five words of each tail are loaded together.
 __device__ __forceinline__ int32_t sm_match64(const SMDP &s, const uint32_t *__restrict__ words,
@@ -535,7 +579,40 @@ __device__ __forceinline__ bool sm_start(SMDP &s, SMRes &res, const uint32_t *__
   return s.lim < 1;
 }
 
+// first maximum (largest Row, then smallest d) of a stored row over diagonals [lo, hi], on a group;
+// v = -1 when the range is empty
+template <int G, int CAP>
+__device__ __forceinline__ void sm_row_first_max(const int32_t *buf, int32_t lo, int32_t hi, int gl, int32_t &v, int32_t &dv) {
+  v = -1;
+  dv = 0;
+  for (int32_t d = lo + gl; d <= hi; d += G) {
+    const int32_t r = buf[d & (CAP - 1)];
+    dv = r > v ? d : dv;
+    v = r > v ? r : v;
+  }
+  group_argmax<G>(v, dv);
+}
+
+// Longest / Max_Score bookkeeping of row er, from that row's first maximum (v at diagonal dv)
+// (prefixEditDistance-forward.C:292-306)
+__device__ __forceinline__ void sm_longest(SMDP &s, const DPTables &tb, int32_t v, int32_t dv, int32_t er) {
+  if (v > s.Longest) {
+    s.Longest = v;
+    const double Score = (double)v * tb.bmv - (double)er;
+    if (Score > s.MaxScore) { s.MaxScore = Score; s.MSL = v; s.MSd = dv; s.MSe = er; }
+  }
+}
+
 // Row s.e + 1 of the DP: the body of group_prefix_dp's row loop.  Returns true when the DP is over.
+//
+// Per cell, each lane keeps only its own first and last kept diagonal and its first maximum of the
+// previous row; the group reduces them once per row.  The previous row's first maximum (its
+// Longest update, prefixEditDistance-forward.C:292-306) is read from the same three band loads the
+// recurrence makes: this row's band [Left, Right] is the previous row's kept range widened by one
+// on each side, whose two new ends hold the -2 sentinels, so every previous-row cell is read
+// exactly once as 1 + EAL[e-1][d].  The update therefore lands one row late, which nothing can
+// see: the branch-point test reads Max_Score only when a cell reaches an end, and there (a rare
+// path) the previous row is scanned first; the last row (e = Error_Limit) is scanned at its end.
 template <int G, int CAP, bool STATS>
 __device__ __forceinline__ bool sm_row(SMDP &s, SMRes &res, const uint32_t *__restrict__ words,
                                        const uint32_t *__restrict__ nmask, const DPTables &tb, int32_t *&pbuf, int32_t *&cbuf,
@@ -559,98 +636,94 @@ __device__ __forceinline__ bool sm_row(SMDP &s, SMRes &res, const uint32_t *__re
   }
   wave_fence();
   const int32_t Left = s.Left, Right = s.Right;
-  int32_t nL = 0x7fffffff, nR = 0;
-  // candidate cells as keys Row << 32 | (2^30 - d): the larger key has the larger Row, then the
-  // smaller d (Row >= 0, |d| < 2^30); 0 = none
-  uint64_t ck = 0, pk = 0;
+  int32_t kmin = 0x7fffffff, kmax = -0x7fffffff;   // this lane's first and last kept diagonal
+  int32_t pv = -1, pd = 0;                         // this lane's first maximum of the previous row
   for (int32_t c0 = Left; c0 <= Right; c0 += G) {
     const int32_t d = c0 + gl;
     const bool act = d <= Right;
     if (STATS) res.slots += G;
+    bool reach = false;
     int32_t Row = 0;
     if (act) {
-      const int32_t x1 = 1 + pbuf[d & MASK], x2 = pbuf[(d - 1) & MASK], x3 = 1 + pbuf[(d + 1) & MASK];
-      Row = x1 > x2 ? x1 : x2;
+      const int32_t p1 = pbuf[d & MASK], x2 = pbuf[(d - 1) & MASK], x3 = 1 + pbuf[(d + 1) & MASK];
+      pd = p1 > pv ? d : pd;
+      pv = p1 > pv ? p1 : pv;
+      Row = 1 + p1;
+      Row = Row > x2 ? Row : x2;
       Row = Row > x3 ? Row : x3;
       // slide along diagonal d while the bases match (prefixEditDistance-forward.C:155-167): a
-      // 16-base window first (most slides end in it), then 64-base windows (long matches)
+      // 32-base window first (nearly all slides end in it), then 64-base windows (long matches)
       const int32_t r1 = s.m - Row, r2 = s.n - Row - d;
-      int32_t rem = r1 < r2 ? r1 : r2;
-      int32_t adv = 0;
-      if (rem > 0) {
+      const int32_t rem0 = r1 < r2 ? r1 : r2;     // >= 0: no cell of the previous row reached an end
+      const int32_t kk = sm_match32(s, words, nmask, Row, Row + d);
+      int32_t adv = kk < rem0 ? kk : rem0;
+      if (STATS) {
+        res.siters++;
+        if (lane_id() == __builtin_ctzll(__ballot(1))) res.wsiters++;
+      }
+      int32_t rem = (kk == 32 && rem0 > 32) ? rem0 - 32 : 0;
+      while (rem > 0) {
         if (STATS) {
           res.siters++;
           if (lane_id() == __builtin_ctzll(__ballot(1))) res.wsiters++;
         }
-        const int32_t kk = sm_match16(s, words, nmask, Row, Row + d);
-        if (kk >= rem) {
-          adv = rem;
-        } else {
-          adv = kk;
-          rem = (kk == 16) ? rem - 16 : 0;
-          while (rem > 0) {
-            if (STATS) {
-              res.siters++;
-              if (lane_id() == __builtin_ctzll(__ballot(1))) res.wsiters++;
-            }
-            const int32_t k64 = sm_match64(s, words, nmask, Row + adv, Row + d + adv);
-            if (k64 >= rem) { adv += rem; break; }
-            adv += k64;
-            if (k64 < 64) break;
-            rem -= 64;
-          }
-        }
+        const int32_t k64 = sm_match64(s, words, nmask, Row + adv, Row + d + adv);
+        if (k64 >= rem) { adv += rem; break; }
+        adv += k64;
+        if (k64 < 64) break;
+        rem -= 64;
       }
-      res.bases += (uint32_t)adv + 1u;
+      if (STATS) res.bases += (uint32_t)adv + 1u;
+      reach = adv == rem0;                        // Row + adv is m, or Row + adv + d is n
       Row += adv;
       cbuf[d & MASK] = Row;
+      const bool keep = Row + (d > 0 ? d : 0) >= eml;   // not pruned (prefixEditDistance-forward.C:245-285)
+      kmin = (keep & (kmin == 0x7fffffff)) ? d : kmin;
+      kmax = keep ? d : kmax;
     }
-    const bool reach = act && (Row == s.m || Row + d == s.n);
-    const uint64_t bl = group_ballot<G>(reach, g);
-    if (bl) {
-      // the end of A or T was reached: branch-point test (prefixEditDistance-forward.C:172-212)
-      const int f = __builtin_ctzll(bl);
-      res.cells += (uint32_t)(f + 1);
-      const int32_t rrow = __shfl(Row, g * G + f);
-      const double Score = (double)rrow * tb.bmv - (double)e;
-      const int32_t Tail_Len = rrow - s.MSL;
-      const double slope = (double)(s.MaxScore - Score) / (double)Tail_Len;
-      const bool abort_ = ((uint32_t)e > tb.min_branch_end_dist / 2u) && ((uint32_t)Tail_Len >= tb.min_branch_end_dist) &&
-                          (slope >= tb.slope_min);
-      if (abort_) { res.mte = false; res.e = s.MSe; res.d = s.MSd; }
-      else        { res.mte = true;  res.e = e;     res.d = c0 + f; }
-      return true;
+    if (__ballot(reach)) {
+      const uint64_t bl = group_ballot<G>(reach, g);
+      if (bl) {
+        // the end of A or T was reached: branch-point test (prefixEditDistance-forward.C:172-212),
+        // after the previous row's Longest update
+        const int f = __builtin_ctzll(bl);
+        res.cells += (uint32_t)(c0 + f - Left + 1);
+        const int32_t rrow = __shfl(Row, g * G + f);
+        if (e >= 2) {
+          int32_t v, dv;
+          sm_row_first_max<G, CAP>(pbuf, Left + 1, Right - 1, gl, v, dv);
+          sm_longest(s, tb, v, dv, e - 1);
+        }
+        const double Score = (double)rrow * tb.bmv - (double)e;
+        const int32_t Tail_Len = rrow - s.MSL;
+        const double slope = (double)(s.MaxScore - Score) / (double)Tail_Len;
+        const bool abort_ = ((uint32_t)e > tb.min_branch_end_dist / 2u) && ((uint32_t)Tail_Len >= tb.min_branch_end_dist) &&
+                            (slope >= tb.slope_min);
+        if (abort_) { res.mte = false; res.e = s.MSe; res.d = s.MSd; }
+        else        { res.mte = true;  res.e = e;     res.d = c0 + f; }
+        return true;
+      }
     }
-    res.cells += (uint32_t)((Right - c0 + 1) < G ? (Right - c0 + 1) : G);
-    // pruned range and first-max cell in the same pass (prefixEditDistance-forward.C:245-299), as
-    // selects: per lane, "committed" is the best cell at or before the group's latest kept cell,
-    // "pending" the best after it, promoted when a later kept cell appears
-    const bool keep = act && (Row + (d > 0 ? d : 0) >= eml);
-    const uint64_t kb = group_ballot<G>(keep, g);
-    const bool any = kb != 0;
-    const int32_t kl = c0 + (63 - __builtin_clzll(kb | 1ull));
-    nL = (any & (nL == 0x7fffffff)) ? c0 + __builtin_ctzll(kb | (1ull << 63)) : nL;
-    nR = any ? kl : nR;
-    const uint64_t key = ((uint64_t)(uint32_t)Row << 32) | (uint32_t)(0x40000000 - d);
-    ck = (any & (pk > ck)) ? pk : ck;
-    const uint64_t p0 = any ? 0ull : pk;
-    const bool in = act & (nL != 0x7fffffff) & (d >= nL);
-    const bool le = any & (d <= kl);
-    ck = (in & le & (key > ck)) ? key : ck;
-    pk = (in & !le & (key > p0)) ? key : p0;
   }
+  res.cells += (uint32_t)(Right - Left + 1);
+  if (e >= 2) {
+    group_argmax<G>(pv, pd);
+    sm_longest(s, tb, pv, pd, e - 1);
+  }
+  // the kept range (prefixEditDistance-forward.C:245-285): first and last kept diagonal
+  int32_t nL = kmin, nR = kmax;
+  group_min_max<G>(nL, nR);
   if (nL == 0x7fffffff) { res.mte = false; res.e = s.MSe; res.d = s.MSd; return true; }
   s.Left = nL;
   s.Right = nR;
-  ck = group_max_u64<G>(ck);
-  const int32_t cbv = (int32_t)(ck >> 32), cbd = 0x40000000 - (int32_t)(uint32_t)ck;
-  if (cbv > s.Longest) {
-    s.Longest = cbv;
-    const double Score = (double)cbv * tb.bmv - (double)e;
-    if (Score > s.MaxScore) { s.MaxScore = Score; s.MSL = cbv; s.MSd = cbd; s.MSe = e; }
+  if (e >= s.lim) {
+    int32_t v, dv;
+    sm_row_first_max<G, CAP>(cbuf, nL, nR, gl, v, dv);
+    sm_longest(s, tb, v, dv, e);
+    res.mte = false; res.e = s.MSe; res.d = s.MSd;
+    return true;
   }
   int32_t *t = pbuf; pbuf = cbuf; cbuf = t;
-  if (e >= s.lim) { res.mte = false; res.e = s.MSe; res.d = s.MSd; return true; }
   return false;
 }
 
@@ -718,10 +791,8 @@ k_fwd_sm(FwdArgs A) {
     const bool overflow = res.overflow;
     if (gl == 0) {
       // 1: not a dovetail (Process_Matches only removes the node), 2: dovetail, 0: not computed
-      A.cache[gnode] = overflow ? 0 : ((rmte && lmte) ? 2 : 1);
+      A.outc[gnode] = pack_outcome(overflow ? 0 : ((rmte && lmte) ? 2 : 1), exit_d, res.cells);
       if (overflow && A.ovf) A.ovf[atomicAdd(A.novf, 1u)] = tk;
-      A.ncells[gnode] = res.cells;
-      A.nexit[gnode] = exit_d;
       if (!overflow) cells_acc += res.cells;
       rows_acc += res.rows;
       slots_acc += res.slots;
@@ -827,18 +898,27 @@ __global__ void k_probe_tasks(const PairRec *__restrict__ pairs, uint64_t np, Fw
 
 // Pairs whose other nodes are all worth a forward DP ahead of Process_Matches: those whose probe
 // was not a dovetail (each later node is reached unless it is itself the next one removed), and
-// those with a dovetail probe but at least SPEC_DOVE_MIN nodes (repeat copies on shifted
-// diagonals survive the alignment's removal; a few nodes near it are computed in vain).  Counts
-// are written in processing order.
+// some of those with a dovetail probe, whose nodes on the probe's alignment Process_Matches will
+// remove without extending them (overlapInCore-Process_String_Overlaps.C:476-490) while repeat
+// copies on shifted diagonals survive.  For a dovetail probe the static rule speculates when the
+// pair has at least SPEC_DOVE_MIN nodes; with a cost model (pred: its log(1 + DP cells) of the
+// pair's Process_Matches, per pair) it speculates when the model expects more than SPEC_DOVE_GAIN
+// times the probe's own cells, i.e. further extensions beyond the probe.  Counts are written in
+// processing order.  Which nodes run ahead never changes a result.
 #define SPEC_DOVE_MIN 16
+#define SPEC_DOVE_GAIN 2.0f
 __global__ void k_spec_count(const PairRec *__restrict__ pairs, uint64_t np, const FwdTask *__restrict__ probe,
-                             const uint8_t *__restrict__ cache, const uint32_t *__restrict__ order, uint32_t *__restrict__ cnt) {
+                             const uint64_t *__restrict__ outc, const float *__restrict__ pred,
+                             const uint32_t *__restrict__ order, uint32_t *__restrict__ cnt) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const uint32_t p = order[i];
   const PairRec pr = pairs[p];
-  const uint8_t pc = cache[pr.hb + probe[p].node];
-  const bool spec = !pr.consistent && pr.nnodes > 1 && (pc == 1 || (pc == 2 && pr.nnodes >= SPEC_DOVE_MIN));
+  const uint64_t po = outc[pr.hb + probe[p].node];
+  const int pc = outcome_kind(po);
+  bool dove_spec = pr.nnodes >= SPEC_DOVE_MIN;
+  if (pred) dove_spec = expm1f(pred[2 * (uint64_t)p]) > SPEC_DOVE_GAIN * (float)outcome_cells(po);
+  const bool spec = !pr.consistent && pr.nnodes > 1 && (pc == 1 || (pc == 2 && dove_spec));
   cnt[i] = spec ? (uint32_t)(pr.nnodes - 1) : 0u;
 }
 

@@ -70,13 +70,23 @@ __device__ __forceinline__ int32_t lane_cmp64(const uint32_t *__restrict__ wa, i
   return 64;
 }
 
-// Wavefronts per SIMD the ring allows (160 KB of LDS per CU over 4 SIMDs), at most 8.
+// words of each view a lane keeps staged in LDS (128 bases)
+#define LN_SW 8
+
+// Wavefronts per SIMD the ring and staged words allow (160 KB of LDS per CU over 4 SIMDs), at most 8.
 template <int CAP>
 constexpr int lane_waves_per_eu() {
-  constexpr int lds = CAP * OIC_WAVE * 4;
+  constexpr int lds = (CAP + 2 * LN_SW) * OIC_WAVE * 4;
   constexpr int w = 163840 / lds / 4;
   return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
+
+// lane states
+#define LN_CELL   0   // computing cells of its current row (the inner loop)
+#define LN_ROWEND 1   // its row is done: prune, first maximum, open the next row
+#define LN_DPEND  2   // its DP is over (dp_* hold the outcome, or the reaching cell when dp_reach)
+#define LN_FETCH  3   // needs its next task
+#define LN_DONE   4   // no tasks left
 
 template <int CAP, bool STATS>
 __global__ void __launch_bounds__(OIC_WAVE) __attribute__((amdgpu_waves_per_eu(lane_waves_per_eu<CAP>(), 8)))
@@ -85,18 +95,20 @@ k_fwd_lane(FwdArgs A) {
   constexpr int32_t MASK = CAP - 1;
   constexpr int32_t NONE = 0x7fffffff;
   __shared__ int32_t ring[CAP * OIC_WAVE];
+  __shared__ uint32_t stage[2 * LN_SW * OIC_WAVE];
   const int lane = lane_id();
   int32_t *const my = ring + lane;              // slot s of this lane: my[s * 64]
+  uint32_t *const sa = stage + lane, *const sb = stage + LN_SW * OIC_WAVE + lane;   // word k: s?[k * 64]
   const uint64_t ntasks = A.ntasks_dev ? (uint64_t)*A.ntasks_dev : A.ntasks;
   if (A.tstamp && lane == 0) atomicMin(&A.tstamp[0], (unsigned long long)wall_clock64());
   const uint32_t *__restrict__ words = A.R.words;
   const DPTables &tb = A.tb;
 
+  int state = LN_FETCH;
   // task
-  int state = 0;                 // 0 needs a task, 1 forward DP, 2 reverse DP, 3 no tasks left
   uint64_t ti = 0;
   int32_t limit = 0, right_e = 0, exit_d = 0;
-  bool s_short = false, rmte = true;
+  bool s_short = false, fwd = true, rmte = true;
   uint32_t cells = 0, bases = 0, rows = 0, windows = 0;
   // the DP in progress
   const uint32_t *wa = words, *wb = words;      // word holding base 0 of each view
@@ -107,15 +119,25 @@ k_fwd_lane(FwdArgs A) {
   int32_t pv = -2, cv = -2, nv = -2;            // prev[d - 1], prev[d], prev[d + 1]
   int32_t Row = 0, rem = 0, r0 = 0;
   bool sliding = false, cont = false, row0 = false;
+  int32_t sta = 0, stb = 0;                     // first staged word of each view (relative to wa / wb)
+  int32_t mina = 0, minb = 0;                   // lowest A and T position the row's cells started from
+  bool fell = false;                            // a cell of the row read past the staged words
   int32_t nL = NONE, nR = 0, cbv = -1, cbd = 0, pbv = -1, pbd = 0;
   int32_t Longest = 0, MSL = 0, MSd = 0, MSe = 0;
   double MaxScore = 0.0;
   // outcome of the DP that just ended
-  bool dmte = false, dovf = false;
-  int32_t de = 0, dd = 0;
+  bool dp_mte = false, dp_ovf = false, dp_reach = false;
+  int32_t dp_e = 0, dp_d = 0;
   uint64_t cells_acc = 0, bases_acc = 0, rows_acc = 0, win_acc = 0, iters = 0;
 
-  // a DP on views (word, base within word, N position, length) starts with its row-0 slide
+  // copy LN_SW words of each view from words ka / kb into the lane's staging slots
+  auto restage = [&](int32_t ka, int32_t kb) {
+    sta = ka;
+    stb = kb;
+#pragma unroll
+    for (int k = 0; k < LN_SW; k++) { sa[k * OIC_WAVE] = wa[ka + k]; sb[k * OIC_WAVE] = wb[kb + k]; }
+  };
+  // a DP on two views starts with its row-0 slide (prefixEditDistance-forward.C:113-132)
   auto begin_dp = [&](int64_t a_b0, bool aN, int32_t am, int64_t b_b0, bool bN, int32_t bn, int32_t l) {
     wa = words + (a_b0 >> 4); ra = (int32_t)(a_b0 & 15);
     wb = words + (b_b0 >> 4); rb = (int32_t)(b_b0 & 15);
@@ -125,8 +147,10 @@ k_fwd_lane(FwdArgs A) {
     row0 = true; sliding = true; cont = false;
     d = 0; Row = 0; r0 = 0; rem = m;            // m <= n
     Longest = 0; MSL = 0; MSd = 0; MSe = 0; MaxScore = 0.0;
+    restage(0, 0);
+    state = LN_CELL;
   };
-  // the reverse DP on the left tails when the forward one reached the end; false: none to run
+  // the reverse DP on the left tails when the forward one reached the end (extend.C:112-132)
   auto begin_reverse = [&]() -> bool {
     const FwdTask tk = A.tasks[ti];
     const PairRec pr = A.pairs[tk.pair];
@@ -141,192 +165,220 @@ k_fwd_lane(FwdArgs A) {
     const bool s_first = S_Right_Begin <= T_Right_Begin;
     if (s_first) begin_dp(s0, sN, S_Left_Begin + 1, t0, tN, T_Left_Begin + 1, limit - right_e);
     else         begin_dp(t0, tN, T_Left_Begin + 1, s0, sN, S_Left_Begin + 1, limit - right_e);
+    fwd = false;
     return true;
+  };
+  // the task's outcome: 1 not a dovetail (Process_Matches only removes the node), 2 dovetail, 0 not
+  // computed (band outgrew the ring)
+  auto record = [&](bool ovf, bool dove) {
+    const FwdTask tk = A.tasks[ti];
+    const uint64_t gnode = A.pairs[tk.pair].hb + tk.node;
+    A.outc[gnode] = pack_outcome(ovf ? 0 : (dove ? 2 : 1), exit_d, cells);
+    if (ovf && A.ovf) A.ovf[atomicAdd(A.novf, 1u)] = tk;
+    if (!ovf) cells_acc += cells;
+    bases_acc += bases;
+    rows_acc += rows;
+    win_acc += windows;
+    state = LN_FETCH;
   };
 
   while (true) {
-    // ---- lanes without a DP take the next tasks, one each, in queue order
-    const uint64_t want = __ballot(state == 0);
-    if (want) {
-      uint32_t base = 0;
-      if (lane == (int)__builtin_ctzll(want)) base = atomicAdd(A.queue, (uint32_t)__builtin_popcountll(want));
-      base = __shfl(base, (int)__builtin_ctzll(want));
-      if (state == 0) {
-        ti = (uint64_t)base + (uint64_t)__builtin_popcountll(want & ((1ull << lane) - 1));
-        if (ti >= ntasks) {
-          state = 3;
-        } else {
-          cells = 0; bases = 0; rows = 0; windows = 0; rmte = true; dovf = false;
-          const FwdTask tk = A.tasks[ti];
-          const PairRec pr = A.pairs[tk.pair];
-          const int4 nd = A.nodes[pr.hb + tk.node];
-          const int32_t S_Len = A.R.len[pr.s_id], T_Len = A.R.len[pr.t_id];
-          const int32_t Start = nd.z, Offset = nd.x, Len = nd.y;
-          const int32_t S_Right_Begin = Start + Len, S_Right_Len = S_Len - S_Right_Begin;
-          const int32_t T_Right_Begin = Offset + Len, T_Right_Len = T_Len - T_Right_Begin;
-          const int32_t Total_Olap = (Start < Offset ? Start : Offset) + Len + (S_Right_Len < T_Right_Len ? S_Right_Len : T_Right_Len);
-          limit = tb.error_bound[Total_Olap];
-          exit_d = T_Right_Begin - S_Right_Begin;
-          right_e = 0;
-          if (S_Right_Len == 0 || T_Right_Len == 0) {
-            // no forward DP: the right end is reached with 0 errors
-            state = begin_reverse() ? 2 : 0;
-            if (state == 0) {
-              const uint64_t gnode = pr.hb + tk.node;
-              A.cache[gnode] = 2;
-              A.ncells[gnode] = 0;
-              A.nexit[gnode] = exit_d;
-            }
-          } else {
-            s_short = S_Right_Len <= T_Right_Len;   // the shorter tail is A (prefixEditDistance-extend.C:81-99)
-            const int64_t s0 = (pr.dir ? A.R.b0r[pr.s_id] : A.R.b0f[pr.s_id]) + S_Right_Begin, t0 = A.R.b0f[pr.t_id] + T_Right_Begin;
-            const bool sN = A.R.hasN[pr.s_id], tN = A.R.hasN[pr.t_id];
-            if (s_short) begin_dp(s0, sN, S_Right_Len, t0, tN, T_Right_Len, limit);
-            else         begin_dp(t0, tN, T_Right_Len, s0, sN, S_Right_Len, limit);
-            state = 1;
-          }
-        }
-      }
-    }
-    const bool run = state == 1 || state == 2;
-    if (__ballot(run) == 0) {
-      if (__ballot(state == 0) == 0) break;   // every lane has run out of tasks
-      continue;
-    }
-    if (STATS) iters++;
-    bool dp_end = false, row_end = false;
-    if (run) {
-      // ---- one step of the current cell: its start (the furthest-reaching maximum), then one
-      // window of its slide (prefixEditDistance-forward.C:155-167)
-      if (!sliding) {
-        nv = (d + 1 <= oR) ? my[((d + 1) & MASK) * OIC_WAVE] : -2;
-        Row = cv + 1;
-        Row = pv > Row ? pv : Row;
-        Row = nv + 1 > Row ? nv + 1 : Row;
-        r0 = Row;
-        const int32_t r1 = m - Row, r2 = n - Row - d;
-        rem = r1 < r2 ? r1 : r2;
-        sliding = true;
-        cont = false;
-      }
-      int32_t k = 0;
-      if (rem > 0) {
-        if (STATS) windows++;
-        if (!cont) k = (na & nb) < 0 ? lane_cmp16(wa, ra + Row, wb, rb + Row + d)
-                                     : lane_cmp16_n(wa, ra + Row, wb, rb + Row + d, A.R.nmask, na, nb);
-        else       k = lane_cmp64(wa, ra + Row, wb, rb + Row + d, A.R.nmask, na, nb);
-      }
-      const int32_t wlen = cont ? 64 : 16;
-      if (k >= rem) { Row += rem; rem = 0; }
-      else          { Row += k; rem = (k == wlen) ? rem - wlen : 0; }
-      if (rem > 0) {
-        cont = true;                             // the slide goes on in the next iteration
-      } else {
-        sliding = false;
-        cells++;
-        bases += (uint32_t)(Row - r0) + 1u;
-        if (row0) {
-          // the exact-prefix slide (forward.C:113-132)
-          row0 = false;
-          if (Row == m) { dmte = true; de = 0; dd = 0; dp_end = true; }
+    // ---------------------------------------------------------------- outer: rows, DPs, tasks
+    // Lanes leave the inner loop with a finished row, a finished DP or no task; each pass below
+    // moves them on, until every lane computes cells again or has run out of tasks.
+    while (__ballot(state == LN_ROWEND || state == LN_DPEND || state == LN_FETCH)) {
+      if (state == LN_ROWEND) {
+        // prune to the first and last kept cell, take the row's first maximum (forward.C:245-299),
+        // open the next row with -2 sentinels just outside the old band (forward.C:147-153)
+        bool go = true;
+        if (e > 0) {
+          if (nL == NONE) { dp_mte = false; dp_e = MSe; dp_d = MSd; state = LN_DPEND; go = false; }   // band emptied
           else {
-            my[0] = Row;
-            e = 0; L = 0; R = 0;
-            if (lim < 1) { dmte = false; de = 0; dd = 0; dp_end = true; }
-            else row_end = true;                 // opens row 1 below
-          }
-        } else {
-          my[(d & MASK) * OIC_WAVE] = Row;
-          if (Row == m || Row + d == n) {
-            // the end of A or T: branch-point test (forward.C:172-212)
-            const double Score = (double)Row * tb.bmv - (double)e;
-            const int32_t Tail_Len = Row - MSL;
-            const double slope = (double)(MaxScore - Score) / (double)Tail_Len;
-            const bool abort_ = ((uint32_t)e > tb.min_branch_end_dist / 2u) && ((uint32_t)Tail_Len >= tb.min_branch_end_dist) &&
-                                (slope >= tb.slope_min);
-            dmte = !abort_;
-            de = abort_ ? MSe : e;
-            dd = abort_ ? MSd : d;
-            dp_end = true;
-          } else {
-            // pruning range and first maximal cell of the row, as running values (forward.C:245-299):
-            // "committed" is the best cell up to the latest kept cell, "pending" the best after it
-            const bool keep = Row + (d > 0 ? d : 0) >= eml;
-            if (keep) {
-              if (nL == NONE) nL = d;
-              nR = d;
-              if (pbv > cbv) { cbv = pbv; cbd = pbd; }
-              pbv = -1;
-              if (Row > cbv) { cbv = Row; cbd = d; }
-            } else if (nL != NONE && Row > pbv) {
-              pbv = Row;
-              pbd = d;
+            L = nL;
+            R = nR;
+            if (cbv > Longest) {
+              Longest = cbv;
+              const double Score = (double)cbv * tb.bmv - (double)e;
+              if (Score > MaxScore) { MaxScore = Score; MSL = cbv; MSd = cbd; MSe = e; }
             }
-            pv = cv;
-            cv = nv;
-            d++;
-            row_end = d > R;
+            if (e >= lim) { dp_mte = false; dp_e = MSe; dp_d = MSd; state = LN_DPEND; go = false; }
+          }
+        }
+        if (go) {
+          e++;
+          if (STATS) rows++;
+          oR = R;
+          L = L - 1;                             // max(Left - 1, -e): Left >= -(e - 1)
+          R = R + 1;
+          if (R - L + 1 > CAP) { dp_ovf = true; state = LN_DPEND; }
+          else {
+            eml = tb.match_limit[e];
+            d = L;
+            pv = -2;
+            cv = -2;
+            nL = NONE; nR = 0; cbv = -1; cbd = 0; pbv = -1; pbd = 0;
+            // The next row's cells start at or after the lowest A and T positions this row's cells
+            // started from (Edit_Array rows only grow, and pruning drops the lowest).  Move the
+            // staged words there when they lag by LN_SW / 4 words or a cell read past them.
+            const int32_t ka = (ra + mina) >> 4, kb = (rb + minb) >> 4;
+            if (fell || ka - sta >= LN_SW / 4 || kb - stb >= LN_SW / 4) restage(ka, kb);
+            fell = false;
+            mina = 0x7fffffff;
+            minb = 0x7fffffff;
+            state = LN_CELL;
+          }
+        }
+      }
+      if (state == LN_DPEND) {
+        if (dp_reach) {
+          // the end of A or T was reached at (Row, d): branch-point test (forward.C:172-212)
+          dp_reach = false;
+          const double Score = (double)Row * tb.bmv - (double)e;
+          const int32_t Tail_Len = Row - MSL;
+          const double slope = (double)(MaxScore - Score) / (double)Tail_Len;
+          const bool abort_ = ((uint32_t)e > tb.min_branch_end_dist / 2u) && ((uint32_t)Tail_Len >= tb.min_branch_end_dist) &&
+                              (slope >= tb.slope_min);
+          dp_mte = !abort_;
+          dp_e = abort_ ? MSe : e;
+          dp_d = abort_ ? MSd : d;
+        }
+        if (dp_ovf) record(true, false);
+        else if (fwd) {
+          rmte = dp_mte;
+          right_e = dp_e;
+          exit_d += s_short ? dp_d : -dp_d;
+          if (!(rmte && begin_reverse())) record(false, rmte);
+        } else {
+          record(false, rmte && dp_mte);
+        }
+      }
+      const uint64_t want = __ballot(state == LN_FETCH);
+      if (want) {
+        uint32_t base = 0;
+        if (lane == (int)__builtin_ctzll(want)) base = atomicAdd(A.queue, (uint32_t)__builtin_popcountll(want));
+        base = __shfl(base, (int)__builtin_ctzll(want));
+        if (state == LN_FETCH) {
+          ti = (uint64_t)base + (uint64_t)__builtin_popcountll(want & ((1ull << lane) - 1));
+          if (ti >= ntasks) {
+            state = LN_DONE;
+          } else {
+            cells = 0; bases = 0; rows = 0; windows = 0;
+            rmte = true; fwd = true; dp_ovf = false; dp_reach = false;
+            const FwdTask tk = A.tasks[ti];
+            const PairRec pr = A.pairs[tk.pair];
+            const int4 nd = A.nodes[pr.hb + tk.node];
+            const int32_t S_Len = A.R.len[pr.s_id], T_Len = A.R.len[pr.t_id];
+            const int32_t Start = nd.z, Offset = nd.x, Len = nd.y;
+            const int32_t S_Right_Begin = Start + Len, S_Right_Len = S_Len - S_Right_Begin;
+            const int32_t T_Right_Begin = Offset + Len, T_Right_Len = T_Len - T_Right_Begin;
+            const int32_t Total_Olap = (Start < Offset ? Start : Offset) + Len + (S_Right_Len < T_Right_Len ? S_Right_Len : T_Right_Len);
+            limit = tb.error_bound[Total_Olap];
+            exit_d = T_Right_Begin - S_Right_Begin;
+            right_e = 0;
+            if (S_Right_Len == 0 || T_Right_Len == 0) {
+              // no forward DP: the right end is reached with 0 errors (prefixEditDistance-extend.C:81-99)
+              if (!begin_reverse()) record(false, true);
+            } else {
+              s_short = S_Right_Len <= T_Right_Len;   // the shorter tail is A
+              const int64_t s0 = (pr.dir ? A.R.b0r[pr.s_id] : A.R.b0f[pr.s_id]) + S_Right_Begin, t0 = A.R.b0f[pr.t_id] + T_Right_Begin;
+              const bool sN = A.R.hasN[pr.s_id], tN = A.R.hasN[pr.t_id];
+              if (s_short) begin_dp(s0, sN, S_Right_Len, t0, tN, T_Right_Len, limit);
+              else         begin_dp(t0, tN, T_Right_Len, s0, sN, S_Right_Len, limit);
+            }
           }
         }
       }
     }
-    if (row_end) {
-      // ---- the row is done: prune, take its first maximum, open the next row
-      bool go = true;
-      if (e > 0) {
-        if (nL == NONE) { dmte = false; de = MSe; dd = MSd; dp_end = true; go = false; }   // band emptied
-        else {
-          L = nL;
-          R = nR;
-          if (cbv > Longest) {
-            Longest = cbv;
-            const double Score = (double)cbv * tb.bmv - (double)e;
-            if (Score > MaxScore) { MaxScore = Score; MSL = cbv; MSd = cbd; MSe = e; }
+    const uint64_t live = __ballot(state != LN_DONE);
+    if (live == 0) break;
+    // ---------------------------------------------------------------- inner: one cell step per lane
+    // A step starts a cell (the furthest-reaching maximum of forward.C:155-160) or continues its
+    // slide by one window (:161-167); a finished cell is stored in place and updates the row's
+    // pruning range and first maximum.  The loop runs while three quarters of the live lanes are
+    // still in their row's cells, so the outer work above is done for many lanes at once.
+    const int nlive = __builtin_popcountll(live);
+    do {
+      if (STATS) iters++;
+      if (state == LN_CELL) {
+        if (!sliding) {
+          nv = (d + 1 <= oR) ? my[((d + 1) & MASK) * OIC_WAVE] : -2;
+          Row = cv + 1;
+          Row = pv > Row ? pv : Row;
+          Row = nv + 1 > Row ? nv + 1 : Row;
+          r0 = Row;
+          const int32_t r1 = m - Row, r2 = n - Row - d;
+          rem = r1 < r2 ? r1 : r2;
+          cont = false;
+          mina = Row < mina ? Row : mina;
+          minb = Row + d < minb ? Row + d : minb;
+        }
+        int32_t k = 0;
+        bool staged = false;
+        if (rem > 0) {
+          if (STATS) windows++;
+          const int32_t qa = ra + Row, qb = rb + Row + d;
+          const int32_t ia = (qa >> 4) - sta, ib = (qb >> 4) - stb;
+          staged = (uint32_t)ia < (uint32_t)(LN_SW - 1) && (uint32_t)ib < (uint32_t)(LN_SW - 1) && (na & nb) < 0;
+          if (staged) {
+            // 16 bases of each view from the staged words (two LDS reads each)
+            const uint32_t xa = __builtin_amdgcn_alignbit(sa[(ia + 1) * OIC_WAVE], sa[ia * OIC_WAVE], (uint32_t)(qa & 15) * 2u);
+            const uint32_t xb = __builtin_amdgcn_alignbit(sb[(ib + 1) * OIC_WAVE], sb[ib * OIC_WAVE], (uint32_t)(qb & 15) * 2u);
+            uint32_t x = xa ^ xb;
+            x = (x | (x >> 1)) & 0x55555555u;
+            k = x ? (int32_t)(__builtin_ctz(x) >> 1) : 16;
+          } else {
+            // past the staged words: from HBM, 64 bases at a time once a slide goes on
+            fell = true;
+            if (!cont) k = (na & nb) < 0 ? lane_cmp16(wa, qa, wb, qb) : lane_cmp16_n(wa, qa, wb, qb, A.R.nmask, na, nb);
+            else       k = lane_cmp64(wa, qa, wb, qb, A.R.nmask, na, nb);
           }
-          if (e >= lim) { dmte = false; de = MSe; dd = MSd; dp_end = true; go = false; }
+        }
+        const int32_t wlen = (cont && !staged) ? 64 : 16;
+        const bool last = k >= rem || k < wlen;
+        Row += k < rem ? k : rem;
+        rem -= wlen;
+        sliding = !last;
+        cont = !last;
+        if (last) {
+          cells++;
+          bases += (uint32_t)(Row - r0) + 1u;
+          if (row0) {
+            row0 = false;
+            if (Row == m) { dp_mte = true; dp_e = 0; dp_d = 0; state = LN_DPEND; }
+            else {
+              my[0] = Row;
+              e = 0; L = 0; R = 0;
+              mina = 0; minb = 0; fell = false;
+              if (lim < 1) { dp_mte = false; dp_e = 0; dp_d = 0; state = LN_DPEND; }
+              else state = LN_ROWEND;            // opens row 1
+            }
+          } else {
+            my[(d & MASK) * OIC_WAVE] = Row;
+            if (Row == m || Row + d == n) {
+              dp_reach = true;
+              state = LN_DPEND;
+            } else {
+              // "committed" is the best cell up to the latest kept cell, "pending" the best after it
+              const bool keep = Row + (d > 0 ? d : 0) >= eml;
+              if (keep) {
+                nL = nL == NONE ? d : nL;
+                nR = d;
+                if (pbv > cbv) { cbv = pbv; cbd = pbd; }
+                pbv = -1;
+                if (Row > cbv) { cbv = Row; cbd = d; }
+              } else if (nL != NONE && Row > pbv) {
+                pbv = Row;
+                pbd = d;
+              }
+              pv = cv;
+              cv = nv;
+              d++;
+              if (d > R) state = LN_ROWEND;
+            }
+          }
         }
       }
-      if (go) {
-        e++;
-        if (STATS) rows++;
-        oR = R;
-        L = L - 1;                               // max(Left - 1, -e): Left >= -(e - 1)
-        R = R + 1;
-        if (R - L + 1 > CAP) { dovf = true; dp_end = true; }
-        else {
-          eml = tb.match_limit[e];
-          d = L;
-          pv = -2;
-          cv = -2;
-          nL = NONE; nR = 0; cbv = -1; cbd = 0; pbv = -1; pbd = 0;
-        }
-      }
-    }
-    if (dp_end) {
-      if (state == 1 && !dovf) {
-        rmte = dmte;
-        right_e = de;
-        exit_d += s_short ? dd : -dd;
-        if (!(rmte && begin_reverse())) state = 0;
-        else state = 2;
-      } else {
-        state = 0;
-      }
-      if (state == 0) {
-        // record the task's outcome: 1 not a dovetail (Process_Matches only removes the node),
-        // 2 dovetail, 0 not computed (band outgrew the tile)
-        const FwdTask tk = A.tasks[ti];
-        const uint64_t gnode = A.pairs[tk.pair].hb + tk.node;
-        A.cache[gnode] = dovf ? 0 : ((rmte && dmte) ? 2 : 1);
-        if (dovf && A.ovf) A.ovf[atomicAdd(A.novf, 1u)] = tk;
-        A.ncells[gnode] = cells;
-        A.nexit[gnode] = exit_d;
-        if (!dovf) cells_acc += cells;
-        bases_acc += bases;
-        rows_acc += rows;
-        win_acc += windows;
-      }
-    }
+    } while (4 * __builtin_popcountll(__ballot(state == LN_CELL)) >= 3 * nlive);
   }
   for (int o = 32; o > 0; o >>= 1) {
     cells_acc += __shfl_xor(cells_acc, o);

@@ -943,14 +943,16 @@ template <int CAP>
 __device__ ExtOut wave_extend(int32_t Start, int32_t Offset, int32_t Len, const SeqV S, const SeqV Src, int32_t S_Len,
                               const SeqV T, const SeqV Trc, int32_t T_Len, const SearchConst &C, int32_t *bufA,
                               int32_t *bufB, const DPScratch &sc, uint64_t &cells, uint64_t &slots, uint32_t &dpcalls,
-                              uint64_t &lane_bases) {
+                              uint64_t &lane_bases, bool known_dovetail = false) {
   const int lane = lane_id();
   ExtOut r = {};
   const int32_t S_Left_Begin = Start - 1, S_Right_Begin = Start + Len, S_Right_Len = S_Len - S_Right_Begin;
   const int32_t T_Left_Begin = Offset - 1, T_Right_Begin = Offset + Len, T_Right_Len = T_Len - T_Right_Begin;
   const int32_t Total_Olap = (Start < Offset ? Start : Offset) + Len + (S_Right_Len < T_Right_Len ? S_Right_Len : T_Right_Len);
   const int32_t Error_Limit = C.tb.error_bound[Total_Olap];
-  const bool need_delta = !C.elide || C.partial;   // with elision, deltas only after both sides end
+  // With elision, deltas only after both sides end, by a second run with row storage; an extension
+  // known to be a dovetail (its forward DPs already ran ahead) keeps rows in its only run.
+  const bool need_delta = !C.elide || C.partial || known_dovetail;
 
   int32_t Right_Errors = 0, Left_Errors = 0, Leftover = 0, nr = 0, nl = 0;
   bool rmte = true, lmte = true;
@@ -1220,6 +1222,16 @@ struct PairOut {
 #define STAT_PPCELLS 13   // cells of the DPs this kernel runs itself (full extensions)
 #define STAT_NSTATS  16
 
+// A node's cached outcome in one 64-bit word, so a finished task makes one store: the kind in bits
+// 62-63 (0 not computed, 1 not a dovetail, 2 dovetail), the right-end diagonal (T_Hi - S_Hi) of
+// the run in bits 32-61 (signed), its DP cells in bits 0-31.  A zeroed word reads "not computed".
+__host__ __device__ __forceinline__ uint64_t pack_outcome(int kind, int32_t exit_d, uint32_t cells) {
+  return ((uint64_t)(uint32_t)kind << 62) | (((uint64_t)(uint32_t)exit_d & 0x3FFFFFFFull) << 32) | (uint64_t)cells;
+}
+__host__ __device__ __forceinline__ int outcome_kind(uint64_t o) { return (int)(o >> 62); }
+__host__ __device__ __forceinline__ uint32_t outcome_cells(uint64_t o) { return (uint32_t)o; }
+__host__ __device__ __forceinline__ int32_t outcome_exit(uint64_t o) { return (int32_t)((int64_t)(o << 2) >> 34); }
+
 struct PairKernelArgs {
   ReadsDev R;
   SearchConst C;
@@ -1231,9 +1243,7 @@ struct PairKernelArgs {
   PairOut *pout;             // [np]
   uint32_t *queue;           // work counter
   unsigned long long *stats; // [STAT_NSTATS]
-  const uint8_t *cache;      // per node forward outcome (1 = ended at a branch point), or nullptr
-  const uint32_t *ncells;    // per node cells of that forward run
-  const int32_t *nexit;      // per node right-end diagonal of that run
+  const uint64_t *outc;      // per node cached forward-extension outcome (pack_outcome), or nullptr
   DPScratch sc0;             // scratch of wave 0; wave w adds w * stride
   int64_t row_stride, val_stride, delta_stride;
 };
@@ -1319,7 +1329,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_process_pairs(PairKernelArgs A) {
     __syncthreads();
     bool overflow = false;
     bool first = true;
-    const uint8_t *cache = A.cache ? A.cache + pr.hb : nullptr;
+    const uint64_t *outc = A.outc ? A.outc + pr.hb : nullptr;   // cached forward-extension outcomes (pack_outcome)
     while (true) {
       // first node with the maximal Len (Process_String_Overlaps.C:424-431)
       int32_t bl = -1, bi = 0x7fffffff;
@@ -1329,25 +1339,26 @@ __global__ void __launch_bounds__(OIC_WAVE) k_process_pairs(PairKernelArgs A) {
       }
       wave_argmax_first(bl, bi);
       if (bl < 0) break;
-      if (cache && cache[bi] == 1) {
+      if (outc && outcome_kind(outc[bi]) == 1) {
         // A node whose extension is known not to be a dovetail is only removed (:473-490); so are
         // all such nodes that would be picked before the first node without that knowledge.
         int32_t ul = -1, ui = 0x7fffffff;
         for (int32_t i = lane; i < nn; i += OIC_WAVE) {
           const int4 nd = nodes[i];
-          if (!nd.w && cache[i] != 1 && nd.y > ul) { ul = nd.y; ui = i; }
+          if (!nd.w && outcome_kind(outc[i]) != 1 && nd.y > ul) { ul = nd.y; ui = i; }
         }
         wave_argmax_first(ul, ui);
         uint64_t cnt = 0, cs = 0;
         if (pr.consistent) {
-          if (lane == 0) { cnt = 1; cs = A.ncells[pr.hb + bi]; nodes[bi].w = 1; }
+          if (lane == 0) { cnt = 1; cs = outcome_cells(outc[bi]); nodes[bi].w = 1; }
         } else {
           for (int32_t i = lane; i < nn; i += OIC_WAVE) {
             const int4 nd = nodes[i];
-            if (!nd.w && cache[i] == 1 && (nd.y > ul || (nd.y == ul && i < ui))) {
+            const uint64_t oi = outc[i];
+            if (!nd.w && outcome_kind(oi) == 1 && (nd.y > ul || (nd.y == ul && i < ui))) {
               nodes[i].w = 1;
               cnt++;
-              cs += A.ncells[pr.hb + i];
+              cs += outcome_cells(oi);
             }
           }
         }
@@ -1355,7 +1366,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_process_pairs(PairKernelArgs A) {
         exts += (uint32_t)cnt;
         po.n_ext += (int32_t)cnt;
         cells += cs;
-        if (first) { po.exit_d = A.nexit[pr.hb + bi]; first = false; }
+        if (first) { po.exit_d = outcome_exit(outc[bi]); first = false; }
         __threadfence_block();
         __syncthreads();
         if (pr.consistent) break;
@@ -1363,7 +1374,8 @@ __global__ void __launch_bounds__(OIC_WAVE) k_process_pairs(PairKernelArgs A) {
       }
       const int4 lm = nodes[bi];
       const uint64_t cx = cells;
-      ExtOut x = wave_extend<CAP>(lm.z, lm.x, lm.y, S, Src, S_Len, T, Trc, T_Len, C, bufA, bufB, sc, cells, slots, dpcalls, lane_bases);
+      ExtOut x = wave_extend<CAP>(lm.z, lm.x, lm.y, S, Src, S_Len, T, Trc, T_Len, C, bufA, bufB, sc, cells, slots, dpcalls, lane_bases,
+                                  outc && outcome_kind(outc[bi]) == 2);
       own_cells += cells - cx;
       exts++;
       po.n_ext++;

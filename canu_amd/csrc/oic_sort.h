@@ -1,6 +1,5 @@
-// oic_sort.h -- device-wide exclusive scan for gfx950 (tiles of 256 threads x 8 items, tile sums
-// scanned recursively).  Key/value sorts go through rocPRIM's stable device radix sort
-// (sort_pairs_rocprim in oic.hip).
+// oic_sort.h -- device-wide exclusive scan (tiles of 256 threads x 8 items, tile sums scanned
+// recursively) and a stable LSD radix sort of (uint64 key, uint64 value) pairs for gfx950.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -105,4 +104,86 @@ static hipError_t scan_excl(const T *in, uint64_t *out, uint64_t n, uint64_t *sc
     *total_host = last_out + (uint64_t)last_in;
   }
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- stable LSD radix sort
+//
+// One pass per BITS-bit digit (8 to 11 bits, chosen per sort so that few passes cover the key
+// bits).  A pass: k_rs_hist counts each tile's digits (4096 keys per tile, LDS histogram); the
+// counts, laid out digit-major (count[digit][tile]), are exclusively scanned, which gives every
+// (digit, tile) its first output slot; k_rs_scatter then ranks the tile's keys among equal digits in
+// input order -- a round of 256 keys at a time: within a wavefront by matching digits with one
+// ballot per digit bit, across the 4 wavefronts through LDS counts, across rounds through a running
+// count per digit -- and writes each pair to its slot.  Traffic per key and pass: the key read
+// twice, the value once, key and value written (40 B).
+
+#define RS_THREADS 256
+#define RS_ITEMS   16
+#define RS_TILE    (RS_THREADS * RS_ITEMS)
+#define RS_WAVES   (RS_THREADS / 64)
+
+template <int BITS>
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t *__restrict__ keys, uint64_t n, int shift,
+                                                        uint32_t *__restrict__ counts, uint32_t ntiles) {
+  constexpr int NB = 1 << BITS;
+  __shared__ uint32_t h[NB];
+  for (int i = threadIdx.x; i < NB; i += RS_THREADS) h[i] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+#pragma unroll 4
+  for (int r = 0; r < RS_ITEMS; r++) {
+    const uint64_t i = base + (uint64_t)r * RS_THREADS + threadIdx.x;
+    if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & (NB - 1)], 1u);
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < NB; b += RS_THREADS) counts[(uint64_t)b * ntiles + blockIdx.x] = h[b];
+}
+
+template <int BITS>
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t *__restrict__ kin, const uint64_t *__restrict__ vin,
+                                                           uint64_t *__restrict__ kout, uint64_t *__restrict__ vout, uint64_t n,
+                                                           int shift, const uint64_t *__restrict__ offs, uint32_t ntiles) {
+  constexpr int NB = 1 << BITS;
+  __shared__ uint64_t next[NB];              // this tile's next output slot per digit
+  __shared__ uint32_t wcnt[RS_WAVES][NB];    // keys of each digit in each wavefront, this round
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int b = tid; b < NB; b += RS_THREADS) {
+    next[b] = offs[(uint64_t)b * ntiles + blockIdx.x];
+#pragma unroll
+    for (int x = 0; x < RS_WAVES; x++) wcnt[x][b] = 0;
+  }
+  __syncthreads();
+  const uint64_t below = (1ull << lane) - 1ull;
+  const uint64_t base = (uint64_t)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ITEMS; r++) {
+    const uint64_t i = base + (uint64_t)r * RS_THREADS + tid;
+    const bool valid = i < n;
+    uint64_t k = 0, v = 0;
+    if (valid) { k = kin[i]; v = vin[i]; }
+    const uint32_t dg = (uint32_t)(k >> shift) & (NB - 1);
+    // lanes of this wavefront holding the same digit
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < BITS; b++) {
+      const uint64_t bl = __ballot((dg >> b) & 1u);
+      peers &= ((dg >> b) & 1u) ? bl : ~bl;
+    }
+    const uint32_t rank = (uint32_t)__builtin_popcountll(peers & below);
+    const uint32_t cnt = (uint32_t)__builtin_popcountll(peers);
+    const bool leader = valid && rank == 0;
+    if (leader) wcnt[w][dg] = cnt;
+    __syncthreads();
+    if (valid) {
+      uint64_t pos = next[dg] + rank;
+      for (int x = 0; x < w; x++) pos += wcnt[x][dg];
+      kout[pos] = k;
+      vout[pos] = v;
+    }
+    __syncthreads();
+    if (leader) {
+      atomicAdd((unsigned long long *)&next[dg], (unsigned long long)cnt);
+      wcnt[w][dg] = 0;
+    }
+    __syncthreads();
+  }
 }

@@ -81,10 +81,11 @@ def prediction_error(targ, pred, feat):
             "learned_rmse_log": float(np.sqrt(np.mean((pred - targ[:, 0]) ** 2)))}
 
 
-def train_for(eng, R=None, engine_kw=None, genome_len=25_000_000, seed=4, epochs=12, max_traces=2_000_000):
-    """Collect traces on a fresh c3-generator instance, train eng's model on the GPU; report quality."""
+def train_for(eng, R=None, engine_kw=None, genome_len=25_000_000, seed=4, epochs=12, max_traces=2_000_000, preset="c3"):
+    """Collect traces on a fresh instance (another seed) of a workload generator, train eng's model on
+    the GPU; report quality."""
     t0 = time.perf_counter()
-    f, t = collect_traces("c3", genome_len, seed, {k: v for k, v in (engine_kw or {}).items() if k != "schedule"},
+    f, t = collect_traces(preset, genome_len, seed, {k: v for k, v in (engine_kw or {}).items() if k != "schedule"},
                           max_traces=max_traces)
     t_tr = time.perf_counter() - t0
     n = len(f)

@@ -89,5 +89,9 @@ def test_gpu_learned_model_beats_static_binning():
     ms = learned.ordering_gain(fte, tte, pred, workers=4096, chunk=80_000)
     print(f"\nheld-out: {err}\nmakespans (cells): {ms}")
     assert err["learned_spearman"] > err["static_spearman"] + 0.02
-    assert ms["learned"] <= 1.02 * ms["static"]
+    # With 4096 workers a chunk's makespan is usually its longest job, which both longest-first
+    # orders reach: the learned order must stay within 1% of the ideal one.  The strict gain in
+    # measured DP time is test_zy_learned_gain's.
+    assert ms["learned"] <= ms["static"]
+    assert ms["learned"] <= 1.01 * ms["ideal"]
     assert ms["learned"] < ms["plain"]
