@@ -198,9 +198,10 @@ def main():
     first = 1
     timed_end = first + (args.warmup + args.steps) * args.batch - 1
     if args.fwd_lanes:
-        eng.set_tuning(fwd_lanes=args.fwd_lanes, fwd_tile=args.fwd_tile or 128)
+        shape = dict(fwd_lanes=args.fwd_lanes, fwd_tile=args.fwd_tile or 128)
         if args.fwd_loop >= 0:
-            eng.set_tuning(fwd_loop=args.fwd_loop)
+            shape["fwd_loop"] = args.fwd_loop
+        eng.set_tuning(**shape)
     if args.fwd_counters:
         eng.set_tuning(fwd_counters=1)
     tuning = eng.tuning()
@@ -223,7 +224,8 @@ def main():
     if lanes == 1:
         fwd_name = f"k_fwd_lane<{tile}, {stats_flag}>"
     else:
-        fwd_name = f"{'k_fwd_tasks' if tuning.get('fwd_loop') else 'k_fwd_sm'}<{lanes}, {tile}, {stats_flag}>"
+        form = {0: "k_fwd_sm", 1: "k_fwd_tasks", 2: "k_fwd_flat"}[tuning.get("fwd_loop", 0)]
+        fwd_name = f"{form}<{lanes}, {tile}, {stats_flag}>"
     kernels = [dict(kernel="k_process_pairs<512>", ms=d["ms_pp_kernel"], cells=d["pp_cells"], bases=d["pp_bases"],
                     launches=d["pp_launches"],
                     timing="HIP events around each launch on its stream")]

@@ -22,6 +22,7 @@
 #include "oic_search.h"
 #include "oic_sched.h"
 #include "oic_batch.h"
+#include "oic_flat.h"
 #include "oic_lane.h"
 #include "oic_mlp.h"
 #include <random>
@@ -365,7 +366,13 @@ extern "C" int oic_set_tuning(oic_ctx *c, const oic_tuning *t) {
                            : (t->fwd_tile == 128 || t->fwd_tile == 256 || (t->fwd_tile == 512 && t->fwd_lanes != 4) ||
                               ((t->fwd_tile == 64 || t->fwd_tile == 32) && (t->fwd_lanes == 8 || t->fwd_lanes == 16)));
   if (t->fwd_order > 1) return fail(c, OIC_EINVAL, "fwd_order must be 0 or 1");
-  if (t->fwd_loop > 1) return fail(c, OIC_EINVAL, "fwd_loop must be 0 or 1");
+  if (t->fwd_loop > 2) return fail(c, OIC_EINVAL, "fwd_loop must be 0, 1 or 2");
+  if (t->fwd_loop == 2) {
+    if ((t->fwd_lanes != 4 && t->fwd_lanes != 8) || (t->fwd_tile != 32 && t->fwd_tile != 64 && t->fwd_tile != 128))
+      return fail(c, OIC_EINVAL, "fwd_loop 2 (flat rows) runs 4 or 8 lanes per DP with fwd_tile 32, 64 or 128");
+    c->tune = *t;
+    return OIC_OK;
+  }
   if (!lanes_ok || !tile_ok)
     return fail(c, OIC_EINVAL, "fwd_lanes must be 1 (tile 16, 32 or 64), or 4, 8, 16 or 32 with fwd_tile 32 or 64 (8 or 16 lanes), "
                                "128, 256 or 512 (not with 4 lanes)");
@@ -742,10 +749,24 @@ static int predict_pairs(oic_ctx *c, uint64_t np, std::vector<float> &pred) {
 // the cost model's per-pair predictions (on the device) when they order this batch, else null
 static const float *spec_pred(oic_ctx *c) { return c->batch_pred ? c->m_pred.as<float>() : nullptr; }
 
+// shapes that only the flat-row form runs (4 lanes per DP with a tile of 64 cells or fewer)
+template <int G, int CAP>
+constexpr bool flat_only() { return G == 4 && CAP <= 64; }
+template <int G, int CAP>
+constexpr bool has_flat() { return (G == 4 || G == 8) && CAP <= 128; }
+
 // the packed kernel in the loop form the tuning selects (one DP per wavefront: the per-task loop)
 template <int G, int CAP, bool STATS>
 static void launch_fwd(oic_ctx *c, const FwdArgs &FA, uint32_t waves) {
-  if constexpr (G == 1) {
+  if constexpr (has_flat<G, CAP>()) {
+    if (c->tune.fwd_loop == 2 || flat_only<G, CAP>()) {
+      hipLaunchKernelGGL((k_fwd_flat<G, CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
+      return;
+    }
+  }
+  if constexpr (flat_only<G, CAP>()) {
+    return;
+  } else if constexpr (G == 1) {
     hipLaunchKernelGGL((k_fwd_lane<CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
     return;
   } else if constexpr (G < 64) {
@@ -754,25 +775,32 @@ static void launch_fwd(oic_ctx *c, const FwdArgs &FA, uint32_t waves) {
       return;
     }
   }
-  hipLaunchKernelGGL((k_fwd_tasks<G, CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
+  if constexpr (!flat_only<G, CAP>())
+    hipLaunchKernelGGL((k_fwd_tasks<G, CAP, STATS>), dim3(waves), dim3(OIC_WAVE), 0, c->st, FA);
 }
 
 // persistent wavefronts for a packed launch: as many as fit on every CU at once (registers and LDS
 // decide), but no more than the tasks can use
 template <int G, int CAP, bool STATS>
 static uint32_t fwd_waves(oic_ctx *c, uint64_t max_tasks) {
-  static int cached[2] = {0, 0};   // per loop form; the same on every device of one architecture
-  const int sm = (G < 64 && c->tune.fwd_loop == 0) ? 1 : 0;
+  static int cached[3] = {0, 0, 0};   // per loop form; the same on every device of one architecture
+  const int sm = (flat_only<G, CAP>() || (has_flat<G, CAP>() && c->tune.fwd_loop == 2)) ? 2
+                 : (G < 64 && c->tune.fwd_loop == 0) ? 1 : 0;
   int per_cu = cached[sm];
   if (per_cu < 1) {
     hipError_t e = hipSuccess;
-    if constexpr (G == 1) {
+    if constexpr (flat_only<G, CAP>()) {
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_flat<G, CAP, STATS>, OIC_WAVE, 0);
+    } else if constexpr (G == 1) {
       e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_lane<CAP, STATS>, OIC_WAVE, 0);
     } else {
+      if constexpr (has_flat<G, CAP>()) {
+        if (sm == 2) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_flat<G, CAP, STATS>, OIC_WAVE, 0);
+      }
       if constexpr (G < 64) {
         if (sm) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_sm<G, CAP, STATS>, OIC_WAVE, 0);
       }
-      if (!sm) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_tasks<G, CAP, STATS>, OIC_WAVE, 0);
+      if (sm == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd_tasks<G, CAP, STATS>, OIC_WAVE, 0);
     }
     if (e != hipSuccess || per_cu < 1) { (void)hipGetLastError(); per_cu = 16; }
     cached[sm] = per_cu;
@@ -805,6 +833,7 @@ static int run_fwd(oic_ctx *c, FwdArgs FA, uint64_t n, double *ms) {
   c->ctr.spec_bases += st[4];
   c->ctr.spec_slide_iters += st[5];
   c->ctr.spec_wave_slide_iters += st[6];
+  c->ctr.spec_wave_chunks += st[7];
   if (G < 64) {   // the packed narrow-band kernel
     c->ctr.fwd_launches++;
     c->ctr.fwd_cells += st[0];
@@ -820,7 +849,7 @@ static int run_fwd_narrow(oic_ctx *c, const FwdArgs &FA, uint64_t n, double *ms)
 #define FWD_CASE(G_, T_) \
   if (g == G_ && t == T_) return c->tune.fwd_counters ? run_fwd<G_, T_, true>(c, FA, n, ms) : run_fwd<G_, T_, false>(c, FA, n, ms);
   FWD_CASE(1, 16) FWD_CASE(1, 32) FWD_CASE(1, 64)
-  FWD_CASE(4, 128) FWD_CASE(4, 256)
+  FWD_CASE(4, 32) FWD_CASE(4, 64) FWD_CASE(4, 128) FWD_CASE(4, 256)
   FWD_CASE(8, 32) FWD_CASE(8, 64) FWD_CASE(8, 128) FWD_CASE(8, 256) FWD_CASE(8, 512)
   FWD_CASE(16, 32) FWD_CASE(16, 64) FWD_CASE(16, 128) FWD_CASE(16, 256) FWD_CASE(16, 512)
   FWD_CASE(32, 128) FWD_CASE(32, 256) FWD_CASE(32, 512)
@@ -932,7 +961,7 @@ static int enqueue_fwd_narrow(oic_ctx *c, const FwdArgs &FA, uint64_t max_tasks,
     return OIC_OK;                                                                                         \
   }
   FWD_CASE(1, 16) FWD_CASE(1, 32) FWD_CASE(1, 64)
-  FWD_CASE(4, 128) FWD_CASE(4, 256)
+  FWD_CASE(4, 32) FWD_CASE(4, 64) FWD_CASE(4, 128) FWD_CASE(4, 256)
   FWD_CASE(8, 32) FWD_CASE(8, 64) FWD_CASE(8, 128) FWD_CASE(8, 256) FWD_CASE(8, 512)
   FWD_CASE(16, 32) FWD_CASE(16, 64) FWD_CASE(16, 128) FWD_CASE(16, 256) FWD_CASE(16, 512)
   FWD_CASE(32, 128) FWD_CASE(32, 256) FWD_CASE(32, 512)
@@ -1073,6 +1102,7 @@ static int run_forward_passes_graph(oic_ctx *c, uint64_t np, uint64_t H, uint64_
     c->ctr.spec_bases += x[4];
     c->ctr.spec_slide_iters += x[5];
     c->ctr.spec_wave_slide_iters += x[6];
+    c->ctr.spec_wave_chunks += x[7];
   }
   for (int i : {0, 3}) {   // the packed narrow-band launches
     if (i == 3 && nt == 0) continue;
@@ -1559,23 +1589,38 @@ extern "C" int oic_autotune(oic_ctx *c, uint32_t bgn_ref, uint32_t end_ref, oic_
         oic_tuning x = cur;
         x.fwd_lanes = g;
         x.fwd_tile = t;
+        if (x.fwd_loop == 2) x.fwd_loop = 0;   // the flat form is tried with the loop forms below
         rc = trial(x, 0, &ms);
         if (rc) { c->ctr = saved; return rc; }
         if (better(ms, bms)) { bms = ms; b = x; }
       }
     cur = b;
   }
-  if (c->P.schedule != OIC_SCHED_PLAIN && !c->P.partial && cur.fwd_lanes > 1) {
-    // the loop form (one DP per lane has a single form)
+  if (c->P.schedule != OIC_SCHED_PLAIN && !c->P.partial) {
+    // the loop form: the other of row-stepped and per-task for the chosen shape (one DP per lane
+    // has a single form), and flat rows with 4 and 8 lanes per DP and 64-cell tiles
     oic_tuning b = cur;
     double bms = 0;
     rc = trial(cur, 0, &bms);
     if (rc) { c->ctr = saved; return rc; }
-    oic_tuning x = cur;
-    x.fwd_loop = 1 - cur.fwd_loop;
-    rc = trial(x, 0, &ms);
-    if (rc) { c->ctr = saved; return rc; }
-    if (better(ms, bms)) b = x;
+    std::vector<oic_tuning> cand;
+    if (cur.fwd_lanes > 1 && cur.fwd_loop < 2) {
+      oic_tuning x = cur;
+      x.fwd_loop = 1 - cur.fwd_loop;
+      cand.push_back(x);
+    }
+    for (uint32_t g : {4u, 8u}) {
+      oic_tuning x = cur;
+      x.fwd_lanes = g;
+      x.fwd_tile = 64;
+      x.fwd_loop = 2;
+      if (!(cur.fwd_loop == 2 && cur.fwd_lanes == g && cur.fwd_tile == 64)) cand.push_back(x);
+    }
+    for (const oic_tuning &x : cand) {
+      rc = trial(x, 0, &ms);
+      if (rc) { c->ctr = saved; return rc; }
+      if (better(ms, bms)) { bms = ms; b = x; }
+    }
     cur = b;
   }
   if (c->P.schedule != OIC_SCHED_PLAIN && !c->P.partial) {

@@ -82,12 +82,13 @@ __device__ __forceinline__ Staged make_staged(int64_t b0, const uint32_t *lds) {
   return t;
 }
 
-// 16 bases from view position p (p >= 0); words = all packed reads
+// 16 bases from view position p (p >= 0); words = all packed reads; SW words of the tail are staged
+template <int SW = OIC_SEQ_WIN>
 __device__ __forceinline__ uint32_t win16_staged(const uint32_t *__restrict__ words, const Staged &t, int32_t p) {
   const int32_t q = t.r0 + p;
   const int32_t wi = q >> 4;
   const uint32_t sh = (uint32_t)(q & 15) * 2u;
-  if (wi + 1 < OIC_SEQ_WIN) return __builtin_amdgcn_alignbit(t.lds[wi + 1], t.lds[wi], sh);
+  if (wi + 1 < SW) return __builtin_amdgcn_alignbit(t.lds[wi + 1], t.lds[wi], sh);
   return __builtin_amdgcn_alignbit(words[t.w0 + wi + 1], words[t.w0 + wi], sh);
 }
 
@@ -460,14 +461,15 @@ struct SMDP {
 struct SMRes {
   bool mte, overflow;
   int32_t e, d;
-  uint32_t cells, rows, slots, wrows, bases, siters, wsiters;
+  uint32_t cells, rows, slots, wrows, bases, siters, wsiters, wchunks;
 };
 
 // 16 bases of both tails from positions pa (in a) and pb (in b), compared: the number of leading
 // equal bases (16 if all are)
+template <int SW = OIC_SEQ_WIN>
 __device__ __forceinline__ int32_t sm_match16(const SMDP &s, const uint32_t *__restrict__ words,
                                               const uint32_t *__restrict__ nmask, int32_t pa, int32_t pb) {
-  uint32_t x = win16_staged(words, s.ta, pa) ^ win16_staged(words, s.tb, pb);
+  uint32_t x = win16_staged<SW>(words, s.ta, pa) ^ win16_staged<SW>(words, s.tb, pb);
   x = (x | (x >> 1)) & 0x55555555u;
   if (s.nA) x &= ~nwin16(nmask, s.a_b0 + pa);
   if (s.nB) x &= ~nwin16(nmask, s.b_b0 + pb);
@@ -477,8 +479,9 @@ __device__ __forceinline__ int32_t sm_match16(const SMDP &s, const uint32_t *__r
 // the five words from word wi of a staged tail: from its LDS copy when they lie inside it, else
 // from HBM (the packed reads carry 64 guard bases per strand and slack after the last one, so the
 // words past a view's end exist)
+template <int SW = OIC_SEQ_WIN>
 __device__ __forceinline__ void words5_staged(const uint32_t *__restrict__ words, const Staged &t, int32_t wi, uint32_t (&w)[5]) {
-  if (wi + 4 < OIC_SEQ_WIN) {
+  if (wi + 4 < SW) {
 #pragma unroll
     for (int k = 0; k < 5; k++) w[k] = t.lds[wi + k];
   } else {
@@ -490,11 +493,12 @@ __device__ __forceinline__ void words5_staged(const uint32_t *__restrict__ words
 
 // the three words from word wi of a staged tail: read from its LDS copy (at a clamped index, so
 // always in bounds), and reloaded from HBM by the lanes whose words lie past the copy
+template <int SW = OIC_SEQ_WIN>
 __device__ __forceinline__ void words3_staged(const uint32_t *__restrict__ words, const Staged &t, int32_t wi, uint32_t &w0,
                                               uint32_t &w1, uint32_t &w2) {
-  const int32_t wl = wi < OIC_SEQ_WIN - 3 ? wi : OIC_SEQ_WIN - 3;
+  const int32_t wl = wi < SW - 3 ? wi : SW - 3;
   w0 = t.lds[wl]; w1 = t.lds[wl + 1]; w2 = t.lds[wl + 2];
-  if (wi > OIC_SEQ_WIN - 3) {
+  if (wi > SW - 3) {
     const uint32_t *g = words + (t.w0 + wi);
     w0 = g[0]; w1 = g[1]; w2 = g[2];
   }
@@ -503,12 +507,13 @@ __device__ __forceinline__ void words3_staged(const uint32_t *__restrict__ words
 // 32 bases of both tails from positions pa and pb, compared: the number of leading equal bases (32
 // if all are).  A slide's first window: at 8 % divergence about 2 % of cells slide past 16 bases,
 // and one such lane makes the whole wavefront run the 64-base continuation; past 32 bases far fewer.
+template <int SW = OIC_SEQ_WIN>
 __device__ __forceinline__ int32_t sm_match32(const SMDP &s, const uint32_t *__restrict__ words,
                                               const uint32_t *__restrict__ nmask, int32_t pa, int32_t pb) {
   const int32_t qa = s.ta.r0 + pa, qb = s.tb.r0 + pb;
   uint32_t a0, a1, a2, b0, b1, b2;
-  words3_staged(words, s.ta, qa >> 4, a0, a1, a2);
-  words3_staged(words, s.tb, qb >> 4, b0, b1, b2);
+  words3_staged<SW>(words, s.ta, qa >> 4, a0, a1, a2);
+  words3_staged<SW>(words, s.tb, qb >> 4, b0, b1, b2);
   const uint32_t sa = (uint32_t)(qa & 15) * 2u, sb = (uint32_t)(qb & 15) * 2u;
   uint32_t x0 = __builtin_amdgcn_alignbit(a1, a0, sa) ^ __builtin_amdgcn_alignbit(b1, b0, sb);
   uint32_t x1 = __builtin_amdgcn_alignbit(a2, a1, sa) ^ __builtin_amdgcn_alignbit(b2, b1, sb);
@@ -522,12 +527,13 @@ __device__ __forceinline__ int32_t sm_match32(const SMDP &s, const uint32_t *__r
 
 // 64 bases of both tails from positions pa and pb, compared: the number of leading equal bases (64
 // if all are).  The five words of each tail are loaded together.
+template <int SW = OIC_SEQ_WIN>
 __device__ __forceinline__ int32_t sm_match64(const SMDP &s, const uint32_t *__restrict__ words,
                                               const uint32_t *__restrict__ nmask, int32_t pa, int32_t pb) {
   const int32_t qa = s.ta.r0 + pa, qb = s.tb.r0 + pb;
   uint32_t a[5], b[5];
-  words5_staged(words, s.ta, qa >> 4, a);
-  words5_staged(words, s.tb, qb >> 4, b);
+  words5_staged<SW>(words, s.ta, qa >> 4, a);
+  words5_staged<SW>(words, s.tb, qb >> 4, b);
   const uint32_t sa = (uint32_t)(qa & 15) * 2u, sb = (uint32_t)(qb & 15) * 2u;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
@@ -542,14 +548,14 @@ __device__ __forceinline__ int32_t sm_match64(const SMDP &s, const uint32_t *__r
 
 // Stage the tails and run row 0 (the exact-prefix slide, prefixEditDistance-forward.C:113-132).
 // Returns true when the DP is already over (res holds its outcome).
-template <int G>
+template <int G, int SW = OIC_SEQ_WIN>
 __device__ __forceinline__ bool sm_start(SMDP &s, SMRes &res, const uint32_t *__restrict__ words,
                                          const uint32_t *__restrict__ nmask, int32_t *pbuf, uint32_t *sa, uint32_t *sb, int g,
                                          int gl) {
   res.mte = false; res.overflow = false; res.e = 0; res.d = 0;
   s.ta = make_staged(s.a_b0, sa);
   s.tb = make_staged(s.b_b0, sb);
-  for (int i = gl; i < OIC_SEQ_WIN; i += G) {
+  for (int i = gl; i < SW; i += G) {
     sa[i] = words[s.ta.w0 + i];
     sb[i] = words[s.tb.w0 + i];
   }
@@ -561,7 +567,7 @@ __device__ __forceinline__ bool sm_start(SMDP &s, SMRes &res, const uint32_t *__
     int32_t k = 0;
     bool stop = true;
     if (rem > 0) {
-      const int32_t kk = sm_match16(s, words, nmask, off, off);
+      const int32_t kk = sm_match16<SW>(s, words, nmask, off, off);
       if (kk >= rem) { k = rem; stop = true; } else { k = kk; stop = kk < 16; }
     }
     const uint64_t bl = group_ballot<G>(stop, g);
@@ -642,6 +648,7 @@ __device__ __forceinline__ bool sm_row(SMDP &s, SMRes &res, const uint32_t *__re
     const int32_t d = c0 + gl;
     const bool act = d <= Right;
     if (STATS) res.slots += G;
+    if (STATS && lane_id() == __builtin_ctzll(__ballot(1))) res.wchunks++;
     bool reach = false;
     int32_t Row = 0;
     if (act) {
@@ -673,7 +680,7 @@ __device__ __forceinline__ bool sm_row(SMDP &s, SMRes &res, const uint32_t *__re
         if (k64 < 64) break;
         rem -= 64;
       }
-      if (STATS) res.bases += (uint32_t)adv + 1u;
+      res.bases += (uint32_t)adv + 1u;
       reach = adv == rem0;                        // Row + adv is m, or Row + adv + d is n
       Row += adv;
       cbuf[d & MASK] = Row;
@@ -747,7 +754,7 @@ k_fwd_sm(FwdArgs A) {
   const int g = lane / G, gl = lane % G;
   int32_t *pbuf = tile[g][0], *cbuf = tile[g][1];
   uint64_t cells_acc = 0;
-  uint32_t rows_acc = 0, slots_acc = 0, wrows_acc = 0, bases_acc = 0, siters_acc = 0, wsiters_acc = 0;
+  uint32_t rows_acc = 0, slots_acc = 0, wrows_acc = 0, bases_acc = 0, siters_acc = 0, wsiters_acc = 0, wchunks_acc = 0;
   uint64_t ti = 0, tend = 0;
   const uint64_t ntasks = A.ntasks_dev ? (uint64_t)*A.ntasks_dev : A.ntasks;
   if (A.tstamp && lane == 0) atomicMin(&A.tstamp[0], (unsigned long long)wall_clock64());
@@ -801,6 +808,7 @@ k_fwd_sm(FwdArgs A) {
     bases_acc += res.bases;
     siters_acc += res.siters;
     wsiters_acc += res.wsiters;
+    wchunks_acc += res.wchunks;
     ti++;
     state = 0;
   };
@@ -822,7 +830,7 @@ k_fwd_sm(FwdArgs A) {
         tend = ti + FWD_CHUNK < ntasks ? ti + FWD_CHUNK : ntasks;
       }
       if (ti >= ntasks) { state = 3; break; }
-      res.cells = 0; res.rows = 0; res.slots = 0; res.wrows = 0; res.bases = 0; res.siters = 0; res.wsiters = 0;
+      res.cells = 0; res.rows = 0; res.slots = 0; res.wrows = 0; res.bases = 0; res.siters = 0; res.wsiters = 0; res.wchunks = 0;
       res.mte = true; res.overflow = false; res.e = 0; res.d = 0;
       lmte = true;
       const FwdTask tk = A.tasks[ti];
@@ -866,7 +874,7 @@ k_fwd_sm(FwdArgs A) {
     }
   }
   uint64_t rows64 = rows_acc, slots64 = slots_acc, wrows64 = wrows_acc, bases64 = bases_acc, si64 = siters_acc,
-           wsi64 = wsiters_acc;
+           wsi64 = wsiters_acc, wch64 = wchunks_acc;
   for (int o = 32; o > 0; o >>= 1) {
     cells_acc += __shfl_xor(cells_acc, o);
     rows64 += __shfl_xor(rows64, o);
@@ -875,6 +883,7 @@ k_fwd_sm(FwdArgs A) {
     bases64 += __shfl_xor(bases64, o);
     si64 += __shfl_xor(si64, o);
     wsi64 += __shfl_xor(wsi64, o);
+    wch64 += __shfl_xor(wch64, o);
   }
   if (A.tstamp && lane == 0) atomicMax(&A.tstamp[1], (unsigned long long)wall_clock64());
   if (lane == 0 && A.stats) {
@@ -882,6 +891,7 @@ k_fwd_sm(FwdArgs A) {
     atomicAdd(&A.stats[4], (unsigned long long)bases64);
     atomicAdd(&A.stats[5], (unsigned long long)si64);
     atomicAdd(&A.stats[6], (unsigned long long)wsi64);
+    atomicAdd(&A.stats[7], (unsigned long long)wch64);
     atomicAdd(&A.stats[0], (unsigned long long)cells_acc);
     atomicAdd(&A.stats[1], (unsigned long long)rows64);
     atomicAdd(&A.stats[2], (unsigned long long)slots64);

@@ -145,6 +145,7 @@ typedef struct oic_counters {
   uint64_t pp_bases;           /* bases it compares in their slides, plus one per cell */
   uint64_t learned_pairs;      /* pairs the learned cost model ordered (OIC_SCHED_LEARNED with a model) */
   uint64_t pp_launches;        /* launches of the Process_Matches kernel */
+  uint64_t spec_wave_chunks;   /* band chunk iterations wavefronts of the row-stepped kernel executed (with tuning.fwd_counters) */
 } oic_counters;
 
 /* Launch-shape knobs.  No setting changes a result; the defaults suit MI355X.  No reference
@@ -169,7 +170,9 @@ typedef struct oic_tuning {
   uint32_t fwd_loop;            /* packed forward-DP loop: 0 (default) advances every DP of a
                                    wavefront one row per iteration and starts a finished DP's next
                                    task at once; 1 runs each DP to its end, so finished DPs wait for
-                                   the slowest of their wavefront */
+                                   the slowest of their wavefront; 2 (flat rows; 4 or 8 lanes per
+                                   DP, fwd_tile 32, 64 or 128) is 0 with each row's cells of all the
+                                   wavefront's DPs spread over its 64 lanes */
 } oic_tuning;
 
 int oic_get_tuning(const oic_ctx *ctx, oic_tuning *out);

@@ -271,6 +271,15 @@ def test_forward_launch_shapes_identical(c3s):
                 res = eng.run()
                 assert_same(res, ref)
                 assert res.counters["dp_cells"] == ref["cells_used"], (loop, lanes, tile)
+    # flat rows: 4 or 8 lanes per DP for its own state, every row's cells over the whole wavefront
+    for lanes in (4, 8):
+        for tile in (32, 64, 128):
+            eng.set_tuning(fwd_lanes=lanes, fwd_tile=tile, fwd_loop=2)
+            res = eng.run()
+            assert_same(res, ref)
+            assert res.counters["dp_cells"] == ref["cells_used"], (2, lanes, tile)
+            if tile == 32:
+                assert res.counters["overflow_retries"] > 0
     eng.set_tuning(fwd_lanes=16, fwd_tile=128, fwd_order=1, fwd_loop=0)
     res = eng.run()
     assert_same(res, ref)
@@ -287,13 +296,16 @@ def test_forward_launch_shapes_identical(c3s):
     with pytest.raises(oic.OicError):
         eng.set_tuning(fwd_lanes=1, fwd_tile=128)
     with pytest.raises(oic.OicError):
-        eng.set_tuning(fwd_loop=2)
+        eng.set_tuning(fwd_loop=2)       # flat rows with 16 lanes per DP
+    with pytest.raises(oic.OicError):
+        eng.set_tuning(fwd_loop=3)
 
 
 def test_forward_loop_forms_do_the_same_work(c3s):
     """The row-stepped loop runs exactly the per-task loop's DPs: the same cells, rows, lane slots
     and bases per speculative pass, in fewer wavefront rows (its slides compare 64-base windows after
-    the first 16 bases, so window counts differ)."""
+    the first 32 bases, so window counts differ); the flat-row form the same cells and rows in fewer
+    band chunk iterations."""
     eng = oic.Engine(kmer=22, maxerate=0.045, minlength=500, threads=4, schedule=oic.SCHED_BATCHED)
     eng.load_reads(c3s)
     got = {}
@@ -304,6 +316,15 @@ def test_forward_loop_forms_do_the_same_work(c3s):
         assert got[0][k] == got[1][k], k
     assert got[0]["spec_rows"] > 0
     assert got[0]["spec_wave_rows"] < got[1]["spec_wave_rows"]
+    # flat rows run the same DPs with their cells spread over the wavefront: fewer chunk iterations
+    eng.set_tuning(fwd_lanes=8, fwd_tile=64, fwd_loop=2, fwd_counters=1)
+    flat = eng.run().counters
+    for k in ("spec_cells", "spec_rows", "dp_cells"):
+        assert flat[k] == got[0][k], k
+    # a row that reaches an end runs all its cells at once, so the cells past the reaching diagonal
+    # compare their bases too
+    assert got[0]["spec_bases"] <= flat["spec_bases"] <= 1.001 * got[0]["spec_bases"]
+    assert 0 < flat["spec_wave_chunks"] < got[0]["spec_wave_chunks"]
 
 
 def test_autotune_keeps_results(c3s):
@@ -312,11 +333,14 @@ def test_autotune_keeps_results(c3s):
     eng.load_reads(c3s)
     eng.build_hash_index(1, c3s.n)
     best, trials = eng.autotune(1, min(c3s.n, 60))
-    # warm-up, the incumbent and 17 other forward shapes, the loop form (shapes of more than one lane
-    # per DP), the task order, the incumbent list threshold and 4 others
-    assert len(trials) == 1 + 18 + (2 if best["fwd_lanes"] > 1 else 0) + 2 + 5
+    # warm-up, the incumbent and 17 other forward shapes; the loop form: the incumbent, the other of
+    # row-stepped and per-task (shapes of more than one lane per DP) and flat rows with 4 and 8 lanes
+    # per DP; the task order; the incumbent list threshold and 4 others
+    assert len(trials) in (1 + 18 + 3 + 2 + 5, 1 + 18 + 4 + 2 + 5)
     shapes = {(t["fwd_lanes"], t["fwd_tile"]) for t in trials if t["stage"] == "fwd"}
-    assert len(shapes) == 18 and (best["fwd_lanes"], best["fwd_tile"]) in shapes
+    assert len(shapes) == 19 and (best["fwd_lanes"], best["fwd_tile"]) in shapes
+    flat = {(t["fwd_lanes"], t["fwd_tile"]) for t in trials if t["stage"] == "fwd" and t["fwd_loop"] == 2}
+    assert flat == {(4, 64), (8, 64)}
     assert eng.tuning() == best
     assert_same(eng.run(), ref)
 
