@@ -110,18 +110,25 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
     L.pkey[lane] = 0ull;
   }
   L.start[lane] = -1;
-  // lanes per DP: the fewest chunk iterations C with sum over DPs of ceil(w / C) <= 64
+  // lanes per DP: the fewest chunk iterations C with sum over DPs of ceil(w / C) <= 64.  With A
+  // active DPs that sum is below W / C + A, so C = ceil(W / (64 - A)) always fits: try ceil(W / 64)
+  // and the next value first.
   const int32_t w = (rowg && !over && gl == 0) ? s.Right - s.Left + 1 : 0;
   const int32_t W = __builtin_amdgcn_readlane(wave_incl_scan(w), OIC_WAVE - 1);
+  const int32_t A = __builtin_popcountll(__ballot(w > 0));
   int32_t C = (W + OIC_WAVE - 1) / OIC_WAVE;
   int32_t nv = 0, incl = 0;
   if (W > 0) {
-    // C = the widest band gives every DP one lane (at most 64 / G DPs), so this ends by then
-    for (int it = 0; it < CAP; it++) {
-      nv = w > 0 ? (w + C - 1) / C : 0;
+    const int32_t Csafe = (W + (OIC_WAVE - A) - 1) / (OIC_WAVE - A);
+    for (int it = 0; it < 3; it++) {
+      // ceil(w / C) for w, C <= 2^12: the float quotient's floor is floor(w / C) or one less
+      // (exact quotients), so one correction step gives the ceiling
+      const float rc = __builtin_amdgcn_rcpf((float)C);
+      const int32_t q = (int32_t)((float)w * rc);
+      nv = q + ((int32_t)__umul24((uint32_t)q, (uint32_t)C) < w ? 1 : 0);
       incl = wave_incl_scan(nv);
       if (__builtin_amdgcn_readlane(incl, OIC_WAVE - 1) <= OIC_WAVE) break;
-      C++;
+      C = it == 0 ? C + 1 : Csafe;
     }
     if (gl == 0 && nv > 0) L.desc[g].nl = nv;
   }

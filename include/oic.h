@@ -155,7 +155,7 @@ typedef struct oic_tuning {
                                    with one wavefront, smaller ones with one thread (default 2048) */
   uint32_t fwd_lanes;           /* lanes per packed forward DP: 1 (64 DPs per wavefront, each lane
                                    walking its own band cell by cell), or 4, 8, 16 or 32 (64/fwd_lanes
-                                   DPs per wavefront, rows spread over the group; default 8) */
+                                   DPs per wavefront, rows spread over the group; default 4) */
   uint32_t fwd_tile;            /* band cells a packed forward DP holds in LDS: 16, 32 or 64 with one
                                    lane per DP; 32 or 64 (8 or 16 lanes), 128, 256 or 512 otherwise;
                                    wider bands re-run one per wavefront, with a 512-cell tile and then
@@ -167,12 +167,12 @@ typedef struct oic_tuning {
   uint32_t fwd_counters;        /* 1: count rows, lane slots and wavefront rows of the packed
                                    forward DPs (spec_rows, spec_lane_slots, spec_wave_rows); costs
                                    time, off by default */
-  uint32_t fwd_loop;            /* packed forward-DP loop: 0 (default) advances every DP of a
-                                   wavefront one row per iteration and starts a finished DP's next
-                                   task at once; 1 runs each DP to its end, so finished DPs wait for
-                                   the slowest of their wavefront; 2 (flat rows; 4 or 8 lanes per
-                                   DP, fwd_tile 32, 64 or 128) is 0 with each row's cells of all the
-                                   wavefront's DPs spread over its 64 lanes */
+  uint32_t fwd_loop;            /* packed forward-DP loop: 0 advances every DP of a wavefront one
+                                   row per iteration and starts a finished DP's next task at once;
+                                   1 runs each DP to its end, so finished DPs wait for the slowest
+                                   of their wavefront; 2 (default; flat rows, with 4 or 8 lanes per
+                                   DP and fwd_tile 32, 64 or 128) is 0 with each row's cells of all
+                                   the wavefront's DPs spread over its 64 lanes */
 } oic_tuning;
 
 int oic_get_tuning(const oic_ctx *ctx, oic_tuning *out);

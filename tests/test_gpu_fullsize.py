@@ -68,24 +68,26 @@ def test_c3_full_size_refs_match_oracle():
     R = readsim.simulate(readsim.PRESETS["c3"])
     assert R.n == 125023
     ref, ctr = check(R, C3_REFS, 0.045, 500, [
-        (oic.SCHED_BATCHED, dict(fwd_lanes=8, fwd_tile=64, list_wave_min_hits=256)),
+        (oic.SCHED_BATCHED, dict(fwd_lanes=4, fwd_tile=64, fwd_loop=2, list_wave_min_hits=256)),
+        (oic.SCHED_BATCHED, dict(fwd_lanes=8, fwd_tile=64, fwd_loop=0, list_wave_min_hits=2048)),
         # a 16-cell ring sends most bands through the 512-cell retry level
-        (oic.SCHED_BATCHED, dict(fwd_lanes=1, fwd_tile=16, list_wave_min_hits=2048)),
+        (oic.SCHED_BATCHED, dict(fwd_lanes=1, fwd_tile=16, fwd_loop=0, list_wave_min_hits=2048)),
         (oic.SCHED_PLAIN, dict(list_wave_min_hits=1 << 31)),
     ])
     assert ref["pairs"] > 2500 and ref["extensions"] > 100_000 and len(ref["records"]) > 0
     assert ctr[0]["list_wave_pairs"] > 1000
-    assert ctr[1]["overflow_retries"] > 1000
+    assert ctr[2]["overflow_retries"] > 1000
 
 
 def test_c5_full_size_refs_match_oracle():
     R = readsim.simulate(readsim.PRESETS["c5"])
     assert R.n == 166527
     ref, ctr = check(R, C5_REFS, 0.01, 1000, [
-        (oic.SCHED_BATCHED, dict(fwd_lanes=16, fwd_tile=128, list_wave_min_hits=256)),
-        (oic.SCHED_BATCHED, dict(fwd_lanes=1, fwd_tile=16, list_wave_min_hits=256)),
+        (oic.SCHED_BATCHED, dict(fwd_lanes=4, fwd_tile=64, fwd_loop=2, list_wave_min_hits=256)),
+        (oic.SCHED_BATCHED, dict(fwd_lanes=16, fwd_tile=128, fwd_loop=0, list_wave_min_hits=256)),
+        (oic.SCHED_BATCHED, dict(fwd_lanes=1, fwd_tile=16, fwd_loop=0, list_wave_min_hits=256)),
     ])
     assert len(ref["records"]) > 100
     # pairs with an offset of more than 128 seed hits ran at the 512-hit list level
-    assert ctr[0]["list_second_level"] > 0 and ctr[1]["list_second_level"] > 0
-    assert ctr[1]["overflow_retries"] > 0
+    assert ctr[0]["list_second_level"] > 0 and ctr[2]["list_second_level"] > 0
+    assert ctr[2]["overflow_retries"] > 0

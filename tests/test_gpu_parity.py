@@ -333,10 +333,11 @@ def test_autotune_keeps_results(c3s):
     eng.load_reads(c3s)
     eng.build_hash_index(1, c3s.n)
     best, trials = eng.autotune(1, min(c3s.n, 60))
-    # warm-up, the incumbent and 17 other forward shapes; the loop form: the incumbent, the other of
-    # row-stepped and per-task (shapes of more than one lane per DP) and flat rows with 4 and 8 lanes
-    # per DP; the task order; the incumbent list threshold and 4 others
-    assert len(trials) in (1 + 18 + 3 + 2 + 5, 1 + 18 + 4 + 2 + 5)
+    # warm-up; the incumbent (flat rows, 4 lanes x 64 cells) and the 18 other forward shapes; the
+    # loop form: the incumbent, the other of row-stepped and per-task (a winning shape of more than
+    # one lane per DP that is not flat) and flat rows with 4 and 8 lanes per DP unless incumbent; the
+    # task order; the incumbent list threshold and 4 others
+    assert len(trials) in (1 + 19 + 2 + 2 + 5, 1 + 19 + 3 + 2 + 5, 1 + 19 + 4 + 2 + 5)
     shapes = {(t["fwd_lanes"], t["fwd_tile"]) for t in trials if t["stage"] == "fwd"}
     assert len(shapes) == 19 and (best["fwd_lanes"], best["fwd_tile"]) in shapes
     flat = {(t["fwd_lanes"], t["fwd_tile"]) for t in trials if t["stage"] == "fwd" and t["fwd_loop"] == 2}

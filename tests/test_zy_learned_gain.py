@@ -43,7 +43,7 @@ def test_learned_order_beats_static_in_dp_time():
     print(f"\ncost model: {info}")
     out = {}
     for name, eng in (("static", static), ("learned", model)):
-        eng.set_tuning(fwd_lanes=8, fwd_tile=64, fwd_loop=0, fwd_graph=1)
+        eng.set_tuning(fwd_lanes=4, fwd_tile=64, fwd_loop=2, fwd_graph=1)
         eng.load_reads(R)
         out[name] = best_of(eng)
         print(f"{name}: DP kernels {out[name][0]:.2f} ms, cells {out[name][1].counters['dp_cells']}")
