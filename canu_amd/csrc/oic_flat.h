@@ -22,8 +22,7 @@
 
 // one DP's current row, for the lanes that run its cells
 struct FlatDesc {
-  int64_t w0a, w0b;     // first staged word of each tail in the packed reads
-  int64_t a_b0, b_b0;   // first base of each tail (N-mask positions)
+  int64_t w0a, w0b;     // first staged word of each tail in the packed reads (base 0 = 16 w0 + r0)
   int32_t Left, Right, eml, m, n, r0a, r0b;
   int32_t nl;           // lanes the row's cells are spread over
   int32_t nab;          // bit 0: A's read has N, bit 1: B's read has N
@@ -32,12 +31,15 @@ struct FlatDesc {
 template <int G>
 constexpr int flat_sw() { return G == 4 ? 32 : OIC_SEQ_WIN; }
 
+// Each DP's band buffer holds its CAP ring cells, then copies of cells 0 and 1 (so the three cells
+// d - 1, d, d + 1 are three consecutive words from (d - 1) mod CAP), then one spare word: strides of
+// CAP + 3 and SW + 1 words, both odd, put the same cell of different DPs in different LDS banks.
 template <int G, int CAP>
 struct FlatLDS {
   static constexpr int NG = OIC_WAVE / G;
   static constexpr int SW = flat_sw<G>();
-  int32_t tile[NG][CAP];
-  uint32_t seqA[NG][SW], seqT[NG][SW];
+  int32_t tile[NG][CAP + 3];
+  uint32_t seqA[NG][SW + 1], seqT[NG][SW + 1];
   FlatDesc desc[NG];
   unsigned long long pkey[NG];
   int32_t kmin[NG], kmax[NG], rmin[NG];
@@ -59,7 +61,7 @@ __device__ __forceinline__ int32_t wave_incl_scan(int32_t v) {
 
 // Wavefronts per SIMD the flat kernel's LDS allows, at most 4: registers are held to that
 // occupancy (128 VGPRs), which the cell loop's per-lane DP copy needs without spilling.
-// (4 x 64: 9.8 KB of LDS per wavefront, 16 per CU.)
+// (4 x 64: 10 KB of LDS per wavefront, 16 per CU.)
 template <int G, int CAP>
 constexpr int flat_waves_per_eu() {
   constexpr int w = (163840 / (int)sizeof(FlatLDS<G, CAP>) + 3) / 4;
@@ -91,12 +93,14 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
       over = true;
     } else {
       int32_t *pbuf = L.tile[g];
-      const int32_t sd = gl + ((gl < 2) ? s.Left - 1 : s.Right - 2);
-      if (gl < 4) pbuf[sd & MASK] = -2;
+      const int32_t sd = (gl + ((gl < 2) ? s.Left - 1 : s.Right - 2)) & MASK;
+      if (gl < 4) {
+        pbuf[sd] = -2;
+        if (sd < 2) pbuf[sd + CAP] = -2;
+      }
       if (gl == 0) {
         FlatDesc &D = L.desc[g];
         D.w0a = s.ta.w0; D.w0b = s.tb.w0;
-        D.a_b0 = s.a_b0; D.b_b0 = s.b_b0;
         D.Left = s.Left; D.Right = s.Right; D.eml = tb.match_limit[e];
         D.m = s.m; D.n = s.n; D.r0a = s.ta.r0; D.r0b = s.tb.r0;
         D.nab = (s.nA ? 1 : 0) | (s.nB ? 2 : 0);
@@ -154,8 +158,8 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
     c.tb.w0 = D.w0b;
     c.tb.r0 = D.r0b;
     c.tb.lds = (lds_words)L.seqT[dg];
-    c.a_b0 = D.a_b0;
-    c.b_b0 = D.b_b0;
+    c.a_b0 = D.w0a * 16 + D.r0a;
+    c.b_b0 = D.w0b * 16 + D.r0b;
     c.nA = (D.nab & 1) != 0;
     c.nB = (D.nab & 2) != 0;
     c.m = D.m;
@@ -175,9 +179,10 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
       // the previous row's cell d - 1 as the run's last lane read it in the chunk before
       const int32_t left = __shfl(p1, used ? last : lane);
       if (act) {
-        const int32_t x2 = (j == 0 && k > 0) ? left : buf[(d - 1) & MASK];
-        p1 = buf[d & MASK];
-        const int32_t x3 = 1 + buf[(d + 1) & MASK];
+        const int32_t *b3 = buf + ((d - 1) & MASK);
+        const int32_t x2 = (j == 0 && k > 0) ? left : b3[0];
+        p1 = b3[1];
+        const int32_t x3 = 1 + b3[2];
         pd = p1 > pv ? d : pd;
         pv = p1 > pv ? p1 : pv;
         int32_t Row = 1 + p1;
@@ -208,7 +213,9 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
         lb += (uint32_t)adv + 1u;
         const bool reach = adv == rem0;
         Row += adv;
-        buf[d & MASK] = Row;
+        const int32_t pos = d & MASK;
+        buf[pos] = Row;
+        if (pos < 2) buf[pos + CAP] = Row;
         rmin = (reach & (rmin == INF)) ? d : rmin;
         const bool keep = Row + (d > 0 ? d : 0) >= eml;   // prefixEditDistance-forward.C:245-285
         kmin = (keep & (kmin == INF)) ? d : kmin;
@@ -314,7 +321,10 @@ k_fwd_flat(FwdArgs A) {
     s.nB = s_first ? tN : sN;
     s.n = s_first ? T_Left_Begin + 1 : S_Left_Begin + 1;
     s.lim = limit - right_e;
-    if (!sm_start<G, SW>(s, res, A.R.words, A.R.nmask, L.tile[g], L.seqA[g], L.seqT[g], g, gl)) return true;
+    if (!sm_start<G, SW>(s, res, A.R.words, A.R.nmask, L.tile[g], L.seqA[g], L.seqT[g], g, gl)) {
+      if (gl == 0) L.tile[g][CAP] = L.tile[g][0];   // the copy of cell 0 (row 0)
+      return true;
+    }
     lmte = res.mte;                          // over at row 0
     return false;
   };
@@ -380,7 +390,11 @@ k_fwd_flat(FwdArgs A) {
       s.nB = s_short ? tN : sN;
       s.n = s_short ? T_Right_Len : S_Right_Len;
       s.lim = limit;
-      if (!sm_start<G, SW>(s, res, A.R.words, A.R.nmask, L.tile[g], L.seqA[g], L.seqT[g], g, gl)) { state = 1; break; }
+      if (!sm_start<G, SW>(s, res, A.R.words, A.R.nmask, L.tile[g], L.seqA[g], L.seqT[g], g, gl)) {
+        if (gl == 0) L.tile[g][CAP] = L.tile[g][0];   // the copy of cell 0 (row 0)
+        state = 1;
+        break;
+      }
       forward_done();
     }
     if (__ballot(state != 3) == 0) break;    // every group has run out of tasks
