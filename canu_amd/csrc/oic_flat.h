@@ -68,6 +68,18 @@ constexpr int flat_waves_per_eu() {
   return w < 1 ? 1 : (w > 4 ? 4 : w);
 }
 
+// the descriptor fields that stay fixed for a DP's rows, written when the DP starts
+template <int G, int CAP>
+__device__ __forceinline__ void flat_desc_start(FlatLDS<G, CAP> &L, const SMDP &s, int g, int gl) {
+  if (gl == 0) {
+    FlatDesc &D = L.desc[g];
+    D.w0a = s.ta.w0; D.w0b = s.tb.w0;
+    D.m = s.m; D.n = s.n; D.r0a = s.ta.r0; D.r0b = s.tb.r0;
+    D.nab = (s.nA ? 1 : 0) | (s.nB ? 2 : 0);
+    L.tile[g][CAP] = L.tile[g][0];   // the copy of cell 0 (row 0)
+  }
+}
+
 // One row of every active DP of the wavefront; called by all lanes.  rowg: this lane's group has
 // a DP in its row phase.  Returns true (group-uniform) when the group's DP is over (res holds its
 // outcome).  The group's DP state s is updated as sm_row updates its own.
@@ -100,10 +112,7 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
       }
       if (gl == 0) {
         FlatDesc &D = L.desc[g];
-        D.w0a = s.ta.w0; D.w0b = s.tb.w0;
         D.Left = s.Left; D.Right = s.Right; D.eml = tb.match_limit[e];
-        D.m = s.m; D.n = s.n; D.r0a = s.ta.r0; D.r0b = s.tb.r0;
-        D.nab = (s.nA ? 1 : 0) | (s.nB ? 2 : 0);
       }
     }
   }
@@ -204,11 +213,11 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
             lstat[2]++;
             if (lane == __builtin_ctzll(__ballot(1))) lstat[3]++;
           }
-          const int32_t k64 = sm_match64<SW>(c, words, nmask, Row + adv, Row + d + adv);
-          if (k64 >= rem) { adv += rem; break; }
-          adv += k64;
-          if (k64 < 64) break;
-          rem -= 64;
+          const int32_t k32 = sm_match32<SW>(c, words, nmask, Row + adv, Row + d + adv);
+          if (k32 >= rem) { adv += rem; break; }
+          adv += k32;
+          if (k32 < 32) break;
+          rem -= 32;
         }
         lb += (uint32_t)adv + 1u;
         const bool reach = adv == rem0;
@@ -322,7 +331,7 @@ k_fwd_flat(FwdArgs A) {
     s.n = s_first ? T_Left_Begin + 1 : S_Left_Begin + 1;
     s.lim = limit - right_e;
     if (!sm_start<G, SW>(s, res, A.R.words, A.R.nmask, L.tile[g], L.seqA[g], L.seqT[g], g, gl)) {
-      if (gl == 0) L.tile[g][CAP] = L.tile[g][0];   // the copy of cell 0 (row 0)
+      flat_desc_start<G, CAP>(L, s, g, gl);
       return true;
     }
     lmte = res.mte;                          // over at row 0
@@ -391,7 +400,7 @@ k_fwd_flat(FwdArgs A) {
       s.n = s_short ? T_Right_Len : S_Right_Len;
       s.lim = limit;
       if (!sm_start<G, SW>(s, res, A.R.words, A.R.nmask, L.tile[g], L.seqA[g], L.seqT[g], g, gl)) {
-        if (gl == 0) L.tile[g][CAP] = L.tile[g][0];   // the copy of cell 0 (row 0)
+        flat_desc_start<G, CAP>(L, s, g, gl);
         state = 1;
         break;
       }

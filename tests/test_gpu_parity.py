@@ -411,7 +411,14 @@ def test_skip_kmer_file(request, fixture, hopeless):
         eng.set_skip_kmers("ACGTACGT\n")
 
 
-def test_run_reads_with_n_and_short_reads():
+@pytest.mark.parametrize("schedule,tuning", [
+    (oic.SCHED_PLAIN, None),
+    (oic.SCHED_BATCHED, dict(fwd_lanes=4, fwd_tile=64, fwd_loop=2)),
+    (oic.SCHED_BATCHED, dict(fwd_lanes=8, fwd_tile=64, fwd_loop=0)),
+    (oic.SCHED_BATCHED, dict(fwd_lanes=1, fwd_tile=32, fwd_loop=0)),
+], ids=["plain", "flat", "row-stepped", "lane"])
+def test_run_reads_with_n_and_short_reads(schedule, tuning):
+    """Reads with N (the packed kernels' N-mask paths) and reads too short to hash or search."""
     rng = np.random.default_rng(9)
     R = readsim.simulate(readsim.PRESETS["c1"].scaled(60_000, seed=44))
     seqs = [R.read(i).decode() for i in range(1, R.n + 1)]
@@ -423,7 +430,9 @@ def test_run_reads_with_n_and_short_reads():
     seqs[1] = seqs[1][:300]     # shorter than --minlength: neither hashed nor searched
     seqs.append("ACGT" * 3)     # shorter than k
     R2 = readsim.from_strings(seqs)
-    assert_same(*run_both(R2))
+    res, ref = run_both(R2, schedule=schedule, tuning=tuning)
+    assert_same(res, ref)
+    assert res.counters["dp_cells"] == ref["cells_used"]
 
 
 def test_run_single_and_two_reads():
