@@ -22,9 +22,9 @@
 
 // one DP's current row, for the lanes that run its cells
 struct FlatDesc {
+  int4 row;             // this row: Left, Right, Edit_Match_Limit[e], lanes its cells are spread over
   int64_t w0a, w0b;     // first staged word of each tail in the packed reads (base 0 = 16 w0 + r0)
-  int32_t Left, Right, eml, m, n, r0a, r0b;
-  int32_t nl;           // lanes the row's cells are spread over
+  int32_t m, n, r0a, r0b;
   int32_t nab;          // bit 0: A's read has N, bit 1: B's read has N
 };
 
@@ -41,8 +41,8 @@ struct FlatLDS {
   int32_t tile[NG][CAP + 3];
   uint32_t seqA[NG][SW + 1], seqT[NG][SW + 1];
   FlatDesc desc[NG];
-  unsigned long long pkey[NG];
-  int32_t kmin[NG], kmax[NG], rmin[NG];
+  unsigned long long pkey[NG];   // the previous row's first maximum: (Row + 1) << 32 | (2^30 - d)
+  int4 red[NG];                  // first kept, last kept and first reaching diagonal, -
   int32_t start[OIC_WAVE];
 };
 
@@ -93,10 +93,11 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
   constexpr int32_t INF = 0x7fffffff;
   const int lane = lane_id();
   bool over = false;
-  int32_t e = 0;
+  int32_t e = 0, eml_row = 0;
   // group phase: the row's band, sentinels (prefixEditDistance-forward.C:140-153)
   if (rowg) {
     e = ++s.e;
+    eml_row = tb.match_limit[e];
     s.Left = (s.Left - 1 > -e) ? s.Left - 1 : -e;
     s.Right = (s.Right + 1 < e) ? s.Right + 1 : e;
     if (STATS) res.rows++;
@@ -110,16 +111,10 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
         pbuf[sd] = -2;
         if (sd < 2) pbuf[sd + CAP] = -2;
       }
-      if (gl == 0) {
-        FlatDesc &D = L.desc[g];
-        D.Left = s.Left; D.Right = s.Right; D.eml = tb.match_limit[e];
-      }
     }
   }
   if (lane < NG) {
-    L.kmin[lane] = INF;
-    L.kmax[lane] = -INF;
-    L.rmin[lane] = INF;
+    L.red[lane] = make_int4(INF, -INF, INF, 0);
     L.pkey[lane] = 0ull;
   }
   L.start[lane] = -1;
@@ -143,7 +138,7 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
       if (__builtin_amdgcn_readlane(incl, OIC_WAVE - 1) <= OIC_WAVE) break;
       C = it == 0 ? C + 1 : Csafe;
     }
-    if (gl == 0 && nv > 0) L.desc[g].nl = nv;
+    if (gl == 0 && nv > 0) L.desc[g].row = make_int4(s.Left, s.Right, eml_row, nv);
   }
   const int32_t S = __builtin_amdgcn_readlane(incl, OIC_WAVE - 1);
   wave_fence();
@@ -173,7 +168,8 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
     c.nB = (D.nab & 2) != 0;
     c.m = D.m;
     c.n = D.n;
-    const int32_t Left = D.Left, Right = D.Right, eml = D.eml, nl = D.nl;
+    const int4 rw = D.row;
+    const int32_t Left = rw.x, Right = rw.y, eml = rw.z, nl = rw.w;
     int32_t *buf = L.tile[dg];
     const int last = s0 + nl - 1;   // the run's last lane
     int32_t kmin = INF, kmax = -INF, rmin = INF, pv = -1, pd = 0, p1 = -2;
@@ -189,9 +185,10 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
       const int32_t left = __shfl(p1, used ? last : lane);
       if (act) {
         const int32_t *b3 = buf + ((d - 1) & MASK);
-        const int32_t x2 = (j == 0 && k > 0) ? left : b3[0];
-        p1 = b3[1];
-        const int32_t x3 = 1 + b3[2];
+        const int32_t l0 = b3[0], l1 = b3[1], l2 = b3[2];   // all three loaded: no branch on the lane
+        const int32_t x2 = (j == 0 && k > 0) ? left : l0;
+        p1 = l1;
+        const int32_t x3 = 1 + l2;
         pd = p1 > pv ? d : pd;
         pv = p1 > pv ? p1 : pv;
         int32_t Row = 1 + p1;
@@ -237,10 +234,10 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
     lstat[1] += lb;
     if (used) {
       if (kmin != INF) {
-        atomicMin(&L.kmin[dg], kmin);
-        atomicMax(&L.kmax[dg], kmax);
+        atomicMin(&L.red[dg].x, kmin);
+        atomicMax(&L.red[dg].y, kmax);
       }
-      if (rmin != INF) atomicMin(&L.rmin[dg], rmin);
+      if (rmin != INF) atomicMin(&L.red[dg].z, rmin);
       if (pv >= 0)
         atomicMax(&L.pkey[dg], ((unsigned long long)(uint32_t)(pv + 1) << 32) | (uint32_t)(0x40000000 - pd));
     }
@@ -249,7 +246,8 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
   // group phase: the row's outcome (prefixEditDistance-forward.C:172-306)
   if (rowg && !over) {
     const int32_t Left = s.Left, Right = s.Right;
-    const int32_t km = L.kmin[g], kx = L.kmax[g], rm = L.rmin[g];
+    const int4 kr = L.red[g];
+    const int32_t km = kr.x, kx = kr.y, rm = kr.z;
     const unsigned long long pk = L.pkey[g];
     const int32_t pvv = (int32_t)(uint32_t)(pk >> 32) - 1, pdd = 0x40000000 - (int32_t)(uint32_t)pk;
     const int32_t *cbuf = L.tile[g];
