@@ -1123,6 +1123,35 @@ static void Add_Match(const search_ctx *C, uint64_t toff, int32_t *start, int of
   WA->mn[*start].Next   = save;
 }
 
+int32_t oo_match_list(uint32_t kmer, const int32_t *offset, const int32_t *toff, int32_t n, int32_t *out,
+                      int32_t cap, int32_t *consistent) {
+  oo_params P;
+  memset(&P, 0, sizeof P);
+  P.kmer = kmer;
+  search_ctx C;
+  memset(&C, 0, sizeof C);
+  C.P = &P;
+  work_area WA;
+  memset(&WA, 0, sizeof WA);
+  WA.mn_size = n + 2;
+  WA.mn = (oo_match *)calloc((size_t)WA.mn_size, sizeof(oo_match));
+  WA.mn_next = 1;
+  int32_t head = 0;
+  int cons = 1;
+  for (int32_t i = 0; i < n; i++) Add_Match(&C, (uint64_t)(uint32_t)toff[i], &head, offset[i], &cons, &WA);
+  int32_t k = 0;
+  for (int32_t p = head; p != 0; p = WA.mn[p].Next) {
+    if (k >= cap) { free(WA.mn); return -1; }
+    out[3 * k] = WA.mn[p].Offset;
+    out[3 * k + 1] = WA.mn[p].Len;
+    out[3 * k + 2] = WA.mn[p].Start;
+    k++;
+  }
+  *consistent = cons;
+  free(WA.mn);
+  return k;
+}
+
 /* Find_Overlaps.C:100-163.  The Match_List lives inside the String_Olap table, which may be
  * reallocated below; Add_Match therefore works on a local copy of the list head. */
 static void Add_Ref(const search_ctx *C, uint32_t StrNum, uint32_t toff, int Offset, int32_t mult, work_area *WA) {

@@ -137,6 +137,13 @@ void oo_result_free(oo_result *r);
 /* ovOverlap helpers */
 uint32_t oo_encode_evalue(double q);    /* ovOverlap.H:31-35 */
 
+/* Test hook: Add_Match (overlapInCore-Find_Overlaps.C:26-96) over one pair's seed hits in arrival
+ * order (offset[i] in S, toff[i] in T), from an empty list.  Writes the final list front first as
+ * (Offset, Len, Start) triples and the consistent flag; returns the node count, or -1 when more
+ * than cap nodes. */
+int32_t oo_match_list(uint32_t kmer, const int32_t *offset, const int32_t *toff, int32_t n, int32_t *out,
+                      int32_t cap, int32_t *consistent);
+
 #ifdef __cplusplus
 }
 #endif

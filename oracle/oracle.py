@@ -112,8 +112,23 @@ def lib():
         L.oo_result_free.argtypes = [ctypes.POINTER(Result)]
         L.oo_encode_evalue.restype = ctypes.c_uint32
         L.oo_encode_evalue.argtypes = [ctypes.c_double]
+        L.oo_match_list.restype = ctypes.c_int32
+        L.oo_match_list.argtypes = [ctypes.c_uint32, _P32, _P32, ctypes.c_int32, _P32, ctypes.c_int32, _P32]
         _lib = L
     return _lib
+
+
+def match_list(kmer, offsets, toffs):
+    """Add_Match over one pair's seed hits in arrival order: the final match list, front first, as
+    (Offset, Len, Start) rows, and the consistent flag."""
+    o = np.ascontiguousarray(offsets, np.int32)
+    t = np.ascontiguousarray(toffs, np.int32)
+    out = np.zeros((max(1, len(o)), 3), np.int32)
+    cons = ctypes.c_int32(0)
+    k = lib().oo_match_list(kmer, o.ctypes.data_as(_P32), t.ctypes.data_as(_P32), len(o), out.ctypes.data_as(_P32),
+                            len(out), ctypes.byref(cons))
+    assert k >= 0
+    return out[:k].copy(), bool(cons.value)
 
 
 def cli_rate(x):
