@@ -1971,6 +1971,61 @@ extern "C" int oic_extend_batch(oic_ctx *c, const oic_ext_task *tasks, uint32_t 
   return OIC_OK;
 }
 
+// Add_Overlap and the pair's selection over explicit extension results, one pair per lane
+__global__ void k_distinct_olaps(SearchConst C, const uint32_t *__restrict__ first, uint32_t n,
+                                 const oic_olap_in *__restrict__ in, oic_olap_out *__restrict__ out,
+                                 int32_t *__restrict__ count) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  OlapDev olap[OIC_MAX_DISTINCT];
+  int32_t ct = 0;
+  for (uint32_t k = first[g]; k < first[g + 1]; k++) {
+    const oic_olap_in x = in[k];
+    add_overlap(C, x.s_lo, x.s_hi, x.t_lo, x.t_hi, x.quality, 0, olap, &ct);
+  }
+  int32_t deleted[OIC_MAX_DISTINCT] = {0, 0, 0};
+  select_distinct(C, olap, ct, deleted);
+  for (int32_t i = 0; i < ct; i++) {
+    const OlapDev &o = olap[i];
+    oic_olap_out r;
+    r.s_lo = o.s_lo; r.s_hi = o.s_hi; r.t_lo = o.t_lo; r.t_hi = o.t_hi;
+    r.min_diag = o.min_diag; r.max_diag = o.max_diag;
+    r.s_left = o.slb; r.s_right = o.srb; r.t_left = o.tlb; r.t_right = o.trb;
+    r.deleted = deleted[i];
+    r.pad = 0;
+    r.quality = o.quality;
+    out[(size_t)g * OIC_MAX_DISTINCT + i] = r;
+  }
+  count[g] = ct;
+}
+
+extern "C" int oic_distinct_olaps_batch(oic_ctx *c, const uint32_t *first, uint32_t n, const oic_olap_in *in,
+                                        oic_olap_out *out, int32_t *count) {
+  if (!c || (n && (!first || !out || !count))) return OIC_EINVAL;
+  if (n == 0) return OIC_OK;
+  if (first[0] != 0) return fail(c, OIC_EINVAL, "first[0] is %u, not 0", first[0]);
+  for (uint32_t g = 0; g < n; g++)
+    if (first[g + 1] < first[g]) return fail(c, OIC_EINVAL, "first[] decreases at pair %u", g);
+  const uint32_t total = first[n];
+  if (total && !in) return OIC_EINVAL;
+  DevBuf df, di, dout, dc;
+  ALLOC(c, df, (size_t)(n + 1) * 4);
+  ALLOC(c, di, (size_t)std::max<uint32_t>(total, 1) * sizeof(oic_olap_in));
+  ALLOC(c, dout, (size_t)n * OIC_MAX_DISTINCT * sizeof(oic_olap_out));
+  ALLOC(c, dc, (size_t)n * 4);
+  HIPCHK(c, hipMemcpyAsync(df.p, first, (size_t)(n + 1) * 4, hipMemcpyHostToDevice, c->st));
+  if (total) HIPCHK(c, hipMemcpyAsync(di.p, in, (size_t)total * sizeof(oic_olap_in), hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipMemsetAsync(dout.p, 0, (size_t)n * OIC_MAX_DISTINCT * sizeof(oic_olap_out), c->st));
+  const SearchConst C = search_const(c, false);
+  k_distinct_olaps<<<(n + 255) / 256, 256, 0, c->st>>>(C, df.as<uint32_t>(), n, di.as<oic_olap_in>(),
+                                                       dout.as<oic_olap_out>(), dc.as<int32_t>());
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(out, dout.p, (size_t)n * OIC_MAX_DISTINCT * sizeof(oic_olap_out), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipMemcpyAsync(count, dc.p, (size_t)n * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return OIC_OK;
+}
+
 // ------------------------------------------------------------------------------------ cost model
 
 extern "C" int oic_set_tracing(oic_ctx *c, int on) {

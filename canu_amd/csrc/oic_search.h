@@ -1141,6 +1141,62 @@ __device__ void add_overlap(const SearchConst &C, int32_t s_lo, int32_t s_hi, in
   (*ct)++;
 }
 
+// The per-pair selection of Process_String_Overlaps.C:501-513 over the distinct overlaps Add_Overlap
+// left: Choose_Best_Partial under partial and unique, Combine_Into_One_Olap under unique,
+// Merge_Intersecting_Olaps otherwise.  Updates the kept entries' bounds; deleted[] starts at zero.
+__device__ void select_distinct(const SearchConst &C, OlapDev *olap, const int32_t ct, int32_t *deleted) {
+  if (ct > 0) {
+    if (C.partial) {
+      if (C.unique) {   // Choose_Best_Partial (:289-311)
+        int32_t best = 0;
+        double mb0 = (1.0 - olap[0].quality) * (2 + olap[0].s_hi - olap[0].s_lo + olap[0].t_hi - olap[0].t_lo);
+        for (int32_t i = 1; i < ct; i++) {
+          const double mb = (1.0 - olap[i].quality) * (2 + olap[i].s_hi - olap[i].s_lo + olap[i].t_hi - olap[i].t_lo);
+          if (mb0 < mb || (mb0 == mb && olap[i].quality < olap[best].quality)) best = i;
+        }
+        for (int32_t i = 0; i < ct; i++) deleted[i] = (i != best);
+      }
+    } else if (C.unique) {   // Combine_Into_One_Olap (:42-96)
+      int32_t best = 0;
+      int32_t mnd = olap[0].min_diag, mxd = olap[0].max_diag, slb = olap[0].slb, srb = olap[0].srb, tlb = olap[0].tlb, trb = olap[0].trb;
+      for (int32_t i = 1; i < ct; i++) {
+        const int32_t si = olap[i].s_hi - olap[i].s_lo, ti = olap[i].t_hi - olap[i].t_lo;
+        const int32_t sb = olap[best].s_hi - olap[best].s_lo, tb2 = olap[best].t_hi - olap[best].t_lo;
+        const int32_t leni = 1 + (si < ti ? si : ti), lenb = 1 + (sb < tb2 ? sb : tb2);
+        if (olap[i].quality < olap[best].quality || (olap[i].quality == olap[best].quality && leni > lenb)) best = i;
+        if (olap[i].min_diag < mnd) mnd = olap[i].min_diag;
+        if (olap[i].max_diag > mxd) mxd = olap[i].max_diag;
+        if (olap[i].slb < slb) slb = olap[i].slb;
+        if (olap[i].srb > srb) srb = olap[i].srb;
+        if (olap[i].tlb < tlb) tlb = olap[i].tlb;
+        if (olap[i].trb > trb) trb = olap[i].trb;
+      }
+      olap[best].min_diag = mnd; olap[best].max_diag = mxd; olap[best].slb = slb; olap[best].srb = srb;
+      olap[best].tlb = tlb; olap[best].trb = trb;
+      for (int32_t i = 0; i < ct; i++) deleted[i] = (i != best);
+    } else {   // Merge_Intersecting_Olaps (:106-162)
+      for (int32_t i = 0; i < ct - 1; i++)
+        for (int32_t j = i + 1; j < ct; j++) {
+          if (deleted[i] || deleted[j]) continue;
+          const int32_t lo = olap[i].min_diag, hi = olap[i].max_diag;
+          if ((lo <= 0 && olap[j].min_diag > 0) || (lo > 0 && olap[j].min_diag <= 0)) continue;
+          if ((lo >= 0 && olap[j].trb - lo - olap[j].slb >= OIC_MIN_INTER) || (lo <= 0 && olap[j].srb + lo - olap[j].tlb >= OIC_MIN_INTER) ||
+              (hi >= 0 && olap[j].trb - hi - olap[j].slb >= OIC_MIN_INTER) || (hi <= 0 && olap[j].srb + hi - olap[j].tlb >= OIC_MIN_INTER)) {
+            OlapDev *keep, *disc;
+            if (olap[i].quality < olap[j].quality) { keep = &olap[i]; disc = &olap[j]; deleted[j] = 1; }
+            else                                   { keep = &olap[j]; disc = &olap[i]; deleted[i] = 1; }
+            if (disc->min_diag < keep->min_diag) keep->min_diag = disc->min_diag;
+            if (disc->max_diag > keep->max_diag) keep->max_diag = disc->max_diag;
+            if (disc->slb < keep->slb) keep->slb = disc->slb;
+            if (disc->srb > keep->srb) keep->srb = disc->srb;
+            if (disc->tlb < keep->tlb) keep->tlb = disc->tlb;
+            if (disc->trb > keep->trb) keep->trb = disc->trb;
+          }
+        }
+    }
+  }
+}
+
 __device__ __forceinline__ uint32_t encode_evalue(double q) {
   return (q < (65535 / 100000.0)) ? (uint32_t)(100000.0 * q + 0.5) : 65535u;
 }
@@ -1419,56 +1475,7 @@ __global__ void __launch_bounds__(OIC_WAVE) k_process_pairs(PairKernelArgs A) {
     if (lane == 0) {
       const int32_t ct = olap_ct;
       int32_t deleted[OIC_MAX_DISTINCT] = {0, 0, 0};
-      if (ct > 0) {
-        if (C.partial) {
-          if (C.unique) {   // Choose_Best_Partial (:289-311)
-            int32_t best = 0;
-            double mb0 = (1.0 - olap[0].quality) * (2 + olap[0].s_hi - olap[0].s_lo + olap[0].t_hi - olap[0].t_lo);
-            for (int32_t i = 1; i < ct; i++) {
-              const double mb = (1.0 - olap[i].quality) * (2 + olap[i].s_hi - olap[i].s_lo + olap[i].t_hi - olap[i].t_lo);
-              if (mb0 < mb || (mb0 == mb && olap[i].quality < olap[best].quality)) best = i;
-            }
-            for (int32_t i = 0; i < ct; i++) deleted[i] = (i != best);
-          }
-        } else if (C.unique) {   // Combine_Into_One_Olap (:42-96)
-          int32_t best = 0;
-          int32_t mnd = olap[0].min_diag, mxd = olap[0].max_diag, slb = olap[0].slb, srb = olap[0].srb, tlb = olap[0].tlb, trb = olap[0].trb;
-          for (int32_t i = 1; i < ct; i++) {
-            const int32_t si = olap[i].s_hi - olap[i].s_lo, ti = olap[i].t_hi - olap[i].t_lo;
-            const int32_t sb = olap[best].s_hi - olap[best].s_lo, tb2 = olap[best].t_hi - olap[best].t_lo;
-            const int32_t leni = 1 + (si < ti ? si : ti), lenb = 1 + (sb < tb2 ? sb : tb2);
-            if (olap[i].quality < olap[best].quality || (olap[i].quality == olap[best].quality && leni > lenb)) best = i;
-            if (olap[i].min_diag < mnd) mnd = olap[i].min_diag;
-            if (olap[i].max_diag > mxd) mxd = olap[i].max_diag;
-            if (olap[i].slb < slb) slb = olap[i].slb;
-            if (olap[i].srb > srb) srb = olap[i].srb;
-            if (olap[i].tlb < tlb) tlb = olap[i].tlb;
-            if (olap[i].trb > trb) trb = olap[i].trb;
-          }
-          olap[best].min_diag = mnd; olap[best].max_diag = mxd; olap[best].slb = slb; olap[best].srb = srb;
-          olap[best].tlb = tlb; olap[best].trb = trb;
-          for (int32_t i = 0; i < ct; i++) deleted[i] = (i != best);
-        } else {   // Merge_Intersecting_Olaps (:106-162)
-          for (int32_t i = 0; i < ct - 1; i++)
-            for (int32_t j = i + 1; j < ct; j++) {
-              if (deleted[i] || deleted[j]) continue;
-              const int32_t lo = olap[i].min_diag, hi = olap[i].max_diag;
-              if ((lo <= 0 && olap[j].min_diag > 0) || (lo > 0 && olap[j].min_diag <= 0)) continue;
-              if ((lo >= 0 && olap[j].trb - lo - olap[j].slb >= OIC_MIN_INTER) || (lo <= 0 && olap[j].srb + lo - olap[j].tlb >= OIC_MIN_INTER) ||
-                  (hi >= 0 && olap[j].trb - hi - olap[j].slb >= OIC_MIN_INTER) || (hi <= 0 && olap[j].srb + hi - olap[j].tlb >= OIC_MIN_INTER)) {
-                OlapDev *keep, *disc;
-                if (olap[i].quality < olap[j].quality) { keep = &olap[i]; disc = &olap[j]; deleted[j] = 1; }
-                else                                   { keep = &olap[j]; disc = &olap[i]; deleted[i] = 1; }
-                if (disc->min_diag < keep->min_diag) keep->min_diag = disc->min_diag;
-                if (disc->max_diag > keep->max_diag) keep->max_diag = disc->max_diag;
-                if (disc->slb < keep->slb) keep->slb = disc->slb;
-                if (disc->srb > keep->srb) keep->srb = disc->srb;
-                if (disc->tlb < keep->tlb) keep->tlb = disc->tlb;
-                if (disc->trb > keep->trb) keep->trb = disc->trb;
-              }
-            }
-        }
-      }
+      select_distinct(C, olap, ct, deleted);
       int32_t nout = 0;
       uint32_t cont = 0, dove = 0;
       for (int32_t i = 0; i < ct; i++) {

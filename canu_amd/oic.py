@@ -29,6 +29,7 @@ EXPORTS = [
     "oic_prefix_dp_batch_noalloc", "oic_tables",
     "oic_dp_tables", "oic_set_tracing", "oic_get_traces", "oic_model_train", "oic_model_get", "oic_model_set",
     "oic_model_predict", "oic_model_grad", "oic_get_tuning", "oic_set_tuning", "oic_autotune", "oic_set_skip_kmers",
+    "oic_distinct_olaps_batch",
 ]
 
 NFEAT, NOUT = 12, 2
@@ -91,6 +92,11 @@ class Counters(ctypes.Structure):
 EXT_TASK = np.dtype([("s_id", "<u4"), ("t_id", "<u4"), ("s_dir", "<i4"), ("start", "<i4"), ("offset", "<i4"), ("len", "<i4")])
 EXT_RESULT = np.dtype([("kind", "<i4"), ("s_lo", "<i4"), ("s_hi", "<i4"), ("t_lo", "<i4"), ("t_hi", "<i4"),
                        ("errors", "<i4"), ("delta_len", "<i4"), ("status", "<i4")])
+OLAP_IN = np.dtype([("s_lo", "<i4"), ("s_hi", "<i4"), ("t_lo", "<i4"), ("t_hi", "<i4"), ("quality", "<f8")])
+OLAP_OUT = np.dtype([("s_lo", "<i4"), ("s_hi", "<i4"), ("t_lo", "<i4"), ("t_hi", "<i4"), ("min_diag", "<i4"),
+                     ("max_diag", "<i4"), ("s_left", "<i4"), ("s_right", "<i4"), ("t_left", "<i4"), ("t_right", "<i4"),
+                     ("deleted", "<i4"), ("pad", "<i4"), ("quality", "<f8")])
+MAX_DISTINCT_OLAPS = 3
 DP_TASK = np.dtype([("a_id", "<u4"), ("b_id", "<u4"), ("a_strand", "<i4"), ("b_strand", "<i4"), ("a_pos", "<i4"),
                     ("b_pos", "<i4"), ("m", "<i4"), ("n", "<i4"), ("error_limit", "<i4"), ("dir", "<i4")])
 DP_RESULT = np.dtype([("errors", "<i4"), ("a_end", "<i4"), ("t_end", "<i4"), ("match_to_end", "<i4"),
@@ -120,6 +126,7 @@ def lib():
         L.oic_get_counters.argtypes = [vp, ctypes.POINTER(Counters)]
         L.oic_extend_batch.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32]
         L.oic_prefix_dp_batch.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32]
+        L.oic_distinct_olaps_batch.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, vp]
         L.oic_prefix_dp_batch_noalloc.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_uint32]
         L.oic_tables.argtypes = [vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)),
                                  ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)), ctypes.POINTER(ctypes.c_uint32)]
@@ -244,6 +251,20 @@ class Engine:
         delta = np.zeros((len(tasks), delta_stride), np.int32)
         self._check(lib().oic_extend_batch(self._ctx, tasks.ctypes.data, len(tasks), res.ctypes.data, delta.ctypes.data, delta_stride))
         return res, delta
+
+    def distinct_olaps(self, groups):
+        """Add_Overlap over each pair's extension results, then the pair's selection under this
+        engine's partial/unique parameters.  groups: one sequence of OLAP_IN rows per pair.  Returns
+        one OLAP_OUT array per pair (its distinct overlaps, deleted ones flagged)."""
+        first = np.zeros(len(groups) + 1, np.uint32)
+        first[1:] = np.cumsum([len(g) for g in groups])
+        rows = np.concatenate([np.asarray(g, OLAP_IN) for g in groups]) if first[-1] else np.zeros(1, OLAP_IN)
+        rows = np.ascontiguousarray(rows, OLAP_IN)
+        out = np.zeros((max(1, len(groups)), MAX_DISTINCT_OLAPS), OLAP_OUT)
+        count = np.zeros(max(1, len(groups)), np.int32)
+        self._check(lib().oic_distinct_olaps_batch(self._ctx, first.ctypes.data, len(groups), rows.ctypes.data,
+                                                   out.ctypes.data, count.ctypes.data))
+        return [out[g, :count[g]].copy() for g in range(len(groups))]
 
     def prefix_dp(self, tasks, delta_stride=4096):
         tasks = np.ascontiguousarray(tasks, DP_TASK)

@@ -249,6 +249,32 @@ typedef struct oic_ext_result {
 int oic_extend_batch(oic_ctx *ctx, const oic_ext_task *tasks, uint32_t n, oic_ext_result *results,
                      int32_t *delta, uint32_t delta_stride);
 
+/* One extension result offered to Add_Overlap: the aligned boxes on S and T and the quality
+ * (errors / length) Process_Matches computes for it. */
+typedef struct oic_olap_in {
+  int32_t s_lo, s_hi, t_lo, t_hi;
+  double  quality;
+} oic_olap_in;
+
+/* One distinct overlap of a pair (Olap_Info_t, overlapInCore.H:207-216, without the delta). */
+typedef struct oic_olap_out {
+  int32_t s_lo, s_hi, t_lo, t_hi;
+  int32_t min_diag, max_diag;
+  int32_t s_left, s_right, t_left, t_right;   /* s/t_left_boundary, s/t_right_boundary */
+  int32_t deleted;                            /* removed by the selection: not output */
+  int32_t pad;
+  double  quality;
+} oic_olap_out;
+
+/* Add_Overlap over each pair's extension results in order, then the pair's selection
+ * (Process_String_Overlaps.C:175-244, 42-162, 289-311, chosen as at :501-513) under the context's
+ * partial and unique parameters: the step of Process_Matches that turns extension results into
+ * output overlaps.  Pair g owns in[first[g] .. first[g+1]) (first has n_pairs + 1 entries, first[0]
+ * is 0, non-decreasing); its distinct overlaps, at most 3, go to out[3g ..] and their number to
+ * count[g]. */
+int oic_distinct_olaps_batch(oic_ctx *ctx, const uint32_t *first, uint32_t n_pairs, const oic_olap_in *in,
+                             oic_olap_out *out, int32_t *count);
+
 /* One prefixEditDistance::forward (dir 0) or ::reverse (dir 1) call on explicit windows.
  * Forward compares a[a_pos ..] with b[b_pos ..] rightwards; reverse compares a[a_pos], a[a_pos-1], ...
  * with b[b_pos], b[b_pos-1], ... as the reference's negative indexing does.  Windows are

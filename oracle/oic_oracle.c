@@ -1338,6 +1338,56 @@ static void Choose_Best_Partial(olap_info *olap, int ct, int deleted[]) {
     deleted[i] = (i != best);
 }
 
+/* Test hook: Add_Overlap over a sequence of extension results (boxes of s_lo, s_hi, t_lo, t_hi and
+ * their qualities), then the selection of Process_String_Overlaps.C:501-513: Choose_Best_Partial
+ * under partial and unique, Combine_Into_One_Olap under unique, Merge_Intersecting_Olaps otherwise.
+ * out receives one row per distinct overlap: s_lo, s_hi, t_lo, t_hi, min_diag, max_diag,
+ * s_left_boundary, s_right_boundary, t_left_boundary, t_right_boundary, deleted; outq its quality.
+ * Returns the distinct overlap count (at most MAX_DISTINCT_OLAPS). */
+int32_t oo_distinct_olaps(int partial, int unique, int32_t n, const int32_t *box, const double *qual,
+                          int32_t *out, double *outq) {
+  oo_params P;
+  memset(&P, 0, sizeof P);
+  P.partial = partial;
+  search_ctx C;
+  memset(&C, 0, sizeof C);
+  C.P = &P;
+  int32_t nodelta[1] = {0};
+  struct oo_dp ed;
+  memset(&ed, 0, sizeof ed);
+  ed.Left_Delta = nodelta;
+  work_area WA;
+  memset(&WA, 0, sizeof WA);
+  WA.ed = &ed;
+  olap_info olap[MAX_DISTINCT_OLAPS];
+  memset(olap, 0, sizeof olap);
+  int32_t deltas[MAX_DISTINCT_OLAPS][1];
+  for (int i = 0; i < MAX_DISTINCT_OLAPS; i++) olap[i].delta = deltas[i];
+  int ct = 0;
+  for (int32_t i = 0; i < n; i++)
+    Add_Overlap(&C, box[4 * i], box[4 * i + 1], box[4 * i + 2], box[4 * i + 3], qual[i], olap, &ct, &WA);
+  int deleted[MAX_DISTINCT_OLAPS] = {0};
+  if (ct > 0) {
+    if (partial) {
+      if (unique) Choose_Best_Partial(olap, ct, deleted);
+    } else {
+      if (unique) Combine_Into_One_Olap(olap, ct, deleted);
+      else        Merge_Intersecting_Olaps(olap, ct, deleted);
+    }
+  }
+  for (int i = 0; i < ct; i++) {
+    const olap_info *o = &olap[i];
+    int32_t *r = out + 11 * i;
+    r[0] = o->s_lo; r[1] = o->s_hi; r[2] = o->t_lo; r[3] = o->t_hi;
+    r[4] = o->min_diag; r[5] = o->max_diag;
+    r[6] = o->s_left_boundary; r[7] = o->s_right_boundary;
+    r[8] = o->t_left_boundary; r[9] = o->t_right_boundary;
+    r[10] = deleted[i];
+    outq[i] = o->quality;
+  }
+  return ct;
+}
+
 /* ovOverlap.H:31-35 */
 uint32_t oo_encode_evalue(double q) {
   return (q < (65535 / 100000.0)) ? (uint32_t)(100000.0 * q + 0.5) : 65535u;

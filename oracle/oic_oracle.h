@@ -143,6 +143,11 @@ uint32_t oo_encode_evalue(double q);    /* ovOverlap.H:31-35 */
  * than cap nodes. */
 int32_t oo_match_list(uint32_t kmer, const int32_t *offset, const int32_t *toff, int32_t n, int32_t *out,
                       int32_t cap, int32_t *consistent);
+/* Add_Overlap over a sequence of extension results, then Choose_Best_Partial /
+ * Combine_Into_One_Olap / Merge_Intersecting_Olaps (Process_String_Overlaps.C:175-311, 501-513);
+ * rows of 11 ints per distinct overlap (box, min/max diag, boundaries, deleted). */
+int32_t oo_distinct_olaps(int partial, int unique, int32_t n, const int32_t *box, const double *qual,
+                          int32_t *out, double *outq);
 
 #ifdef __cplusplus
 }

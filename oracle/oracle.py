@@ -114,6 +114,9 @@ def lib():
         L.oo_encode_evalue.argtypes = [ctypes.c_double]
         L.oo_match_list.restype = ctypes.c_int32
         L.oo_match_list.argtypes = [ctypes.c_uint32, _P32, _P32, ctypes.c_int32, _P32, ctypes.c_int32, _P32]
+        L.oo_distinct_olaps.restype = ctypes.c_int32
+        L.oo_distinct_olaps.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int32, _P32,
+                                        ctypes.POINTER(ctypes.c_double), _P32, ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -129,6 +132,24 @@ def match_list(kmer, offsets, toffs):
                             len(out), ctypes.byref(cons))
     assert k >= 0
     return out[:k].copy(), bool(cons.value)
+
+
+MAX_DISTINCT_OLAPS = 3
+
+
+def distinct_olaps(boxes, quals, partial=False, unique=False):
+    """Add_Overlap over extension results (rows s_lo, s_hi, t_lo, t_hi with their qualities), then the
+    per-pair selection: rows (s_lo, s_hi, t_lo, t_hi, min_diag, max_diag, s_left, s_right, t_left,
+    t_right, deleted) and the qualities of the distinct overlaps."""
+    bx = np.ascontiguousarray(boxes, np.int32).reshape(-1, 4)
+    q = np.ascontiguousarray(quals, np.float64)
+    assert len(q) == len(bx)
+    out = np.zeros((MAX_DISTINCT_OLAPS, 11), np.int32)
+    oq = np.zeros(MAX_DISTINCT_OLAPS, np.float64)
+    pd = ctypes.POINTER(ctypes.c_double)
+    k = lib().oo_distinct_olaps(int(partial), int(unique), len(bx), bx.ctypes.data_as(_P32), q.ctypes.data_as(pd),
+                                out.ctypes.data_as(_P32), oq.ctypes.data_as(pd))
+    return out[:k].copy(), oq[:k].copy()
 
 
 def cli_rate(x):
