@@ -119,6 +119,7 @@ def run_steps(eng, n, batch, steps, warmup, first):
     for i in range(warmup + steps):
         bgn = first + i * batch
         end = min(n, bgn + batch - 1)
+        cs = eng.counters()
         t0 = time.perf_counter()
         res = eng.search(bgn, end)
         dt = time.perf_counter() - t0
@@ -127,7 +128,10 @@ def run_steps(eng, n, batch, steps, warmup, first):
             recs += len(res.records)
         if i == warmup - 1:
             c0 = eng.counters()
-        log(f"  step {i - warmup:+d}: ref {bgn}-{end} {dt*1e3:.1f} ms, {len(res.records)} overlaps")
+        ce = eng.counters()
+        stages = " ".join(f"{k[3:]} {ce[k] - cs[k]:.0f}" for k in ("ms_gather", "ms_lists", "ms_extend", "ms_alloc"))
+        log(f"  step {i - warmup:+d}: ref {bgn}-{end} {dt*1e3:.1f} ms ({stages} ms; {ce['allocs'] - cs['allocs']} allocations), "
+            f"{len(res.records)} overlaps")
     c1 = eng.counters()
     d = {k: c1[k] - c0[k] for k in c1}
     return per, recs, d

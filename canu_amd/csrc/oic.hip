@@ -107,6 +107,7 @@ struct oic_ctx {
   DevBuf d_skip, x_tscreen, g_sscreen;
   oic_tuning tune{};
   uint64_t hit_cap = 0;                 // seed hits per search pass
+  uint64_t h_alloc = 0;                 // seed hits the per-hit pass buffers are sized for
   int wallclock_khz = 0;                // rate of wall_clock64()
   DevBuf p_pairs, p_tmp, p_nodes, p_recs, p_pout, p_order, p_queue, p_stats, p_nout, p_noff, p_outrec, p_retry;
   DevBuf sc_rowL, sc_rowR, sc_rowOff, sc_rowVal, sc_dbuf, sc_ld, sc_rd;
@@ -147,10 +148,16 @@ static int lookup_scratch(oic_ctx *c, int dev);
     hipError_t _e = (x);                                                                  \
     if (_e != hipSuccess) return fail((c), OIC_EHIP, "%s: %s (%s:%d)", #x, hipGetErrorString(_e), __FILE__, __LINE__); \
   } while (0)
-#define ALLOC(c, buf, bytes)                                                              \
+#define ALLOC(c, buf, nbytes)                                                             \
   do {                                                                                    \
-    if ((buf).ensure(bytes) != hipSuccess)                                                \
-      return fail((c), OIC_ENOMEM, "device allocation of %zu bytes failed (%s:%d)", (size_t)(bytes), __FILE__, __LINE__); \
+    const size_t _b = (nbytes);                                                           \
+    if (_b > (buf).bytes || !(buf).p) {                                                   \
+      const double _t = now_ms();                                                         \
+      if ((buf).ensure(_b) != hipSuccess)                                                 \
+        return fail((c), OIC_ENOMEM, "device allocation of %zu bytes failed (%s:%d)", _b, __FILE__, __LINE__); \
+      (c)->ctr.allocs++;                                                                  \
+      (c)->ctr.ms_alloc += now_ms() - _t;                                                 \
+    }                                                                                     \
   } while (0)
 
 static DPTables dev_tables(oic_ctx *c) {
@@ -1194,9 +1201,14 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   c->ctr.kmer_probes += P;
   c->ctr.hits += H;
   if (H == 0) { c->ctr.ms_gather += now_ms() - t0; return OIC_OK; }
-  ALLOC(c, c->s_k1, H * 8);
-  ALLOC(c, c->s_v1, H * 8);
-  if (ensure_sort_temp(c, H) != OIC_OK) return OIC_ENOMEM;
+  // The per-hit buffers grow to at least twice the largest pass so far, capped at the pass limit:
+  // a pass slightly larger than every earlier one then reuses them instead of freeing and
+  // reallocating some 100 bytes per hit (tens of GB at the limit).
+  if (H > c->h_alloc) c->h_alloc = std::max<uint64_t>(H, std::min<uint64_t>(2 * c->h_alloc, c->hit_cap));
+  const uint64_t HA = c->h_alloc;
+  ALLOC(c, c->s_k1, HA * 8);
+  ALLOC(c, c->s_v1, HA * 8);
+  if (ensure_sort_temp(c, HA) != OIC_OK) return OIC_ENOMEM;
   uint64_t *hk = c->s_k1.as<uint64_t>(), *hv = c->s_v1.as<uint64_t>();
   hipLaunchKernelGGL(k_hit_emit_tiles, dim3((unsigned)((H + EMIT_TILE - 1) / EMIT_TILE)), dim3(256), 0, c->st, c->g_off.as<uint64_t>(),
                      c->g_phi.as<uint64_t>(), c->g_pval.as<uint64_t>(), P, H, GA.X.occ, GA.tbits, hk, hv);
@@ -1208,8 +1220,8 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
     const int rc = sort_pairs(c, &hk, &hv, H, GA.tbits);
     if (rc) return rc;
   }
-  ALLOC(c, c->s_flag, H * 4);
-  ALLOC(c, c->s_idx, H * 8);
+  ALLOC(c, c->s_flag, HA * 4);
+  ALLOC(c, c->s_idx, HA * 8);
   uint64_t np = 0;
   {
     const unsigned g = (unsigned)((H + 255) / 256);
@@ -1223,10 +1235,10 @@ static int search_batch(oic_ctx *c, uint32_t s0, uint32_t ns, const std::vector<
   c->ctr.pairs += np;
   c->ctr.ms_gather += now_ms() - t0;
   const double t1 = now_ms();
-  ALLOC(c, c->p_tmp, H * 16);
-  ALLOC(c, c->p_nodes, H * 16);
+  ALLOC(c, c->p_tmp, HA * 16);
+  ALLOC(c, c->p_nodes, HA * 16);
   uint64_t batch_nodes = 0;
-  ALLOC(c, c->l_scr, H * (8 * 4 + 1));
+  ALLOC(c, c->l_scr, HA * (8 * 4 + 1));
   {
     ListScratch LS;
     int32_t *b = c->l_scr.as<int32_t>();

@@ -86,7 +86,8 @@ class Counters(ctypes.Structure):
                                                "list_handed_back", "list_second_level")] + \
                [(n, ctypes.c_double) for n in ("ms_index", "ms_gather", "ms_lists", "ms_extend", "ms_total", "ms_dp_kernel",
                                                "ms_fwd_kernel", "ms_fwd_narrow", "ms_fwd_graph", "ms_pp_kernel")] + \
-               [(n, ctypes.c_uint64) for n in ("pp_cells", "pp_bases", "learned_pairs", "pp_launches", "spec_wave_chunks")]
+               [(n, ctypes.c_uint64) for n in ("pp_cells", "pp_bases", "learned_pairs", "pp_launches", "spec_wave_chunks", "allocs")] + \
+               [("ms_alloc", ctypes.c_double)]
 
 
 EXT_TASK = np.dtype([("s_id", "<u4"), ("t_id", "<u4"), ("s_dir", "<i4"), ("start", "<i4"), ("offset", "<i4"), ("len", "<i4")])

@@ -146,6 +146,8 @@ typedef struct oic_counters {
   uint64_t learned_pairs;      /* pairs the learned cost model ordered (OIC_SCHED_LEARNED with a model) */
   uint64_t pp_launches;        /* launches of the Process_Matches kernel */
   uint64_t spec_wave_chunks;   /* band chunk iterations wavefronts of the row-stepped kernel executed (with tuning.fwd_counters) */
+  uint64_t allocs;             /* device work-buffer allocations the context made (first use or growth) */
+  double   ms_alloc;           /* host time those allocations took, with the free of the smaller buffer */
 } oic_counters;
 
 /* Launch-shape knobs.  No setting changes a result; the defaults suit MI355X.  No reference
