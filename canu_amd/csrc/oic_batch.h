@@ -87,7 +87,7 @@ template <int SW = OIC_SEQ_WIN>
 __device__ __forceinline__ uint32_t win16_staged(const uint32_t *__restrict__ words, const Staged &t, int32_t p) {
   const int32_t q = t.r0 + p;
   const int32_t wi = q >> 4;
-  const uint32_t sh = (uint32_t)(q & 15) * 2u;
+  const uint32_t sh = (uint32_t)q * 2u;   // alignbit shifts by this modulo 32: 2 (q mod 16)
   if (wi + 1 < SW) return __builtin_amdgcn_alignbit(t.lds[wi + 1], t.lds[wi], sh);
   return __builtin_amdgcn_alignbit(words[t.w0 + wi + 1], words[t.w0 + wi], sh);
 }
@@ -464,15 +464,22 @@ struct SMRes {
   uint32_t cells, rows, slots, wrows, bases, siters, wsiters, wchunks;
 };
 
+// N bases match any base (prefixEditDistance-forward.C:155-167 compares bytes, and an 'n' there is
+// treated as equal): the two XOR bits of every N base in a 16-base window, cleared by the caller
+__device__ __forceinline__ uint32_t nbits16(const uint32_t *__restrict__ nmask, int64_t p) {
+  const uint32_t n = nwin16(nmask, p);
+  return n | (n << 1);
+}
+
 // 16 bases of both tails from positions pa (in a) and pb (in b), compared: the number of leading
 // equal bases (16 if all are)
 template <int SW = OIC_SEQ_WIN>
 __device__ __forceinline__ int32_t sm_match16(const SMDP &s, const uint32_t *__restrict__ words,
                                               const uint32_t *__restrict__ nmask, int32_t pa, int32_t pb) {
+  // the XOR's lowest set bit lies in the first differing base (two bits a base)
   uint32_t x = win16_staged<SW>(words, s.ta, pa) ^ win16_staged<SW>(words, s.tb, pb);
-  x = (x | (x >> 1)) & 0x55555555u;
-  if (s.nA) x &= ~nwin16(nmask, s.a_b0 + pa);
-  if (s.nB) x &= ~nwin16(nmask, s.b_b0 + pb);
+  if (s.nA) x &= ~nbits16(nmask, s.a_b0 + pa);
+  if (s.nB) x &= ~nbits16(nmask, s.b_b0 + pb);
   return x ? (int32_t)(__builtin_ctz(x) >> 1) : 16;
 }
 
@@ -514,14 +521,13 @@ __device__ __forceinline__ int32_t sm_match32(const SMDP &s, const uint32_t *__r
   uint32_t a0, a1, a2, b0, b1, b2;
   words3_staged<SW>(words, s.ta, qa >> 4, a0, a1, a2);
   words3_staged<SW>(words, s.tb, qb >> 4, b0, b1, b2);
-  const uint32_t sa = (uint32_t)(qa & 15) * 2u, sb = (uint32_t)(qb & 15) * 2u;
+  // alignbit shifts by its operand modulo 32: 2q is 2 (q mod 16) there
+  const uint32_t sa = (uint32_t)qa * 2u, sb = (uint32_t)qb * 2u;
   uint32_t x0 = __builtin_amdgcn_alignbit(a1, a0, sa) ^ __builtin_amdgcn_alignbit(b1, b0, sb);
   uint32_t x1 = __builtin_amdgcn_alignbit(a2, a1, sa) ^ __builtin_amdgcn_alignbit(b2, b1, sb);
-  x0 = (x0 | (x0 >> 1)) & 0x55555555u;
-  x1 = (x1 | (x1 >> 1)) & 0x55555555u;
-  if (s.nA) { x0 &= ~nwin16(nmask, s.a_b0 + pa); x1 &= ~nwin16(nmask, s.a_b0 + pa + 16); }
-  if (s.nB) { x0 &= ~nwin16(nmask, s.b_b0 + pb); x1 &= ~nwin16(nmask, s.b_b0 + pb + 16); }
-  // the first mismatch bit of x1:x0 (two bits a base), 64 when there is none
+  if (s.nA) { x0 &= ~nbits16(nmask, s.a_b0 + pa); x1 &= ~nbits16(nmask, s.a_b0 + pa + 16); }
+  if (s.nB) { x0 &= ~nbits16(nmask, s.b_b0 + pb); x1 &= ~nbits16(nmask, s.b_b0 + pb + 16); }
+  // the lowest set bit of x1:x0 lies in the first differing base (two bits a base); 64 when none
   return (int32_t)(__builtin_ctzg(((uint64_t)x1 << 32) | x0, 64) >> 1);
 }
 
@@ -534,13 +540,12 @@ __device__ __forceinline__ int32_t sm_match64(const SMDP &s, const uint32_t *__r
   uint32_t a[5], b[5];
   words5_staged<SW>(words, s.ta, qa >> 4, a);
   words5_staged<SW>(words, s.tb, qb >> 4, b);
-  const uint32_t sa = (uint32_t)(qa & 15) * 2u, sb = (uint32_t)(qb & 15) * 2u;
+  const uint32_t sa = (uint32_t)qa * 2u, sb = (uint32_t)qb * 2u;   // alignbit takes them modulo 32
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     uint32_t x = __builtin_amdgcn_alignbit(a[k + 1], a[k], sa) ^ __builtin_amdgcn_alignbit(b[k + 1], b[k], sb);
-    x = (x | (x >> 1)) & 0x55555555u;
-    if (s.nA) x &= ~nwin16(nmask, s.a_b0 + pa + 16 * k);
-    if (s.nB) x &= ~nwin16(nmask, s.b_b0 + pb + 16 * k);
+    if (s.nA) x &= ~nbits16(nmask, s.a_b0 + pa + 16 * k);
+    if (s.nB) x &= ~nbits16(nmask, s.b_b0 + pb + 16 * k);
     if (x) return 16 * k + (int32_t)(__builtin_ctz(x) >> 1);
   }
   return 64;

@@ -38,8 +38,10 @@ template <int G, int CAP>
 struct FlatLDS {
   static constexpr int NG = OIC_WAVE / G;
   static constexpr int SW = flat_sw<G>();
+  // a DP's two staged tails side by side (A, then T; SW + 1 words each): one base register
+  // addresses both within the 8-bit dword offsets of paired LDS reads
+  uint32_t seq[NG][2][SW + 1];
   int32_t tile[NG][CAP + 3];
-  uint32_t seqA[NG][SW + 1], seqT[NG][SW + 1];
   FlatDesc desc[NG];
   unsigned long long pkey[NG];   // the previous row's first maximum: (Row + 1) << 32 | (2^30 - d)
   int4 red[NG];                  // first kept, last kept and first reaching diagonal, -
@@ -158,10 +160,10 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
     SMDP c;
     c.ta.w0 = D.w0a;
     c.ta.r0 = D.r0a;
-    c.ta.lds = (lds_words)L.seqA[dg];
+    c.ta.lds = (lds_words)L.seq[dg][0];
     c.tb.w0 = D.w0b;
     c.tb.r0 = D.r0b;
-    c.tb.lds = (lds_words)L.seqT[dg];
+    c.tb.lds = (lds_words)L.seq[dg][1];
     c.a_b0 = D.w0a * 16 + D.r0a;
     c.b_b0 = D.w0b * 16 + D.r0b;
     c.nA = (D.nab & 1) != 0;
@@ -328,7 +330,7 @@ k_fwd_flat(FwdArgs A) {
     s.nB = s_first ? tN : sN;
     s.n = s_first ? T_Left_Begin + 1 : S_Left_Begin + 1;
     s.lim = limit - right_e;
-    if (!sm_start<G, SW>(s, res, A.R.words, A.R.nmask, L.tile[g], L.seqA[g], L.seqT[g], g, gl)) {
+    if (!sm_start<G, SW>(s, res, A.R.words, A.R.nmask, L.tile[g], L.seq[g][0], L.seq[g][1], g, gl)) {
       flat_desc_start<G, CAP>(L, s, g, gl);
       return true;
     }
@@ -397,7 +399,7 @@ k_fwd_flat(FwdArgs A) {
       s.nB = s_short ? tN : sN;
       s.n = s_short ? T_Right_Len : S_Right_Len;
       s.lim = limit;
-      if (!sm_start<G, SW>(s, res, A.R.words, A.R.nmask, L.tile[g], L.seqA[g], L.seqT[g], g, gl)) {
+      if (!sm_start<G, SW>(s, res, A.R.words, A.R.nmask, L.tile[g], L.seq[g][0], L.seq[g][1], g, gl)) {
         flat_desc_start<G, CAP>(L, s, g, gl);
         state = 1;
         break;
