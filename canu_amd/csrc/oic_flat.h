@@ -16,7 +16,9 @@
 // in a chunk every lane reads its three cells of the previous row before any lane writes, and the
 // one cell a run's first lane needs from the chunk before (its left neighbour, already
 // overwritten) comes from the lane that read it there, by a shuffle.  With 4 lanes per DP the
-// first 512 bases of each tail are staged (SW = 32 words), else 1024.
+// first 128 bases of each tail are staged (SW = 8 words), else 1024: longer windows measured slower
+// on both the repeat-rich and the low-error reads, their copy at every DP start costing more than
+// the cached reads past them.
 
 #pragma once
 
@@ -29,7 +31,7 @@ struct FlatDesc {
 };
 
 template <int G>
-constexpr int flat_sw() { return G == 4 ? 32 : OIC_SEQ_WIN; }
+constexpr int flat_sw() { return G == 4 ? 8 : OIC_SEQ_WIN; }
 
 // Each DP's band buffer holds its CAP ring cells, then copies of cells 0 and 1 (so the three cells
 // d - 1, d, d + 1 are three consecutive words from (d - 1) mod CAP), then one spare word: strides of
@@ -196,8 +198,7 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
         int32_t Row = 1 + p1;
         Row = Row > x2 ? Row : x2;
         Row = Row > x3 ? Row : x3;
-        // slide along diagonal d (prefixEditDistance-forward.C:155-167): a 32-base window, then
-        // 64-base windows
+        // slide along diagonal d (prefixEditDistance-forward.C:155-167), in 32-base windows
         const int32_t r1 = c.m - Row, r2 = c.n - Row - d;
         const int32_t rem0 = r1 < r2 ? r1 : r2;
         const int32_t kk = sm_match32<SW>(c, words, nmask, Row, Row + d);
