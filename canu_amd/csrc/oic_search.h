@@ -673,6 +673,31 @@ __global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
         wb = h;
         fill();
       }
+      if (nA == 1) {
+        // The next offsets, each holding one hit on the live node's diagonal, only extend that node
+        // (the one-hit step below, with nothing dying and nothing created): take up to 64 at once.
+        // Hit h + lane and h + lane + 1 lie in the window ([h, h + 64) is held, the window is 128).
+        const uint64_t hh = h + (uint64_t)lane;
+        bool ext = false;
+        uint64_t v = 0;
+        if (hh < he) {
+          v = sW[hh - wb];
+          const int32_t oi = (int32_t)(v & 0x1fffff);
+          const int32_t gi = (int32_t)((v >> 21) & 0x1fffff) - oi;
+          const bool single = hh + 1 >= he || (int32_t)(sW[hh + 1 - wb] & 0x1fffff) != oi;
+          ext = oi == prev_o + 1 + lane && gi == sLd[cur][0] && single;
+        }
+        const uint64_t b = __ballot(ext);
+        const int32_t r = (b == ~0ull) ? 64 : __builtin_ctzll(~b);
+        if (r > 0) {
+          if (lane < r) maxm = max(maxm, (int32_t)(v >> 42));
+          if (lane == 0) sR[cur][0].y += r;
+          __syncthreads();
+          h += (uint64_t)r;
+          prev_o += r;
+          continue;
+        }
+      }
       const int32_t o = (int32_t)(sW[h - wb] & 0x1fffff);
       int32_t c = 0;
       for (;;) {
