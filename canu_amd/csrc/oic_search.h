@@ -625,10 +625,10 @@ __global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
   __shared__ int32_t sLd[2][CAP];   // live diagonals in hit order of the last offset (descending)
   __shared__ int4 sR[2][CAP];       // live nodes: Offset, Len, Start, list position
   __shared__ int32_t sLq[2][CAP];   // list position -> live index
-  __shared__ int32_t sQ[CAP];       // per hit: live index of its node, or -1
   __shared__ int32_t sT[CAP];       // per list position: the hit touching it, or INF
   __shared__ int32_t sM[CAP];       // exclusive prefix max of sT
-  __shared__ int32_t sRk[CAP];      // per hit: 2 x rank among created / moved hits, +1 if it is one
+  __shared__ int32_t sQ[CAP > LW_CAP_FIRST ? CAP : 1];    // per hit: live index of its node, or -1 (512-hit level)
+  __shared__ int32_t sRk[CAP > LW_CAP_FIRST ? CAP : 1];   // per hit: 2 x rank among created / moved hits, +1 if it is one
   __shared__ int32_t sIp[CAP];      // per list position: rank among nodes extended in place
   __shared__ uint64_t sW[128];      // hit values [wb, wb + 128) of the pair, refilled every 64 hits
   __shared__ uint32_t sWork;
@@ -770,103 +770,226 @@ __global__ void __launch_bounds__(64) k_build_lists_wave(BigListArgs A) {
         cur ^= 1;
         continue;
       }
-      // hits -> live nodes
-      for (int32_t j = lane; j < c; j += 64) {
-        const uint64_t v = hit(h + j);
-        const int32_t g = (int32_t)((v >> 21) & 0x1fffff) - o;
-        maxm = max(maxm, (int32_t)(v >> 42));
-        int32_t lo = 0, hi = nA;
-        while (lo < hi) {
-          const int32_t mid = (lo + hi) >> 1;
-          if (Ld[mid] > g) lo = mid + 1;
-          else hi = mid;
+      int32_t ncre = 0;
+      if constexpr (CAP <= LW_CAP_FIRST) {
+        // hits -> live nodes.  Each lane keeps, for the hits it holds (j = lane + 64 t), the live
+        // index of the hit's node (-1: none), that node's list position, the hit's diagonal and
+        // its rank among created / moved hits, in registers.
+        constexpr int NJ = CAP / 64;
+        int32_t qr[NJ], wr[NJ], gr[NJ], rkr[NJ];
+  #pragma unroll
+        for (int t = 0; t < NJ; t++) {
+          const int32_t j = lane + 64 * t;
+          qr[t] = -1; wr[t] = -1; gr[t] = 0; rkr[t] = 0;
+          if (64 * t < c && j < c) {   // the first test is uniform: rounds past c are skipped whole
+            const uint64_t v = hit(h + j);
+            const int32_t g = (int32_t)((v >> 21) & 0x1fffff) - o;
+            maxm = max(maxm, (int32_t)(v >> 42));
+            int32_t lo = 0, hi = nA;
+            while (lo < hi) {
+              const int32_t mid = (lo + hi) >> 1;
+              if (Ld[mid] > g) lo = mid + 1;
+              else hi = mid;
+            }
+            qr[t] = (lo < nA && Ld[lo] == g) ? lo : -1;
+            gr[t] = g;
+            Ld2[j] = g;
+          }
         }
-        sQ[j] = (lo < nA && Ld[lo] == g) ? lo : -1;
-        Ld2[j] = g;
-      }
-      for (int32_t k = lane; k < nA; k += 64) sT[k] = INF;
-      __syncthreads();
-      for (int32_t j = lane; j < c; j += 64) {
-        const int32_t q = sQ[j];
-        if (q >= 0) sT[R[q].w] = j;
-      }
-      __syncthreads();
-      {
-        int32_t carry = -1;
-        for (int32_t k0 = 0; k0 < nA; k0 += 64) {
-          const int32_t k = k0 + lane;
-          const int32_t t = (k < nA) ? sT[k] : -1;
-          const int32_t inc = wave_incl_max_i32(t);
-          const int32_t up = __shfl_up(inc, 1);
-          if (k < nA) sM[k] = (lane == 0) ? carry : max(up, carry);
-          carry = max(carry, __shfl(inc, 63));
+        for (int32_t k = lane; k < nA; k += 64) sT[k] = INF;
+        __syncthreads();
+  #pragma unroll
+        for (int t = 0; t < NJ; t++) {
+          const int32_t j = lane + 64 * t;
+          if (64 * t < c && j < c && qr[t] >= 0) {
+            wr[t] = R[qr[t]].w;
+            sT[wr[t]] = j;
+          }
         }
-      }
-      __syncthreads();
-      // created / moved hits
-      int32_t nfr = 0, ncre = 0;
-      bool bad = false;
-      for (int32_t j0 = 0; j0 < c; j0 += 64) {
-        const int32_t j = j0 + lane;
-        bool cre = false, cf = false;
-        if (j < c) {
+        __syncthreads();
+        {
+          int32_t carry = -1;
+          for (int32_t k0 = 0; k0 < nA; k0 += 64) {
+            const int32_t k = k0 + lane;
+            const int32_t t = (k < nA) ? sT[k] : -1;
+            const int32_t inc = wave_incl_max_i32(t);
+            const int32_t up = __shfl_up(inc, 1);
+            if (k < nA) sM[k] = (lane == 0) ? carry : max(up, carry);
+            carry = max(carry, __shfl(inc, 63));
+          }
+        }
+        __syncthreads();
+        // created / moved hits
+        int32_t nfr = 0;
+        bool bad = false;
+  #pragma unroll
+        for (int t = 0; t < NJ; t++) {
+          if (64 * t < c) {   // uniform
+            const int32_t j = lane + 64 * t;
+            bool cre = false, cf = false;
+            if (j < c) {
+              cre = qr[t] < 0;
+              cf = cre || sM[wr[t]] > j;
+            }
+            const uint64_t bcf = __ballot(cf), bcre = __ballot(cre);
+            rkr[t] = (nfr + (int32_t)__popcll(bcf & lt)) * 2 + (cf ? 1 : 0);
+            if (cre && (j > 0 || nA > 0)) bad = true;
+            nfr += (int32_t)__popcll(bcf);
+            ncre += (int32_t)__popcll(bcre);
+          }
+        }
+        if (nA == 0 && nnode > 0) {                    // first hit makes a node with no live node
+          const int32_t es = head.z + head.y - K + 1, dg = head.x - head.z;
+          if (abs(dg - Ld2[0]) > 3 || o < es + K - 2) bad = true;
+        }
+        if (__ballot(bad)) consistent = false;
+        // list positions: extended in place, dead
+        const int32_t sz = nA - (c - ncre);
+        {
+          int32_t ni = 0, nd = 0;
+          for (int32_t k0 = 0; k0 < nA; k0 += 64) {
+            const int32_t k = k0 + lane;
+            bool inpl = false, dead = false;
+            if (k < nA) {
+              const int32_t t = sT[k];
+              dead = t == INF;
+              inpl = !dead && sM[k] < t;
+            }
+            const uint64_t bi = __ballot(inpl), bd = __ballot(dead);
+            if (k < nA) sIp[k] = ni + (int32_t)__popcll(bi & lt);
+            if (dead) {
+              const int32_t r = nd + (int32_t)__popcll(bd & lt);
+              int4 rec = R[Lq[k]];
+              rec.w = ndq + sz - r;
+              dq[ndq + r] = rec;
+            }
+            ni += (int32_t)__popcll(bi);
+            nd += (int32_t)__popcll(bd);
+          }
+          ndq += sz;
+        }
+        __syncthreads();
+        // the new live set, in hit order, with list positions
+  #pragma unroll
+        for (int t = 0; t < NJ; t++) {
+          const int32_t j = lane + 64 * t;
+          if (64 * t < c && j < c) {
+            const int32_t q = qr[t], rk = rkr[t];
+            int4 rec;
+            int32_t P;
+            if (q < 0) {
+              rec = make_int4(gr[t] + o, K, o, 0);
+              P = nfr - 1 - (rk >> 1);
+            } else {
+              rec = R[q];
+              rec.y += 1;
+              P = (rk & 1) ? nfr - 1 - (rk >> 1) : nfr + sIp[wr[t]];
+            }
+            rec.w = P;
+            R2[j] = rec;
+            Lq2[P] = j;
+          }
+        }
+      } else {
+        // the 512-hit level: per-hit values in LDS (eight rounds of registers measured slower there)
+        // hits -> live nodes
+        for (int32_t j = lane; j < c; j += 64) {
+          const uint64_t v = hit(h + j);
+          const int32_t g = (int32_t)((v >> 21) & 0x1fffff) - o;
+          maxm = max(maxm, (int32_t)(v >> 42));
+          int32_t lo = 0, hi = nA;
+          while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (Ld[mid] > g) lo = mid + 1;
+            else hi = mid;
+          }
+          sQ[j] = (lo < nA && Ld[lo] == g) ? lo : -1;
+          Ld2[j] = g;
+        }
+        for (int32_t k = lane; k < nA; k += 64) sT[k] = INF;
+        __syncthreads();
+        for (int32_t j = lane; j < c; j += 64) {
           const int32_t q = sQ[j];
-          cre = q < 0;
-          cf = cre || sM[R[q].w] > j;
+          if (q >= 0) sT[R[q].w] = j;
         }
-        const uint64_t bcf = __ballot(cf), bcre = __ballot(cre);
-        if (j < c) sRk[j] = (nfr + (int32_t)__popcll(bcf & lt)) * 2 + (cf ? 1 : 0);
-        if (cre && (j > 0 || nA > 0)) bad = true;
-        nfr += (int32_t)__popcll(bcf);
-        ncre += (int32_t)__popcll(bcre);
-      }
-      if (nA == 0 && nnode > 0) {                    // first hit makes a node with no live node
-        const int32_t es = head.z + head.y - K + 1, dg = head.x - head.z;
-        if (abs(dg - Ld2[0]) > 3 || o < es + K - 2) bad = true;
-      }
-      if (__ballot(bad)) consistent = false;
-      // list positions: extended in place, dead
-      const int32_t sz = nA - (c - ncre);
-      {
-        int32_t ni = 0, nd = 0;
-        for (int32_t k0 = 0; k0 < nA; k0 += 64) {
-          const int32_t k = k0 + lane;
-          bool inpl = false, dead = false;
-          if (k < nA) {
-            const int32_t t = sT[k];
-            dead = t == INF;
-            inpl = !dead && sM[k] < t;
+        __syncthreads();
+        {
+          int32_t carry = -1;
+          for (int32_t k0 = 0; k0 < nA; k0 += 64) {
+            const int32_t k = k0 + lane;
+            const int32_t t = (k < nA) ? sT[k] : -1;
+            const int32_t inc = wave_incl_max_i32(t);
+            const int32_t up = __shfl_up(inc, 1);
+            if (k < nA) sM[k] = (lane == 0) ? carry : max(up, carry);
+            carry = max(carry, __shfl(inc, 63));
           }
-          const uint64_t bi = __ballot(inpl), bd = __ballot(dead);
-          if (k < nA) sIp[k] = ni + (int32_t)__popcll(bi & lt);
-          if (dead) {
-            const int32_t r = nd + (int32_t)__popcll(bd & lt);
-            int4 rec = R[Lq[k]];
-            rec.w = ndq + sz - r;
-            dq[ndq + r] = rec;
+        }
+        __syncthreads();
+        // created / moved hits
+        int32_t nfr = 0;
+        bool bad = false;
+        for (int32_t j0 = 0; j0 < c; j0 += 64) {
+          const int32_t j = j0 + lane;
+          bool cre = false, cf = false;
+          if (j < c) {
+            const int32_t q = sQ[j];
+            cre = q < 0;
+            cf = cre || sM[R[q].w] > j;
           }
-          ni += (int32_t)__popcll(bi);
-          nd += (int32_t)__popcll(bd);
+          const uint64_t bcf = __ballot(cf), bcre = __ballot(cre);
+          if (j < c) sRk[j] = (nfr + (int32_t)__popcll(bcf & lt)) * 2 + (cf ? 1 : 0);
+          if (cre && (j > 0 || nA > 0)) bad = true;
+          nfr += (int32_t)__popcll(bcf);
+          ncre += (int32_t)__popcll(bcre);
         }
-        ndq += sz;
-      }
-      __syncthreads();
-      // the new live set, in hit order, with list positions
-      for (int32_t j = lane; j < c; j += 64) {
-        const int32_t q = sQ[j], rk = sRk[j];
-        int4 rec;
-        int32_t P;
-        if (q < 0) {
-          rec = make_int4(Ld2[j] + o, K, o, 0);
-          P = nfr - 1 - (rk >> 1);
-        } else {
-          rec = R[q];
-          rec.y += 1;
-          P = (rk & 1) ? nfr - 1 - (rk >> 1) : nfr + sIp[rec.w];
+        if (nA == 0 && nnode > 0) {                    // first hit makes a node with no live node
+          const int32_t es = head.z + head.y - K + 1, dg = head.x - head.z;
+          if (abs(dg - Ld2[0]) > 3 || o < es + K - 2) bad = true;
         }
-        rec.w = P;
-        R2[j] = rec;
-        Lq2[P] = j;
+        if (__ballot(bad)) consistent = false;
+        // list positions: extended in place, dead
+        const int32_t sz = nA - (c - ncre);
+        {
+          int32_t ni = 0, nd = 0;
+          for (int32_t k0 = 0; k0 < nA; k0 += 64) {
+            const int32_t k = k0 + lane;
+            bool inpl = false, dead = false;
+            if (k < nA) {
+              const int32_t t = sT[k];
+              dead = t == INF;
+              inpl = !dead && sM[k] < t;
+            }
+            const uint64_t bi = __ballot(inpl), bd = __ballot(dead);
+            if (k < nA) sIp[k] = ni + (int32_t)__popcll(bi & lt);
+            if (dead) {
+              const int32_t r = nd + (int32_t)__popcll(bd & lt);
+              int4 rec = R[Lq[k]];
+              rec.w = ndq + sz - r;
+              dq[ndq + r] = rec;
+            }
+            ni += (int32_t)__popcll(bi);
+            nd += (int32_t)__popcll(bd);
+          }
+          ndq += sz;
+        }
+        __syncthreads();
+        // the new live set, in hit order, with list positions
+        for (int32_t j = lane; j < c; j += 64) {
+          const int32_t q = sQ[j], rk = sRk[j];
+          int4 rec;
+          int32_t P;
+          if (q < 0) {
+            rec = make_int4(Ld2[j] + o, K, o, 0);
+            P = nfr - 1 - (rk >> 1);
+          } else {
+            rec = R[q];
+            rec.y += 1;
+            P = (rk & 1) ? nfr - 1 - (rk >> 1) : nfr + sIp[rec.w];
+          }
+          rec.w = P;
+          R2[j] = rec;
+          Lq2[P] = j;
+        }
       }
       __syncthreads();
       nnode += ncre;
