@@ -1388,6 +1388,19 @@ int32_t oo_distinct_olaps(int partial, int unique, int32_t n, const int32_t *box
   return ct;
 }
 
+/* Test hook: Lies_On_Alignment (Process_String_Overlaps.C:260-281) for the match node at S position
+ * start and T position offset, against the alignment from (s_lo, t_lo) with Left_Delta = delta[0..n). */
+int32_t oo_lies_on_alignment(int32_t start, int32_t offset, int32_t s_lo, int32_t t_lo, const int32_t *delta, int32_t n) {
+  struct oo_dp ed;
+  memset(&ed, 0, sizeof ed);
+  ed.Left_Delta = (int32_t *)delta;
+  ed.Left_Delta_Len = n;
+  work_area WA;
+  memset(&WA, 0, sizeof WA);
+  WA.ed = &ed;
+  return Lies_On_Alignment(start, offset, s_lo, t_lo, &WA);
+}
+
 /* ovOverlap.H:31-35 */
 uint32_t oo_encode_evalue(double q) {
   return (q < (65535 / 100000.0)) ? (uint32_t)(100000.0 * q + 0.5) : 65535u;

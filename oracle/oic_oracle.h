@@ -148,6 +148,8 @@ int32_t oo_match_list(uint32_t kmer, const int32_t *offset, const int32_t *toff,
  * rows of 11 ints per distinct overlap (box, min/max diag, boundaries, deleted). */
 int32_t oo_distinct_olaps(int partial, int unique, int32_t n, const int32_t *box, const double *qual,
                           int32_t *out, double *outq);
+/* Lies_On_Alignment (Process_String_Overlaps.C:260-281) against an explicit Left_Delta. */
+int32_t oo_lies_on_alignment(int32_t start, int32_t offset, int32_t s_lo, int32_t t_lo, const int32_t *delta, int32_t n);
 
 #ifdef __cplusplus
 }

@@ -114,6 +114,8 @@ def lib():
         L.oo_encode_evalue.argtypes = [ctypes.c_double]
         L.oo_match_list.restype = ctypes.c_int32
         L.oo_match_list.argtypes = [ctypes.c_uint32, _P32, _P32, ctypes.c_int32, _P32, ctypes.c_int32, _P32]
+        L.oo_lies_on_alignment.restype = ctypes.c_int32
+        L.oo_lies_on_alignment.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P32, ctypes.c_int32]
         L.oo_distinct_olaps.restype = ctypes.c_int32
         L.oo_distinct_olaps.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int32, _P32,
                                         ctypes.POINTER(ctypes.c_double), _P32, ctypes.POINTER(ctypes.c_double)]
@@ -150,6 +152,13 @@ def distinct_olaps(boxes, quals, partial=False, unique=False):
     k = lib().oo_distinct_olaps(int(partial), int(unique), len(bx), bx.ctypes.data_as(_P32), q.ctypes.data_as(pd),
                                 out.ctypes.data_as(_P32), oq.ctypes.data_as(pd))
     return out[:k].copy(), oq[:k].copy()
+
+
+def lies_on_alignment(start, offset, s_lo, t_lo, delta):
+    """Lies_On_Alignment: whether the match node at (start, offset) lies on the alignment from
+    (s_lo, t_lo) whose Left_Delta is delta."""
+    d = np.ascontiguousarray(delta if len(delta) else [0], np.int32)
+    return bool(lib().oo_lies_on_alignment(start, offset, s_lo, t_lo, d.ctypes.data_as(_P32), len(delta)))
 
 
 def cli_rate(x):
