@@ -1,7 +1,7 @@
 /*
  * oic_oracle.c -- CPU restatement of canu's overlapInCore hot path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker, and the timed CPU baseline "port").
+ * TEST INFRASTRUCTURE ONLY (the tests' oracle, and the timed CPU baseline "port").
  * Nothing under canu_amd/ links, loads or calls this file.
  *
  * Restated from (paths relative to canu's tree):
@@ -886,7 +886,7 @@ static void occ_push(occ_run *r, uint64_t slot, uint64_t occ) {
   r->v[r->n++] = occ;
 }
 
-/* Checker-side shortcut for small ref ranges against large hash ranges (no reference counterpart).
+/* Oracle-side shortcut for small ref ranges against large hash ranges (no reference counterpart).
  * Find_Overlaps looks up only the k-mers of the ref reads it searches (both orientations) and
  * Mark_Skip_Kmers only the skip k-mers (both orientations), so an index holding just those keys,
  * each with its complete occurrence list in the order build_index gives it, yields the same

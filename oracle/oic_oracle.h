@@ -1,7 +1,7 @@
 /*
  * oic_oracle.h -- CPU restatement of canu's overlapInCore hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  This library is the tests' oracle: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
  * product path (canu_amd/, liboic_hip.so) never links or calls it.
  *
@@ -94,10 +94,10 @@ typedef struct oo_params {
   const char *skipKmers;       /* -k <file>: nSkip k-mers of kmer ACGT letters each, back to back, in
                                   file order (parsed as Mark_Skip_Kmers does); NULL = no file */
   uint64_t  nSkip;
-  int32_t   restrictIndex;     /* checker shortcut, no reference counterpart: index only the k-mers the
+  int32_t   restrictIndex;     /* oracle shortcut, no reference counterpart: index only the k-mers the
                                   searched ref reads and the skip k-mers hold (same search; the hash range
                                   must fit one block, else oo_run returns -2) */
-  const uint32_t *refList;     /* checker shortcut: when nRefList > 0, search exactly these ref IDs (in
+  const uint32_t *refList;     /* oracle shortcut: when nRefList > 0, search exactly these ref IDs (in
                                   place of bgnRef..endRef and the -t chunk rule) */
   uint64_t  nRefList;
 } oo_params;

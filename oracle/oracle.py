@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (liboic_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg, and only as the checker.
+cpu_baseline leg, and only as the reference results are compared with.
 """
 import ctypes
 import os

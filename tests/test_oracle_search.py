@@ -252,7 +252,7 @@ def _same_run(a, b):
 
 @pytest.mark.parametrize("case", ["c3_mid_refs", "c3_low_refs_skip", "c5_partial", "c2_no_hopeless_skip"])
 def test_restricted_index_searches_like_the_full_index(case):
-    """The checker's restricted index (only the k-mers of the searched ref reads and the skip
+    """The oracle's restricted index (only the k-mers of the searched ref reads and the skip
     k-mers) gives the full index's records, .stats and work counters."""
     preset, glen, seed, kw = {
         "c3_mid_refs": ("c3", 400_000, 41, dict(bgnRef=300, endRef=340)),
@@ -315,7 +315,7 @@ def test_record_orientation_and_hangs_match_truth():
 
 
 def test_ref_list_searches_exactly_the_listed_reads():
-    """The checker's explicit ref list (restricted index) gives the union of single-read searches
+    """The oracle's explicit ref list (restricted index) gives the union of single-read searches
     (a range [r, r + 1] under -t 1 searches read r only: the chunk rule skips r + 1)."""
     R = readsim.simulate(readsim.PRESETS["c3"].scaled(300_000, seed=47))
     refs = [5, 9, 17, 40]
