@@ -124,7 +124,7 @@ __device__ __forceinline__ bool flat_row(FlatLDS<G, CAP> &L, bool rowg, SMDP &s,
   L.start[lane] = -1;
   // lanes per DP: few chunk iterations C with sum over DPs of ceil(w / C) <= 64.  With A active
   // DPs that sum is below W / C + A, so C = ceil(W / (64 - A)) always fits: try ceil(W / 64) and
-  // the next value first (the fewest in about 95 % of rows even of 16 random bands up to 125 wide).
+  // the next value first (the fewest in about 95 % of rows of up to 16 random bands up to 125 wide).
   const int32_t w = (rowg && !over && gl == 0) ? s.Right - s.Left + 1 : 0;
   const int32_t W = __builtin_amdgcn_readlane(wave_incl_scan(w), OIC_WAVE - 1);
   const int32_t A = __builtin_popcountll(__ballot(w > 0));
